@@ -1,0 +1,136 @@
+/* sdbscan_amd.h -- C ABI of the MI355X (gfx950) sDbscan neighbourhood-discovery path.
+ *
+ * The reference (NinhPham/sDbscan) runs this path inside the C++ class sDbscan
+ * (src/sDbscan.h:10-223).  Its device-free stages are replaced one to one:
+ *
+ *   sdb_set_params     <- sDbscan::set_params / set_kernel_params (src/sDbscan.h:79-124)
+ *   sdb_load_host      <- the copy matrix_X = X of fit_sDbscan (src/sDbscan.cpp:857)
+ *   sdb_build_index    <- transformData (src/Utilities.h:289-335) + rp_parIndex
+ *                         (src/sDbscan.cpp:19-209); sign bits and RFF matrix come from
+ *                         the host RNG calls of src/Utilities.h:47-122
+ *   sdb_find_core      <- rp_findCorePoints (src/sDbscan.cpp:217-408)
+ *   sdb_form_clusters  <- formCluster (src/sDbscan.cpp:509-637), labelNoise excluded
+ *   sdb_core_dist      <- rp_findCoreDist (src/sDbscan.cpp:1314-1474)
+ *   sdb_optics_reach   <- the reachability values of formOptics_scikit
+ *                         (src/sDbscan.cpp:1588-1703) for a given ordering
+ *   sdb_get_index      <- the protected matrix_topK / matrix_topM (src/sDbscan.h:54-55)
+ *
+ * All entry points return 0 on success and a nonzero SDB_E* code otherwise;
+ * sdb_last_error() describes the last failure.  No exception or exit() crosses
+ * this boundary.  Matrices use the reference's layouts: X is d x n column-major
+ * (n rows of d floats), topK is 2K x n column-major, topM is 2M x n_proj
+ * column-major.  A context is not thread safe (like the reference object). */
+#ifndef SDBSCAN_AMD_H
+#define SDBSCAN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sdb_metric { SDB_COSINE = 0, SDB_L1 = 1, SDB_L2 = 2, SDB_CHI2 = 3, SDB_JS = 4 };
+enum sdb_precision { SDB_FP32 = 0, SDB_BF16 = 1 };
+enum sdb_status {
+    SDB_OK = 0,
+    SDB_EINVAL = 1,     /* bad argument or parameter combination */
+    SDB_EHIP = 2,       /* HIP runtime error */
+    SDB_ENOMEM = 3,     /* device allocation failed */
+    SDB_ENEGATIVE = 4,  /* Chi2/JS input with a negative entry (src/Utilities.h:308-312) */
+    SDB_EZERO = 5,      /* zero point: zero sum (src/Utilities.h:320-324) or zero norm */
+    SDB_ESTATE = 6,     /* call order violated (e.g. find_core before build_index) */
+    SDB_EUNSUPPORTED = 7
+};
+
+typedef struct sdb_ctx sdb_ctx;
+
+typedef struct sdb_params {
+    long long n_points;
+    int n_features;
+    int n_proj;            /* numProj */
+    int topK;              /* k */
+    int topM;              /* m */
+    int metric;            /* enum sdb_metric */
+    int ker_n_features;    /* kerDim (Cosine: n_features) */
+    float ker_sigma;
+    float ker_intervalSampling;
+    int fhtDim;            /* 2^ceil(log2(max(n_proj, ker_n_features))) */
+    int precision;         /* enum sdb_precision: distance tiles */
+} sdb_params;
+
+typedef struct sdb_stats {
+    double t_transform_ms, t_index_ms, t_core_ms, t_cluster_ms;  /* phase times (HIP events) */
+    double t_dist_ms;             /* distance kernel, HIP events on the context stream */
+    long long n_pairs;            /* (point, candidate) slots evaluated by the tiles */
+    long long n_queue;            /* pairs inside the error band of eps */
+    long long n_recheck32, n_recheck64, n_recheck_hits;
+    long long n_hubs;             /* distinct points over all top-M lists */
+    long long n_tie_points;       /* points whose top-K was replayed on the host */
+    long long n_tie_rows;         /* top-M rows replayed on the host */
+    long long n_core;
+    int n_components;
+    double band;                  /* error band of the fast distance */
+    double band32;                /* error band of the FP32 recheck */
+} sdb_stats;
+
+/* Context lifecycle (one HIP stream per context). */
+int sdb_create(int device, sdb_ctx** out);
+void sdb_destroy(sdb_ctx* ctx);
+const char* sdb_last_error(const sdb_ctx* ctx);
+
+int sdb_set_params(sdb_ctx* ctx, const sdb_params* p);
+
+/* Input points: host buffer (copied) or device buffer (borrowed; must stay
+ * valid until the next load or destroy).  X is n rows of d floats. */
+int sdb_load_host(sdb_ctx* ctx, const float* X, long long n, int d);
+int sdb_load_device(sdb_ctx* ctx, const float* X_dev, long long n, int d);
+
+/* Transform + embedding + HD3 projections + top-K + top-M.  hd3_bits holds
+ * 3 * fhtDim bytes (0/1) in the order of bitHD3Generator; R is the
+ * (ker_n_features/2) x n_features column-major RFF matrix for L1/L2 (NULL
+ * otherwise). */
+int sdb_build_index(sdb_ctx* ctx, const uint8_t* hd3_bits, const float* R);
+
+/* Neighbourhood counts and core flags for (eps, minPts); core_out (n bytes)
+ * and degree_out (n ints) are optional. */
+int sdb_find_core(sdb_ctx* ctx, float eps, int minPts, uint8_t* core_out, int* degree_out);
+
+/* Cluster labels after sdb_find_core; n_clusters as the reference reports
+ * it (components + 1, or 2 when there is none). */
+int sdb_form_clusters(sdb_ctx* ctx, int* labels_out, int* n_clusters_out);
+
+/* Index matrices in the reference layouts (either pointer may be NULL). */
+int sdb_get_index(sdb_ctx* ctx, int* topK_out, int* topM_out);
+
+/* Transformed points (n x n_features) as used by the distance tiles. */
+int sdb_get_transformed(sdb_ctx* ctx, float* X_out);
+
+/* Deduplicated neighbourhoods after sdb_find_core as CSR sorted by index:
+ * ptr has n+1 entries; idx/dist hold ptr[n] entries (dist = FP32 distance,
+ * may be NULL).  Call with idx == NULL to get ptr only. */
+int sdb_get_neighbors(sdb_ctx* ctx, long long* ptr, int* idx, float* dist);
+
+/* sOPTICS: core distances (the (minPts-1)-th smallest neighbour distance of
+ * each core point, 0 otherwise) after sdb_find_core. */
+int sdb_core_dist(sdb_ctx* ctx, int minPts, float* core_dist_out);
+
+/* Reachability of formOptics_scikit for a given ordering (n ints), computed
+ * on the device from the neighbourhoods and core distances. */
+int sdb_optics_reach(sdb_ctx* ctx, const int* ordering, float* reach_out);
+
+int sdb_get_stats(const sdb_ctx* ctx, sdb_stats* out);
+
+/* Synthetic workloads generated on the device into the context's input
+ * buffer (no host copy): kind 0 = mixture of 10 sparse nonnegative centres
+ * plus N(0, sigma^2) noise, clipped to [0,1] and L2-normalised. */
+int sdb_generate(sdb_ctx* ctx, int kind, long long n, int d, unsigned long long seed, float sigma);
+
+/* Copy rows [row0, row0 + rows) of the current input to the host. */
+int sdb_copy_input(sdb_ctx* ctx, long long row0, long long rows, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDBSCAN_AMD_H */

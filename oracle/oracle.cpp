@@ -1,0 +1,589 @@
+// oracle.cpp -- CPU restatement of sDbscan's neighbourhood-discovery path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP product path
+// (sdbscan_amd/csrc) and the CPU baseline timed by bench.py.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+// product library never links, loads or falls back to it.
+//
+// Every function restates one step of the reference (NinhPham/sDbscan) and
+// cites the file:line it follows.  Data-structure choices of the reference are
+// kept where they decide the cost profile (heaps, n-bit per-point dedup
+// bitset, a global critical section per edge push, unordered_set dedup, serial
+// BFS) so that the timed CPU baseline behaves like the reference.
+//
+// Conventions
+//   X      : n rows of d floats (row r = point r); this is the reference's
+//            column-major d x n Eigen matrix, byte for byte.
+//   proj   : P x n, row-major (row = random vector), i.e. MATRIX_FHT.
+//   topK   : n x 2K int32 (row = point; K close then K far) == matrix_topK.
+//   topM   : P x 2M int32 (row = random vector; M close then M far) == matrix_topM.
+//
+// Parity status of each function is recorded in the header comment of the
+// function and in the test that pins it.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <queue>
+#include <random>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+enum Dist { D_COSINE = 0, D_L1 = 1, D_L2 = 2, D_CHI2 = 3, D_JS = 4 };
+
+// (index, value) pair ordered by value only -- src/Header.h:48-75.
+struct VPair {
+    int idx;
+    float val;
+    VPair(int i, double v) : idx(i), val((float)v) {}
+    bool operator>(const VPair& o) const { return val > o.val; }
+    bool operator<(const VPair& o) const { return val < o.val; }
+};
+// src/Header.h:100
+typedef std::priority_queue<VPair, std::vector<VPair>, std::greater<VPair>> MinHeap;
+
+// Sum of 64 per-lane partial sums combined by an xor butterfly with offsets
+// 32,16,8,4,2,1 -- the reduction order of the HIP kernels (a fixed order is
+// needed because the reference's Eigen reduction order is not reproducible).
+double lane_tree(double part[64]) {
+    for (int off = 32; off >= 1; off >>= 1) {
+        double nxt[64];
+        for (int l = 0; l < 64; ++l) nxt[l] = part[l] + part[l ^ off];
+        std::memcpy(part, nxt, sizeof(nxt));
+    }
+    return part[0];
+}
+
+double sumsq_lanes(const float* x, int d) {
+    double part[64];
+    for (int l = 0; l < 64; ++l) {
+        double acc = 0.0;
+        for (int i = l; i < d; i += 64) acc += (double)x[i] * (double)x[i];
+        part[l] = acc;
+    }
+    return lane_tree(part);
+}
+
+double sum_lanes(const float* x, int d) {
+    double part[64];
+    for (int l = 0; l < 64; ++l) {
+        double acc = 0.0;
+        for (int i = l; i < d; i += 64) acc += (double)x[i];
+        part[l] = acc;
+    }
+    return lane_tree(part);
+}
+
+// Butterfly of the FFHT kernels, src/fht_avx.c:1013-1251 (log_n = 10) and the
+// same generated pattern for every log_n >= 3 (src/fht_avx.c:33-56 for
+// log_n = 3): in-register strides 1, 2, 4 negate through (0 - v); wider
+// strides and log_n <= 2 (src/fht_avx.c:2-31) use plain u + v, u - v.  The
+// forms differ from the textbook butterfly only in the sign of exact zeros.
+void ffht(float* v, int log_n) {
+    const int n = 1 << log_n;
+    for (int lev = 0; lev < log_n; ++lev) {
+        const int s = 1 << lev;
+        for (int j = 0; j < n; j += 2 * s)
+            for (int k = 0; k < s; ++k) {
+                float a = v[j + k], b = v[j + k + s];
+                if (log_n >= 3 && s == 1) {
+                    float t = 0.0f - b;
+                    v[j + k] = a - t;
+                    v[j + k + s] = a + t;
+                } else if (log_n >= 3 && s <= 4) {
+                    float t = 0.0f - b;
+                    v[j + k] = a + b;
+                    v[j + k + s] = a + t;
+                } else {
+                    v[j + k] = a + b;
+                    v[j + k + s] = a - b;
+                }
+            }
+    }
+}
+
+// Embedding of one transformed point into ker floats (zero beyond what the
+// metric fills).  Cosine: src/sDbscan.cpp:56-57.  L1/L2: src/sDbscan.cpp:59-64
+// with the mat-vec done as a sequential fmaf chain.  Chi2:
+// src/Utilities.h:132-164.  JS: src/Utilities.h:174-208.
+void embed_point(const float* x, int d, int dist, int ker, float L, const float* R,
+                 float* out) {
+    std::fill(out, out + ker, 0.0f);
+    if (dist == D_COSINE) {
+        std::copy(x, x + d, out);
+    } else if (dist == D_L1 || dist == D_L2) {
+        const int half = ker / 2;
+        for (int r = 0; r < half; ++r) {
+            float acc = 0.0f;
+            for (int c = 0; c < d; ++c) acc = std::fmaf(R[(size_t)c * half + r], x[c], acc);
+            out[r] = std::cos(acc);
+            out[half + r] = std::sin(acc);
+        }
+    } else if (dist == D_CHI2) {
+        const int comps = ((ker / d) - 1) / 2;
+        for (int j = 0; j < d; ++j)
+            if (x[j] > 0) out[j] = std::sqrt(x[j] * L);
+        for (int i = 1; i <= comps; ++i) {
+            const int base = d + (i - 1) * 2 * d;
+            for (int j = 0; j < d; ++j)
+                if (x[j] > 0) {
+                    float f = (float)std::sqrt(2.0 * x[j] * L / std::cosh(M_PI * i * L));
+                    float ph = (float)i * L * std::log(x[j]);
+                    out[base + j] = f * std::cos(ph);
+                    out[base + d + j] = f * std::sin(ph);
+                }
+        }
+    } else if (dist == D_JS) {
+        const int comps = ((ker / d) - 1) / 2;
+        for (int j = 0; j < d; ++j)
+            if (x[j] > 0) out[j] = (float)std::sqrt(x[j] * L * 2.0 / std::log(4.0));
+        for (int i = 1; i <= comps; ++i) {
+            const int base = d + (i - 1) * 2 * d;
+            const float iL = (float)i * L;
+            float kap = (float)(2.0 / (std::log(4.0) * (1.0f + 4.0f * iL * iL) * std::cosh(M_PI * i * L)));
+            for (int j = 0; j < d; ++j)
+                if (x[j] > 0) {
+                    float f = (float)std::sqrt(2.0 * x[j] * L * kap);
+                    float ph = iL * std::log(x[j]);
+                    out[base + j] = f * std::cos(ph);
+                    out[base + d + j] = f * std::sin(ph);
+                }
+        }
+    }
+}
+
+// FP32 distance with sequential reductions (src/Utilities.h:212-287).  The
+// reference's Eigen reduction order is not reproducible, so this is the
+// "reference-like" value; exact-mode decisions use dist64 below.
+float dist32(const float* x, const float* y, int d, int dist) {
+    if (dist == D_COSINE) {
+        float s = 0.0f;
+        for (int i = 0; i < d; ++i) s += x[i] * y[i];
+        return 1.0f - s;
+    } else if (dist == D_L1) {
+        float s = 0.0f;
+        for (int i = 0; i < d; ++i) s += std::fabs(x[i] - y[i]);
+        return s;
+    } else if (dist == D_L2) {
+        float s = 0.0f;
+        for (int i = 0; i < d; ++i) { float t = x[i] - y[i]; s += t * t; }
+        return std::sqrt(s);
+    } else if (dist == D_CHI2) {
+        float t = 0.0f;
+        for (int i = 0; i < d; ++i)
+            if (x[i] > 0 && y[i] > 0) t += x[i] * y[i] / (x[i] + y[i]);
+        return (float)(1.0 - 2.0 * t);
+    } else {
+        const float e = 1e-6f;
+        float s1 = 0.0f, s2 = 0.0f;
+        for (int i = 0; i < d; ++i) {
+            float xe = x[i] + e, ye = y[i] + e;
+            s1 += xe * (float)(std::log((xe + ye) / xe) / std::log(2.0));
+            s2 += ye * (float)(std::log((xe + ye) / ye) / std::log(2.0));
+        }
+        return (float)(1.0 - (s1 + s2) / 2.0);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- host RNG
+// src/Utilities.h:47-64: default_random_engine(seed) + uniform_int<uint32_t>(0,1)
+void orc_hd3_bits(int nbits, int seed, uint8_t* out) {
+    std::default_random_engine gen((unsigned)seed);
+    std::uniform_int_distribution<uint32_t> u(0, 1);
+    for (int i = 0; i < nbits; ++i) out[i] = (uint8_t)(u(gen) & 1u);
+}
+
+// src/Utilities.h:76-93 (column-major rows x cols fill, column first)
+void orc_gauss(int rows, int cols, float mean, float stddev, int seed, float* out) {
+    std::default_random_engine gen((unsigned)seed);
+    std::normal_distribution<float> nd(mean, stddev);
+    for (int c = 0; c < cols; ++c)
+        for (int r = 0; r < rows; ++r) out[(size_t)c * rows + r] = nd(gen);
+}
+
+// src/Utilities.h:105-122
+void orc_cauchy(int rows, int cols, float x0, float gamma, int seed, float* out) {
+    std::default_random_engine gen((unsigned)seed);
+    std::cauchy_distribution<float> cd(x0, gamma);
+    for (int c = 0; c < cols; ++c)
+        for (int r = 0; r < rows; ++r) out[(size_t)c * rows + r] = cd(gen);
+}
+
+// ---------------------------------------------------------------- transform
+// src/Utilities.h:289-335.  Returns 0 ok, 1 negative entry (Chi2/JS), 2 zero
+// point (Chi2/JS zero sum; Cosine zero norm, which the reference divides by).
+int orc_transform(float* X, int n, int d, int dist) {
+    int err = 0;
+    if (dist == D_COSINE) {
+#pragma omp parallel for reduction(max : err)
+        for (int p = 0; p < n; ++p) {
+            float* x = X + (size_t)p * d;
+            float nrm = (float)std::sqrt(sumsq_lanes(x, d));
+            if (!(nrm > 0.0f)) { err = 2; continue; }
+            for (int i = 0; i < d; ++i) x[i] = x[i] / nrm;
+        }
+    } else if (dist == D_CHI2 || dist == D_JS) {
+        for (size_t i = 0; i < (size_t)n * d; ++i)
+            if (X[i] < 0) return 1;
+#pragma omp parallel for reduction(max : err)
+        for (int p = 0; p < n; ++p) {
+            float* x = X + (size_t)p * d;
+            float s = (float)sum_lanes(x, d);
+            if (s <= 0.0f) { err = 2; continue; }
+            for (int i = 0; i < d; ++i) x[i] = x[i] / s;
+        }
+    }
+    return err;
+}
+
+// ------------------------------------------------------------------- FFHT
+void orc_fht(float* buf, int log_n) { ffht(buf, log_n); }
+
+// Textbook increasing-stride FWHT (src/Utilities.h:340-384), used as a pin.
+void orc_fwht_textbook(float* v, int log_n) {
+    const int n = 1 << log_n;
+    for (int lev = 0; lev < log_n; ++lev) {
+        const int s = 1 << lev;
+        for (int j = 0; j < n; j += 2 * s)
+            for (int k = 0; k < s; ++k) {
+                float a = v[j + k];
+                v[j + k] = a + v[j + k + s];
+                v[j + k + s] = a - v[j + k + s];
+            }
+    }
+}
+
+void orc_embed(const float* X, int n, int d, int dist, int ker, float L, const float* R,
+               float* E) {
+#pragma omp parallel for
+    for (int p = 0; p < n; ++p) embed_point(X + (size_t)p * d, d, dist, ker, L, R, E + (size_t)p * ker);
+}
+
+// HD3 rotation of already-embedded points (src/sDbscan.cpp:74-91): zero pad
+// to D, then 3 x {sign flip, FFHT}; keep the first P outputs.
+void orc_project_embedded(const float* E, int n, int ker, const uint8_t* bits, int D, int P,
+                          float* proj) {
+    int log_n = 0;
+    while ((1 << log_n) < D) ++log_n;
+#pragma omp parallel for
+    for (int p = 0; p < n; ++p) {
+        std::vector<float> rot(D, 0.0f);
+        std::copy(E + (size_t)p * ker, E + (size_t)p * ker + ker, rot.begin());
+        for (int r = 0; r < 3; ++r) {
+            for (int j = 0; j < D; ++j) rot[j] *= (float)(2 * (int)bits[r * D + j] - 1);
+            ffht(rot.data(), log_n);
+        }
+        for (int j = 0; j < P; ++j) proj[(size_t)j * n + p] = rot[j];
+    }
+}
+
+// -------------------------------------------------------- top-K / top-M
+// src/sDbscan.cpp:97-143: per point, K largest / K smallest of P values.
+void orc_topk(const float* proj, int n, int P, int K, int* topK) {
+#pragma omp parallel for
+    for (int p = 0; p < n; ++p) {
+        MinHeap close, far;
+        for (int j = 0; j < P; ++j) {
+            float v = proj[(size_t)j * n + p];
+            if ((int)close.size() < K) close.emplace(j, v);
+            else if (v > close.top().val) { close.pop(); close.emplace(j, v); }
+            if ((int)far.size() < K) far.emplace(j, -v);
+            else if (-v > far.top().val) { far.pop(); far.emplace(j, -v); }
+        }
+        for (int k = K - 1; k >= 0; --k) {
+            topK[(size_t)p * 2 * K + k] = close.top().idx; close.pop();
+            topK[(size_t)p * 2 * K + K + k] = far.top().idx; far.pop();
+        }
+    }
+}
+
+// Same heap semantics over one value vector (used for a single point or row).
+void orc_heap_select(const float* vals, int len, int k, int* out_close, int* out_far) {
+    MinHeap close, far;
+    for (int j = 0; j < len; ++j) {
+        float v = vals[j];
+        if ((int)close.size() < k) close.emplace(j, v);
+        else if (v > close.top().val) { close.pop(); close.emplace(j, v); }
+        if ((int)far.size() < k) far.emplace(j, -v);
+        else if (-v > far.top().val) { far.pop(); far.emplace(j, -v); }
+    }
+    for (int m = k - 1; m >= 0; --m) {
+        out_close[m] = close.top().idx; close.pop();
+        out_far[m] = far.top().idx; far.pop();
+    }
+}
+
+// src/sDbscan.cpp:151-208: per random vector, M largest / M smallest over n.
+void orc_topm(const float* proj, int n, int P, int M, int* topM) {
+#pragma omp parallel for
+    for (int r = 0; r < P; ++r)
+        orc_heap_select(proj + (size_t)r * n, n, M, topM + (size_t)r * 2 * M,
+                        topM + (size_t)r * 2 * M + M);
+}
+
+// ----------------------------------------------------------- distances
+float orc_dist32(const float* x, const float* y, int d, int dist) { return dist32(x, y, d, dist); }
+
+// FP64 distance in the fixed order of the HIP boundary recheck: per lane
+// l = 0..63 a sequential sum over i = l, l + 64, ..., then the xor butterfly
+// of lane_tree.  Same expression per term as the HIP kernel.
+double orc_dist64(const float* x, const float* y, int d, int dist) {
+    double part[64];
+    const double ln2 = 0.69314718055994530942;
+    for (int l = 0; l < 64; ++l) {
+        double s = 0.0;
+        for (int i = l; i < d; i += 64) {
+            const double xv = x[i], yv = y[i];
+            if (dist == D_COSINE) s = s + xv * yv;
+            else if (dist == D_L1) s = s + std::fabs(xv - yv);
+            else if (dist == D_L2) { double t = xv - yv; s = s + t * t; }
+            else if (dist == D_CHI2) { if (x[i] > 0 && y[i] > 0) s = s + (xv * yv) / (xv + yv); }
+            else {
+                const double xe = xv + 1e-6, ye = yv + 1e-6, t = xe + ye;
+                s = s + (xe * (std::log(t / xe) / ln2) + ye * (std::log(t / ye) / ln2));
+            }
+        }
+        part[l] = s;
+    }
+    const double S = lane_tree(part);
+    if (dist == D_COSINE) return 1.0 - S;
+    if (dist == D_L1) return S;
+    if (dist == D_L2) return std::sqrt(S);
+    if (dist == D_CHI2) return 1.0 - 2.0 * S;
+    return 1.0 - S / 2.0;
+}
+
+// ------------------------------------------------------------ find core
+// src/sDbscan.cpp:217-408.  mode 0: decide with the FP32 value (reference
+// arithmetic, sequential order); mode 1: decide with the FP64 value (the
+// decision rule of the HIP path).  Outputs the deduplicated neighbour count of
+// every point, the core flags, the neighbour CSR (sorted by index; kept for
+// every point, core or not) and the number of computed pairs whose FP64
+// distance lies within 1e-6 of eps.  Optionally the FP32 distance per CSR
+// entry (nbr_dist may be null).
+//
+// csr_ptr must hold n+1 entries; csr_idx/csr_dist are allocated here and
+// released with orc_free.
+struct CoreResult {
+    std::vector<int> ptr, idx;
+    std::vector<float> dd;
+};
+
+void* orc_find_core(const float* X, int n, int d, int dist, const int* topK, int K,
+                    const int* topM, int M, float eps, int minPts, int mode,
+                    uint8_t* core, int* degree, long long* near_pairs, long long* n_dist) {
+    std::vector<std::vector<std::pair<int, float>>> nbr(n);
+    long long near = 0, ndist = 0;
+    const int words = (n + 63) / 64;
+#pragma omp parallel for reduction(+ : near, ndist) schedule(dynamic, 64)
+    for (int p = 0; p < n; ++p) {
+        std::vector<uint64_t> seen(words, 0ull);  // src/sDbscan.cpp:260
+        const float* xp = X + (size_t)p * d;
+        for (int k = 0; k < K; ++k) {
+            for (int side = 0; side < 2; ++side) {
+                const int r = topK[(size_t)p * 2 * K + k + side * K];
+                for (int i = 0; i < M; ++i) {
+                    const int q = topM[(size_t)r * 2 * M + side * M + i];
+                    if (q == p) continue;
+                    if (seen[q >> 6] >> (q & 63) & 1ull) continue;
+                    seen[q >> 6] |= 1ull << (q & 63);
+                    const float* xq = X + (size_t)q * d;
+                    ++ndist;
+                    const float d32 = dist32(xp, xq, d, dist);
+                    bool hit = d32 <= eps;
+                    if (mode == 1 || near_pairs) {
+                        const double d64 = orc_dist64(xp, xq, d, dist);
+                        if (std::fabs(d64 - (double)eps) < 1e-6) ++near;
+                        if (mode == 1) hit = d64 <= (double)eps;
+                    }
+                    if (hit) {
+#pragma omp critical
+                        {
+                            nbr[p].emplace_back(q, d32);
+                            nbr[q].emplace_back(p, d32);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    CoreResult* res = new CoreResult();
+    res->ptr.assign(n + 1, 0);
+#pragma omp parallel for
+    for (int p = 0; p < n; ++p) {
+        // src/sDbscan.cpp:374-400 dedups with unordered_set; the kept order is
+        // irrelevant to labels, so the CSR is sorted by index.
+        std::unordered_map<int, float> uniq(nbr[p].begin(), nbr[p].end());
+        std::vector<std::pair<int, float>> v(uniq.begin(), uniq.end());
+        std::sort(v.begin(), v.end());
+        nbr[p].swap(v);
+        degree[p] = (int)nbr[p].size();
+        core[p] = (int)nbr[p].size() >= minPts - 1 ? 1 : 0;
+    }
+    for (int p = 0; p < n; ++p) res->ptr[p + 1] = res->ptr[p] + (int)nbr[p].size();
+    res->idx.resize(res->ptr[n]);
+    res->dd.resize(res->ptr[n]);
+    for (int p = 0; p < n; ++p)
+        for (size_t j = 0; j < nbr[p].size(); ++j) {
+            res->idx[res->ptr[p] + j] = nbr[p][j].first;
+            res->dd[res->ptr[p] + j] = nbr[p][j].second;
+        }
+    if (near_pairs) *near_pairs = near;
+    if (n_dist) *n_dist = ndist;
+    return res;
+}
+
+long long orc_csr_size(void* h) { return (long long)((CoreResult*)h)->idx.size(); }
+
+void orc_csr_copy(void* h, int* ptr, int* idx, float* dd) {
+    CoreResult* r = (CoreResult*)h;
+    std::copy(r->ptr.begin(), r->ptr.end(), ptr);
+    std::copy(r->idx.begin(), r->idx.end(), idx);
+    if (dd) std::copy(r->dd.begin(), r->dd.end(), dd);
+}
+
+void orc_free(void* h) { delete (CoreResult*)h; }
+
+// ----------------------------------------------------------- form cluster
+// src/sDbscan.cpp:509-637 (serial BFS; labelNoise not included).  nbr CSR must
+// be symmetric; only core rows are traversed, as in the reference where
+// non-core neighbourhoods are cleared.
+int orc_form_cluster(int n, const uint8_t* core, const int* ptr, const int* idx, int* labels) {
+    std::fill(labels, labels + n, -1);
+    int cid = -1, n_clusters = 0;
+    const int words = (n + 63) / 64;
+    for (int p = 0; p < n; ++p) {
+        if (!core[p] || labels[p] != -1) continue;
+        ++cid;
+        std::unordered_set<int> seeds;
+        seeds.insert(p);
+        std::vector<uint64_t> conn(words, 0ull);  // src/sDbscan.cpp:556
+        conn[p >> 6] |= 1ull << (p & 63);
+        while (!seeds.empty()) {
+            int xi = *seeds.begin();
+            seeds.erase(seeds.begin());
+            for (int j = ptr[xi]; j < ptr[xi + 1]; ++j) {
+                int xj = idx[j];
+                bool c = conn[xj >> 6] >> (xj & 63) & 1ull;
+                if (core[xj]) {
+                    if (!c) {
+                        conn[xj >> 6] |= 1ull << (xj & 63);
+                        if (labels[xj] == -1) seeds.insert(xj);
+                    }
+                } else {
+                    conn[xj >> 6] |= 1ull << (xj & 63);
+                }
+            }
+        }
+        for (int w = 0; w < words; ++w) {
+            uint64_t b = conn[w];
+            while (b) {
+                int q = w * 64 + __builtin_ctzll(b);
+                b &= b - 1;
+                if (labels[q] == -1) labels[q] = cid;
+            }
+        }
+        n_clusters = cid;
+    }
+    return n_clusters + 2;  // src/sDbscan.cpp:633
+}
+
+// ------------------------------------------------------------- sOPTICS
+// Core distance (src/sDbscan.cpp:1423-1447): the (minPts-1)-th smallest
+// neighbour distance of a core point, 0 otherwise.
+void orc_core_dist(int n, int minPts, const int* ptr, const float* dd, uint8_t* core,
+                   float* core_dist) {
+#pragma omp parallel for
+    for (int p = 0; p < n; ++p) {
+        int cnt = ptr[p + 1] - ptr[p];
+        core_dist[p] = 0.0f;
+        core[p] = cnt >= minPts - 1 ? 1 : 0;
+        if (core[p]) {
+            std::vector<float> v(dd + ptr[p], dd + ptr[p + 1]);
+            std::nth_element(v.begin(), v.begin() + (minPts - 2), v.end());
+            core_dist[p] = v[minPts - 2];
+        }
+    }
+}
+
+// Ordering and reachability (src/sDbscan.cpp:1588-1703).  Neighbours are
+// visited in CSR order.
+void orc_optics(int n, const uint8_t* core, const float* core_dist, const int* ptr,
+                const int* idx, const float* dd, int* ordering, float* reach) {
+    const float NEG = -2147483648.0f;  // src/Header.h:20
+    std::fill(reach, reach + n, NEG);
+    std::vector<uint8_t> done(n, 0);
+    int pos = 0;
+    auto expand = [&](int xi, MinHeap& seeds) {
+        float cd = core_dist[xi];
+        for (int j = ptr[xi]; j < ptr[xi + 1]; ++j) {
+            int xj = idx[j];
+            if (done[xj]) continue;
+            float r = std::max(cd, dd[j]);
+            if (reach[xj] == NEG) { reach[xj] = r; seeds.emplace(xj, reach[xj]); }
+            else if (reach[xj] > r) { reach[xj] = r; seeds.emplace(xj, reach[xj]); }
+        }
+    };
+    for (int xi = 0; xi < n; ++xi) {
+        if (done[xi]) continue;
+        done[xi] = 1;
+        ordering[pos++] = xi;
+        if (!core[xi]) continue;
+        MinHeap seeds;
+        expand(xi, seeds);
+        while (!seeds.empty()) {
+            int xj = seeds.top().idx;
+            seeds.pop();
+            if (done[xj]) continue;
+            done[xj] = 1;
+            ordering[pos++] = xj;
+            if (core[xj]) expand(xj, seeds);
+        }
+    }
+}
+
+// ------------------------------------------------------ full fit (baseline)
+// fit_sDbscan (src/sDbscan.cpp:830-890) on the CPU with the reference's phase
+// structure.  X is transformed in place.  Phase times in ms go to t_ms[4]
+// (transform, index, core, cluster).  Returns n_clusters_.
+int orc_fit_sdbscan(float* X, int n, int d, int dist, int P, int K, int M, int ker, float L,
+                    const float* R, const uint8_t* bits, int D, float eps, int minPts,
+                    int n_threads, int* labels, uint8_t* core_out, double* t_ms) {
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    auto t0 = now();
+    orc_transform(X, n, d, dist);
+    auto t1 = now();
+    std::vector<float> E((size_t)n * ker), proj((size_t)P * n);
+    orc_embed(X, n, d, dist, ker, L, R, E.data());
+    orc_project_embedded(E.data(), n, ker, bits, D, P, proj.data());
+    std::vector<int> topK((size_t)n * 2 * K), topM((size_t)P * 2 * M);
+    orc_topk(proj.data(), n, P, K, topK.data());
+    orc_topm(proj.data(), n, P, M, topM.data());
+    auto t2 = now();
+    std::vector<int> deg(n);
+    void* h = orc_find_core(X, n, d, dist, topK.data(), K, topM.data(), M, eps, minPts, 0,
+                            core_out, deg.data(), nullptr, nullptr);
+    auto t3 = now();
+    CoreResult* r = (CoreResult*)h;
+    int nc = orc_form_cluster(n, core_out, r->ptr.data(), r->idx.data(), labels);
+    auto t4 = now();
+    orc_free(h);
+    if (t_ms) { t_ms[0] = ms(t0, t1); t_ms[1] = ms(t1, t2); t_ms[2] = ms(t2, t3); t_ms[3] = ms(t3, t4); }
+    return nc;
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

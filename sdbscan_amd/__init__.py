@@ -1,0 +1,233 @@
+"""sdbscan_amd -- sDbscan's neighbourhood-discovery path on AMD Instinct MI355X.
+
+The product is two in-tree shared libraries built by ``sdbscan_amd.build``:
+
+* ``libsdbscan_amd.so`` -- HIP kernels for gfx950 behind the C ABI declared in
+  ``include/sdbscan_amd.h``;
+* ``sDbscan.cpython-*.so`` -- the pybind11 module ``sDbscan``, a drop-in for
+  the reference's module (same class, methods and keyword arguments).
+
+``capi()`` returns the C ABI through ctypes; ``module()`` imports the drop-in
+module.  Both raise if the libraries are not built: there is no CPU fallback.
+"""
+import ctypes
+import glob
+import importlib.util
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsdbscan_amd.so")
+
+COSINE, L1, L2, CHI2, JS = 0, 1, 2, 3, 4
+FP32, BF16 = 0, 1
+METRICS = {"Cosine": COSINE, "L1": L1, "L2": L2, "Chi2": CHI2, "JS": JS}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("n_points", ctypes.c_longlong), ("n_features", ctypes.c_int),
+                ("n_proj", ctypes.c_int), ("topK", ctypes.c_int), ("topM", ctypes.c_int),
+                ("metric", ctypes.c_int), ("ker_n_features", ctypes.c_int),
+                ("ker_sigma", ctypes.c_float), ("ker_intervalSampling", ctypes.c_float),
+                ("fhtDim", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("t_transform_ms", ctypes.c_double), ("t_index_ms", ctypes.c_double),
+                ("t_core_ms", ctypes.c_double), ("t_cluster_ms", ctypes.c_double),
+                ("t_dist_ms", ctypes.c_double), ("n_pairs", ctypes.c_longlong),
+                ("n_queue", ctypes.c_longlong), ("n_recheck32", ctypes.c_longlong),
+                ("n_recheck64", ctypes.c_longlong), ("n_recheck_hits", ctypes.c_longlong),
+                ("n_hubs", ctypes.c_longlong), ("n_tie_points", ctypes.c_longlong),
+                ("n_tie_rows", ctypes.c_longlong), ("n_core", ctypes.c_longlong),
+                ("n_components", ctypes.c_int), ("band", ctypes.c_double),
+                ("band32", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_P = ctypes.c_void_p
+_SIGS = {
+    "sdb_create": ([ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "sdb_destroy": ([_P], None),
+    "sdb_last_error": ([_P], ctypes.c_char_p),
+    "sdb_set_params": ([_P, ctypes.POINTER(Params)], ctypes.c_int),
+    "sdb_load_host": ([_P, _P, ctypes.c_longlong, ctypes.c_int], ctypes.c_int),
+    "sdb_load_device": ([_P, _P, ctypes.c_longlong, ctypes.c_int], ctypes.c_int),
+    "sdb_build_index": ([_P, _P, _P], ctypes.c_int),
+    "sdb_find_core": ([_P, ctypes.c_float, ctypes.c_int, _P, _P], ctypes.c_int),
+    "sdb_form_clusters": ([_P, _P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "sdb_get_index": ([_P, _P, _P], ctypes.c_int),
+    "sdb_get_transformed": ([_P, _P], ctypes.c_int),
+    "sdb_get_neighbors": ([_P, _P, _P, _P], ctypes.c_int),
+    "sdb_core_dist": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "sdb_optics_reach": ([_P, _P, _P], ctypes.c_int),
+    "sdb_get_stats": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
+    "sdb_generate": ([_P, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_ulonglong,
+                      ctypes.c_float], ctypes.c_int),
+    "sdb_copy_input": ([_P, ctypes.c_longlong, ctypes.c_longlong, _P], ctypes.c_int),
+}
+
+_lib = None
+
+
+def capi():
+    """The C ABI of libsdbscan_amd.so (ctypes, signatures set)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run python -m sdbscan_amd.build")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = lib
+    return _lib
+
+
+def module():
+    """The pybind11 drop-in module ``sDbscan``."""
+    cands = glob.glob(os.path.join(HERE, "sDbscan*.so"))
+    if not cands:
+        raise RuntimeError("sDbscan module is not built: run python -m sdbscan_amd.build")
+    spec = importlib.util.spec_from_file_location("sDbscan", cands[0])
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def fht_dim(n_proj, ker_n_features):
+    """fhtDim of src/sDbscan.h:107."""
+    import math
+    return 1 << int(math.ceil(math.log2(max(n_proj, ker_n_features))))
+
+
+class SdbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Context:
+    """Thin ctypes wrapper over one sdb_ctx (tests and bench use it)."""
+
+    def __init__(self, device=0):
+        self.lib = capi()
+        h = ctypes.c_void_p()
+        rc = self.lib.sdb_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise SdbError(rc, "sdb_create failed (no HIP device?)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.sdb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SdbError(rc, self.lib.sdb_last_error(self.h).decode())
+
+    def set_params(self, n, d, n_proj=1024, topK=10, topM=50, metric=COSINE, ker_n_features=None,
+                   ker_sigma=1.0, ker_intervalSampling=0.4, precision=FP32):
+        if ker_n_features is None or metric == COSINE:
+            ker_n_features = d if metric == COSINE else 1024
+        p = Params(n, d, n_proj, topK, topM, metric, ker_n_features, ker_sigma,
+                   ker_intervalSampling, fht_dim(n_proj, ker_n_features), precision)
+        self.params = p
+        self._chk(self.lib.sdb_set_params(self.h, ctypes.byref(p)))
+        return p
+
+    def load(self, X):
+        import numpy as np
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        self._chk(self.lib.sdb_load_host(self.h, X.ctypes.data, X.shape[0], X.shape[1]))
+
+    def generate(self, n, d, seed, sigma=0.1, kind=0):
+        self._chk(self.lib.sdb_generate(self.h, kind, n, d, seed, sigma))
+
+    def build_index(self, bits, R=None):
+        import numpy as np
+        bits = np.ascontiguousarray(bits, dtype=np.uint8)
+        Rp = None
+        if R is not None:
+            R = np.ascontiguousarray(R, dtype=np.float32)
+            Rp = R.ctypes.data
+        self._chk(self.lib.sdb_build_index(self.h, bits.ctypes.data, Rp))
+
+    def find_core(self, eps, minPts, want=True):
+        import numpy as np
+        n = self.params.n_points
+        core = np.zeros(n, np.uint8) if want else None
+        deg = np.zeros(n, np.int32) if want else None
+        self._chk(self.lib.sdb_find_core(self.h, eps, minPts,
+                                         core.ctypes.data if want else None,
+                                         deg.ctypes.data if want else None))
+        return core, deg
+
+    def form_clusters(self, want=True):
+        import numpy as np
+        n = self.params.n_points
+        lab = np.zeros(n, np.int32) if want else None
+        nc = ctypes.c_int()
+        self._chk(self.lib.sdb_form_clusters(self.h, lab.ctypes.data if want else None,
+                                             ctypes.byref(nc)))
+        return lab, nc.value
+
+    def get_index(self):
+        import numpy as np
+        p = self.params
+        tk = np.zeros((p.n_points, 2 * p.topK), np.int32)
+        tm = np.zeros((p.n_proj, 2 * p.topM), np.int32)
+        self._chk(self.lib.sdb_get_index(self.h, tk.ctypes.data, tm.ctypes.data))
+        return tk, tm
+
+    def get_transformed(self):
+        import numpy as np
+        p = self.params
+        X = np.zeros((p.n_points, p.n_features), np.float32)
+        self._chk(self.lib.sdb_get_transformed(self.h, X.ctypes.data))
+        return X
+
+    def get_neighbors(self):
+        import numpy as np
+        n = self.params.n_points
+        ptr = np.zeros(n + 1, np.int64)
+        self._chk(self.lib.sdb_get_neighbors(self.h, ptr.ctypes.data, None, None))
+        tot = int(ptr[-1])
+        idx = np.zeros(max(tot, 1), np.int32)
+        dist = np.zeros(max(tot, 1), np.float32)
+        self._chk(self.lib.sdb_get_neighbors(self.h, ptr.ctypes.data, idx.ctypes.data,
+                                             dist.ctypes.data))
+        return ptr, idx[:tot], dist[:tot]
+
+    def core_dist(self, minPts):
+        import numpy as np
+        out = np.zeros(self.params.n_points, np.float32)
+        self._chk(self.lib.sdb_core_dist(self.h, minPts, out.ctypes.data))
+        return out
+
+    def optics_reach(self, ordering):
+        import numpy as np
+        ordering = np.ascontiguousarray(ordering, dtype=np.int32)
+        out = np.zeros(self.params.n_points, np.float32)
+        self._chk(self.lib.sdb_optics_reach(self.h, ordering.ctypes.data, out.ctypes.data))
+        return out
+
+    def stats(self):
+        s = Stats()
+        self._chk(self.lib.sdb_get_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def copy_input(self, row0, rows):
+        import numpy as np
+        out = np.zeros((rows, self.params.n_features), np.float32)
+        self._chk(self.lib.sdb_copy_input(self.h, row0, rows, out.ctypes.data))
+        return out

@@ -1,0 +1,335 @@
+// sDbscan.cpp -- host side of the drop-in class: parameters, the reference's host
+// RNG calls, phase timers and the sOPTICS ordering pass.  Every device stage goes
+// through the C ABI (include/sdbscan_amd.h).
+#include "sDbscan.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <fstream>
+#include <functional>
+#include <iostream>
+#include <queue>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+
+namespace {
+
+int metric_of(const std::string& dist) {
+    if (dist == "Cosine") return SDB_COSINE;
+    if (dist == "L1") return SDB_L1;
+    if (dist == "L2") return SDB_L2;
+    if (dist == "Chi2") return SDB_CHI2;
+    if (dist == "JS") return SDB_JS;
+    return -1;
+}
+
+std::string int2str(long long x) {
+    std::stringstream ss;
+    ss << x;
+    return ss.str();
+}
+
+unsigned seed_of(int random_seed) {
+    unsigned s = (unsigned)std::chrono::system_clock::now().time_since_epoch().count();
+    if (random_seed >= 0) s = (unsigned)random_seed;
+    return s;
+}
+
+// (index, value) pair ordered by value only (src/Header.h:48-75, :100).
+struct OPair {
+    int idx;
+    float val;
+    OPair(int i, double v) : idx(i), val((float)v) {}
+    bool operator>(const OPair& o) const { return val > o.val; }
+};
+typedef std::priority_queue<OPair, std::vector<OPair>, std::greater<OPair>> OMinHeap;
+
+}  // namespace
+
+sDbscan::sDbscan(int n, int d) : n_points(n), n_features(d) {}
+
+sDbscan::~sDbscan() {
+    if (ctx) sdb_destroy(ctx);
+}
+
+void sDbscan::set_threads(int t) { n_threads = t <= 0 ? omp_get_max_threads() : t; }
+
+// src/sDbscan.h:79-108
+void sDbscan::set_params(int numProj, int k, int m, std::string dist, int kerDim, float kerSigma,
+                         float kerSam, int noiseClustering, float prob, bool ver,
+                         std::string filename, int numThreads, int randomSeed) {
+    n_proj = numProj;
+    topK = k;
+    topM = m;
+    distance = dist;
+    ker_n_features = kerDim;
+    ker_sigma = kerSigma;
+    ker_intervalSampling = kerSam;
+    samplingProb = prob;
+    clusterNoise = noiseClustering;
+    verbose = ver;
+    set_threads(numThreads);
+    seed = randomSeed;
+    output = filename;
+    if (distance == "Cosine") ker_n_features = n_features;
+    fhtDim = 1 << int(std::ceil(std::log2(std::max(n_proj, ker_n_features))));
+}
+
+// src/sDbscan.h:110-124
+void sDbscan::set_kernel_params(std::string dist, int kerDim, float kerSigma, float kerSam) {
+    distance = dist;
+    ker_n_features = kerDim;
+    ker_sigma = kerSigma;
+    ker_intervalSampling = kerSam;
+    if (distance == "Cosine") ker_n_features = n_features;
+    fhtDim = 1 << int(std::ceil(std::log2(std::max(n_proj, ker_n_features))));
+}
+
+// src/sDbscan.h:127-140
+void sDbscan::set_sngParams(std::string dist, float prob, int noiseClustering, bool ver,
+                            std::string filename, int numThreads, int randomSeed) {
+    distance = dist;
+    samplingProb = prob;
+    if (noiseClustering > 0) clusterNoise = 2;
+    verbose = ver;
+    set_threads(numThreads);
+    seed = randomSeed;
+    output = filename;
+}
+
+// src/sDbscan.h:142-163
+void sDbscan::clear() {
+    labels_.clear();
+    n_clusters_ = 0;
+    ordering_.clear();
+    reachability_.clear();
+    matrix_R.clear();
+    bitHD3.clear();
+}
+
+void sDbscan::set_precision(const std::string& p) {
+    if (p == "FP32" || p == "fp32") precision = SDB_FP32;
+    else if (p == "BF16" || p == "bf16") precision = SDB_BF16;
+    else throw std::invalid_argument("precision must be FP32 or BF16");
+}
+
+void sDbscan::set_device(int dev) {
+    if (ctx) { sdb_destroy(ctx); ctx = nullptr; }
+    device = dev;
+}
+
+void sDbscan::ensure_ctx() {
+    if (ctx) return;
+    int rc = sdb_create(device, &ctx);
+    if (rc != SDB_OK) {
+        ctx = nullptr;
+        throw std::runtime_error("sdb_create failed (no usable HIP device " + int2str(device) + ")");
+    }
+}
+
+void sDbscan::check(int rc, const char* what) {
+    if (rc == SDB_OK) return;
+    std::string msg = std::string(what) + ": " + (ctx ? sdb_last_error(ctx) : "no context");
+    if (rc == SDB_ENEGATIVE || rc == SDB_EZERO) msg = ctx ? sdb_last_error(ctx) : what;
+    if (rc == SDB_EINVAL || rc == SDB_EUNSUPPORTED) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+// Host RNG: bitHD3Generator, gaussGenerator, cauchyGenerator
+// (src/Utilities.h:47-122), called in the order of rp_parIndex (src/sDbscan.cpp:25-34).
+void sDbscan::make_random() {
+    const int half = ker_n_features / 2;
+    matrix_R.clear();
+    if (distance == "L1" || distance == "L2") {
+        std::default_random_engine gen(seed_of(seed));
+        matrix_R.assign((size_t)half * n_features, 0.0f);
+        if (distance == "L1") {
+            std::cauchy_distribution<float> cd(0, 1.0 / ker_sigma);
+            for (int c = 0; c < n_features; ++c)
+                for (int r = 0; r < half; ++r) matrix_R[(size_t)c * half + r] = cd(gen);
+        } else {
+            std::normal_distribution<float> nd(0, 1.0 / ker_sigma);
+            for (int c = 0; c < n_features; ++c)
+                for (int r = 0; r < half; ++r) matrix_R[(size_t)c * half + r] = nd(gen);
+        }
+    }
+    const int nbits = fhtDim * n_rotate;
+    std::default_random_engine gen(seed_of(seed));
+    std::uniform_int_distribution<uint32_t> u(0, 1);
+    bitHD3.assign(nbits, 0);
+    for (int i = 0; i < nbits; ++i) bitHD3[i] = (uint8_t)(u(gen) & 1u);
+}
+
+void sDbscan::prepare(const float* X) {
+    const int metric = metric_of(distance);
+    if (metric < 0) throw std::invalid_argument("Error: The distance is not support");
+    if (fhtDim <= 0) set_params(n_proj, topK, topM, distance, ker_n_features, ker_sigma,
+                                ker_intervalSampling, clusterNoise, samplingProb, verbose, output,
+                                n_threads, seed);
+    if (clusterNoise != 0)
+        throw std::invalid_argument("clusterNoise (labelNoise, src/sDbscan.cpp:647-822) is not part of this build");
+    ensure_ctx();
+    sdb_params p{};
+    p.n_points = n_points;
+    p.n_features = n_features;
+    p.n_proj = n_proj;
+    p.topK = topK;
+    p.topM = topM;
+    p.metric = metric;
+    p.ker_n_features = metric == SDB_COSINE ? n_features : ker_n_features;
+    p.ker_sigma = ker_sigma;
+    p.ker_intervalSampling = ker_intervalSampling;
+    p.fhtDim = fhtDim;
+    p.precision = precision;
+    check(sdb_set_params(ctx, &p), "set_params");
+    check(sdb_load_host(ctx, X, n_points, n_features), "load");
+    make_random();
+}
+
+void sDbscan::print_params(float eps, int minPts) const {
+    std::cout << "eps: " << eps << std::endl;
+    std::cout << "minPts: " << minPts << std::endl;
+    std::cout << "n_points: " << n_points << std::endl;
+    std::cout << "n_features: " << n_features << std::endl;
+    std::cout << "n_proj: " << n_proj << std::endl;
+    std::cout << "topK: " << topK << std::endl;
+    std::cout << "topM: " << topM << std::endl;
+    std::cout << "distance: " << distance << std::endl;
+    std::cout << "cluster noise: " << clusterNoise << std::endl;
+    std::cout << "kernel features: " << ker_n_features << std::endl;
+    std::cout << "sigma: " << ker_sigma << std::endl;
+    std::cout << "interval sampling: " << ker_intervalSampling << std::endl;
+    std::cout << "sDbscan-1NN prob: " << samplingProb << std::endl;
+    std::cout << "n_threads: " << n_threads << std::endl;
+}
+
+// File names of src/sDbscan.cpp:880-886 and :1759-1765.
+std::string sDbscan::out_name(float eps, int minPts, bool optics) const {
+    return output + "_" + distance + "_Eps_" + int2str(std::lround(1000 * eps)) + "_MinPts_" +
+           int2str(minPts) + (optics ? "_NumEmbed_" : "_KerFeatures_") + int2str(ker_n_features) +
+           "_NumProj_" + int2str(n_proj) + "_TopM_" + int2str(topM) + "_TopK_" + int2str(topK);
+}
+
+// src/sDbscan.cpp:830-890
+void sDbscan::fit_sDbscan(const float* X, float eps, int minPts) {
+    if (verbose) print_params(eps, minPts);
+    prepare(X);
+    check(sdb_build_index(ctx, bitHD3.data(), matrix_R.empty() ? nullptr : matrix_R.data()), "build_index");
+    check(sdb_find_core(ctx, eps, minPts, nullptr, nullptr), "find_core");
+    labels_.assign(n_points, -1);
+    check(sdb_form_clusters(ctx, labels_.data(), &n_clusters_), "form_clusters");
+    if (verbose) {
+        sdb_stats s{};
+        sdb_get_stats(ctx, &s);
+        std::cout << "Check X supporting distance time = " << (long long)s.t_transform_ms << "[ms]" << std::endl;
+        std::cout << "Build index time = " << (long long)s.t_index_ms << "[ms]" << std::endl;
+        std::cout << "Number of core points: " << s.n_core << std::endl;
+        std::cout << "Find core points time = " << (long long)s.t_core_ms << "[ms]" << std::endl;
+        std::cout << "Number of clusters: " << n_clusters_ << std::endl;
+        std::cout << "Form clusters time  (including clustering noise) = " << (long long)s.t_cluster_ms << "[ms]" << std::endl;
+        std::ofstream f(out_name(eps, minPts, false));
+        for (int v : labels_) f << v << '\n';
+    }
+}
+
+// src/sDbscan.cpp:980-1039: one index, n_tests x (core points + clusters)
+void sDbscan::test_sDbscan(const float* X, float eps, float rangeEps, int n_tests, int minPts) {
+    std::cout << "base eps: " << eps << std::endl;
+    std::cout << "range eps: " << rangeEps << std::endl;
+    std::cout << "n_tests: " << n_tests << std::endl;
+    prepare(X);
+    verbose = true;
+    check(sdb_build_index(ctx, bitHD3.data(), matrix_R.empty() ? nullptr : matrix_R.data()), "build_index");
+    sdb_stats s{};
+    sdb_get_stats(ctx, &s);
+    std::cout << "Build index time = " << (long long)(s.t_transform_ms + s.t_index_ms) << "[ms]" << std::endl;
+    for (int i = 0; i < n_tests; ++i) {
+        const float e = eps + 1.0f * i * rangeEps;
+        std::cout << "Eps: " << e << std::endl;
+        check(sdb_find_core(ctx, e, minPts, nullptr, nullptr), "find_core");
+        labels_.assign(n_points, -1);
+        check(sdb_form_clusters(ctx, labels_.data(), &n_clusters_), "form_clusters");
+        sdb_get_stats(ctx, &s);
+        std::cout << "Find core points time = " << (long long)s.t_core_ms << "[ms]" << std::endl;
+        std::cout << "Form clusters time (including clustering noise) = " << (long long)s.t_cluster_ms << "[ms]" << std::endl;
+        std::ofstream f(out_name(e, minPts, false));
+        for (int v : labels_) f << v << '\n';
+    }
+}
+
+// src/sDbscan.cpp:1711-1769.  Neighbourhoods and core distances come from
+// the device; the ordering pass is the sequential heap walk of
+// formOptics_scikit (src/sDbscan.cpp:1588-1703) over neighbourhoods sorted by
+// index; the reachability values are then computed on the device for that
+// ordering.
+void sDbscan::fit_sOptics(const float* X, float eps, int minPts) {
+    if (verbose) print_params(eps, minPts);
+    prepare(X);
+    check(sdb_build_index(ctx, bitHD3.data(), matrix_R.empty() ? nullptr : matrix_R.data()), "build_index");
+    std::vector<uint8_t> core(n_points);
+    check(sdb_find_core(ctx, eps, minPts, core.data(), nullptr), "find_core");
+    std::vector<long long> ptr(n_points + 1);
+    check(sdb_get_neighbors(ctx, ptr.data(), nullptr, nullptr), "neighbors");
+    std::vector<int> idx(std::max<long long>(ptr[n_points], 1));
+    std::vector<float> dd(std::max<long long>(ptr[n_points], 1));
+    check(sdb_get_neighbors(ctx, ptr.data(), idx.data(), dd.data()), "neighbors");
+    std::vector<float> cdist(n_points);
+    check(sdb_core_dist(ctx, std::max(minPts, 2), cdist.data()), "core_dist");
+
+    const float NEG = -2147483648.0f;  // src/Header.h:20
+    std::vector<float> reach(n_points, NEG);
+    std::vector<uint8_t> done(n_points, 0);
+    ordering_.clear();
+    ordering_.reserve(n_points);
+    auto expand = [&](int xi, OMinHeap& seeds) {
+        const float cd = cdist[xi];
+        for (long long j = ptr[xi]; j < ptr[xi + 1]; ++j) {
+            const int xj = idx[j];
+            if (done[xj]) continue;
+            const float r = std::max(cd, dd[j]);
+            if (reach[xj] == NEG || reach[xj] > r) {
+                reach[xj] = r;
+                seeds.emplace(xj, reach[xj]);
+            }
+        }
+    };
+    for (int xi = 0; xi < n_points; ++xi) {
+        if (done[xi]) continue;
+        done[xi] = 1;
+        ordering_.push_back(xi);
+        if (!core[xi]) continue;
+        OMinHeap seeds;
+        expand(xi, seeds);
+        while (!seeds.empty()) {
+            const int xj = seeds.top().idx;
+            seeds.pop();
+            if (done[xj]) continue;
+            done[xj] = 1;
+            ordering_.push_back(xj);
+            if (core[xj]) expand(xj, seeds);
+        }
+    }
+    reachability_.assign(n_points, NEG);
+    check(sdb_optics_reach(ctx, ordering_.data(), reachability_.data()), "optics_reach");
+    if (!output.empty()) {
+        std::ofstream f(out_name(eps, minPts, true));
+        for (int k = 0; k < n_points; ++k) f << ordering_[k] << " " << reachability_[k] << '\n';
+    }
+}
+
+sdb_stats sDbscan::get_stats() const {
+    sdb_stats s{};
+    if (ctx) sdb_get_stats(ctx, &s);
+    return s;
+}
+
+void sDbscan::get_index(std::vector<int>& topK_out, std::vector<int>& topM_out) {
+    ensure_ctx();
+    topK_out.assign((size_t)n_points * 2 * topK, 0);
+    topM_out.assign((size_t)n_proj * 2 * topM, 0);
+    check(sdb_get_index(ctx, topK_out.data(), topM_out.data()), "get_index");
+}

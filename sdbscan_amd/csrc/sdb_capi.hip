@@ -1,0 +1,813 @@
+// C ABI of the sDbscan HIP path: context, device buffers, pipeline order and
+// the host replay of top-K / top-M selections whose boundary is tied.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "../../include/sdbscan_amd.h"
+#include "sdb_kernels.hpp"
+
+namespace sdb {
+int launch_recheck_k(const DistArgs& a, const int* topK, int K, int P, int precision,
+                     double band32, int n_entries, unsigned long long* stats, hipStream_t st);
+int launch_gen_mixture(float* X, long long n, int d, unsigned long long seed, float sigma,
+                       hipStream_t st);
+int launch_csr_fill(const ClusterArgs& c, const long long* ptr, int* cursor, int* idx,
+                    hipStream_t st);
+int launch_csr_dist(const float* X, int ldx, int d, int metric, long long n, const long long* ptr,
+                    const int* idx, float* dist, hipStream_t st);
+int launch_reach(long long n, const long long* ptr, const int* idx, const float* dist,
+                 const uint8_t* core, const float* core_dist, const int* pos, uint32_t* reach_key,
+                 hipStream_t st);
+int launch_kth(long long n, const long long* ptr, const float* sorted, const uint8_t* core,
+               int k, float* out, hipStream_t st);
+}  // namespace sdb
+
+using namespace sdb;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct sdb_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[8];
+    std::string err;
+    sdb_params prm{};
+    bool have_params = false;
+    // input
+    const float* X = nullptr;  // device, n x d
+    DevBuf Xown;
+    long long n = 0;
+    int d = 0;
+    int ldx = 0;
+    // transformed
+    DevBuf Xn, Xb;
+    // index
+    DevBuf bits, R, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
+        cand_idx, topM, flags, lists, is_hub, hub_flags, hub_excl, hub_id, hub_cnt, hub_ptr,
+        hub_cur, hub_mem, dup, inv, F, grp_cnt, grp_ptr, grp_cur, grp, scan_tmp, total, err_flag,
+        row_slot, row_buf;
+    bool indexed = false;
+    int H = 0, L2 = 0, W = 0, S = 0;
+    std::vector<int> h_grp_ptr;
+    // find_core
+    DevBuf E, colsum, queue, qcount, tiles, degree, core, stats64;
+    long long qcap = 0;
+    bool cored = false;
+    float eps = 0.f;
+    int minPts = 0;
+    // clusters
+    DevBuf corehub, parent, border, labels, adj, rootf, rank;
+    // neighbourhoods / optics
+    DevBuf nb_deg, nb_ptr, nb_cur, nb_idx, nb_dist, nb_sorted, core_dist, pos, reach;
+    bool have_nb = false;
+    sdb_stats stats{};
+};
+
+namespace {
+
+int fail(sdb_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (call);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(c, SDB_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+#define LCHK(expr)                                                                       \
+    do {                                                                                 \
+        int _r = (expr);                                                                 \
+        if (_r != 0)                                                                     \
+            return fail(c, _r == -2 || _r == -3 ? SDB_EUNSUPPORTED : SDB_EHIP,           \
+                        "launch %s failed (%d)%s", #expr, _r,                            \
+                        _r > 0 ? hipGetErrorString((hipError_t)_r) : "");                \
+    } while (0)
+
+int ensure(sdb_ctx* c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return 0;
+    if (b.p) { hipFree(b.p); b.p = nullptr; b.bytes = 0; }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, SDB_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return 0;
+}
+
+#define ENSURE(buf, bytes)                    \
+    do {                                      \
+        int _r = ensure(c, (buf), (bytes));   \
+        if (_r) return _r;                    \
+    } while (0)
+
+template <typename T>
+T* P_(DevBuf& b) { return (T*)b.p; }
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+// Heap selection of src/sDbscan.cpp:97-143 / 151-208: min-heaps of size k,
+// elements inserted in index order, replaced only by a strictly larger value.
+struct HPair {
+    int idx;
+    float val;
+    HPair(int i, double v) : idx(i), val((float)v) {}
+    bool operator>(const HPair& o) const { return val > o.val; }
+};
+typedef std::priority_queue<HPair, std::vector<HPair>, std::greater<HPair>> HMin;
+
+void heap_select(const float* v, long long len, int k, int* out_close, int* out_far) {
+    HMin close, far;
+    for (long long j = 0; j < len; ++j) {
+        const float x = v[j];
+        if ((int)close.size() < k) close.emplace((int)j, x);
+        else if (x > close.top().val) { close.pop(); close.emplace((int)j, x); }
+        if ((int)far.size() < k) far.emplace((int)j, -x);
+        else if (-x > far.top().val) { far.pop(); far.emplace((int)j, -x); }
+    }
+    for (int m = k - 1; m >= 0; --m) {
+        out_close[m] = close.top().idx; close.pop();
+        out_far[m] = far.top().idx; far.pop();
+    }
+}
+
+int ilog2(int x) { int l = 0; while ((1 << l) < x) ++l; return l; }
+
+}  // namespace
+
+extern "C" {
+
+int sdb_create(int device, sdb_ctx** out) {
+    if (!out) return SDB_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { (void)hipGetLastError(); return SDB_EHIP; }
+    if (device < 0 || device >= ndev) return SDB_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return SDB_EHIP;
+    sdb_ctx* c = new sdb_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return SDB_EHIP; }
+    for (int i = 0; i < 8; ++i) hipEventCreate(&c->ev[i]);
+    *out = c;
+    return SDB_OK;
+}
+
+void sdb_destroy(sdb_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->st);
+    DevBuf* all[] = {&c->Xown, &c->Xn, &c->Xb, &c->bits, &c->R, &c->topK, &c->tie_idx, &c->tie_vals,
+                     &c->tie_count, &c->cmax, &c->cmin, &c->thi, &c->tlo, &c->cand_cnt, &c->cand_key,
+                     &c->cand_idx, &c->topM, &c->flags, &c->lists, &c->is_hub, &c->hub_flags,
+                     &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
+                     &c->dup, &c->inv, &c->F, &c->grp_cnt, &c->grp_ptr, &c->grp_cur, &c->grp,
+                     &c->scan_tmp, &c->total, &c->err_flag, &c->row_slot, &c->row_buf, &c->E,
+                     &c->colsum, &c->queue, &c->qcount, &c->tiles, &c->degree, &c->core,
+                     &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
+                     &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
+                     &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach};
+    for (DevBuf* b : all)
+        if (b->p) hipFree(b->p);
+    for (int i = 0; i < 8; ++i) hipEventDestroy(c->ev[i]);
+    hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char* sdb_last_error(const sdb_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int sdb_set_params(sdb_ctx* c, const sdb_params* p) {
+    if (!c || !p) return SDB_EINVAL;
+    const sdb_params& q = *p;
+    if (q.n_points <= 0 || q.n_features <= 0) return fail(c, SDB_EINVAL, "n_points and n_features must be positive");
+    if (q.metric < 0 || q.metric > 4) return fail(c, SDB_EINVAL, "unknown metric %d", q.metric);
+    if (q.topK <= 0 || q.topM <= 0 || q.n_proj <= 0) return fail(c, SDB_EINVAL, "topK, topM and n_proj must be positive");
+    if (q.topK > q.n_proj) return fail(c, SDB_EINVAL, "topK (%d) exceeds n_proj (%d)", q.topK, q.n_proj);
+    if (q.topM > q.n_points) return fail(c, SDB_EINVAL, "topM (%d) exceeds n_points", q.topM);
+    if (q.topM > 32 * kMaxW) return fail(c, SDB_EUNSUPPORTED, "topM above %d is not supported", 32 * kMaxW);
+    if (q.topK > 64) return fail(c, SDB_EUNSUPPORTED, "topK above 64 is not supported");
+    if (q.fhtDim < 64 || q.fhtDim > 4096 || (q.fhtDim & (q.fhtDim - 1)))
+        return fail(c, SDB_EUNSUPPORTED, "fhtDim %d outside [64, 4096]", q.fhtDim);
+    if (q.n_proj > q.fhtDim || q.ker_n_features > q.fhtDim)
+        return fail(c, SDB_EINVAL, "n_proj and ker_n_features must not exceed fhtDim");
+    if (q.metric == SDB_COSINE && q.ker_n_features != q.n_features)
+        return fail(c, SDB_EINVAL, "Cosine requires ker_n_features == n_features");
+    if (q.precision != SDB_FP32 && q.precision != SDB_BF16) return fail(c, SDB_EINVAL, "unknown precision");
+    if (q.precision == SDB_BF16 && q.metric != SDB_COSINE)
+        return fail(c, SDB_EUNSUPPORTED, "BF16 distance tiles are implemented for Cosine only");
+    if ((long long)q.n_points * 2 * q.topK >= (1ll << 32)) return fail(c, SDB_EUNSUPPORTED, "n_points * 2 * topK must stay below 2^32");
+    if (2 * q.n_proj >= (1 << 24)) return fail(c, SDB_EUNSUPPORTED, "n_proj too large");
+    c->prm = q;
+    c->have_params = true;
+    c->indexed = false;
+    c->cored = false;
+    c->have_nb = false;
+    return SDB_OK;
+}
+
+static int set_input_dims(sdb_ctx* c, long long n, int d) {
+    if (!c->have_params) return fail(c, SDB_ESTATE, "set_params first");
+    if (n != c->prm.n_points || d != c->prm.n_features)
+        return fail(c, SDB_EINVAL, "input is %lld x %d but params say %lld x %d", n, d, c->prm.n_points, c->prm.n_features);
+    c->n = n;
+    c->d = d;
+    c->ldx = (d + 31) / 32 * 32;
+    c->indexed = c->cored = c->have_nb = false;
+    return 0;
+}
+
+int sdb_load_host(sdb_ctx* c, const float* X, long long n, int d) {
+    if (!c || !X) return SDB_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    int r = set_input_dims(c, n, d);
+    if (r) return r;
+    ENSURE(c->Xown, (size_t)n * d * 4);
+    HIPCHK(hipMemcpyAsync(c->Xown.p, X, (size_t)n * d * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->X = P_<float>(c->Xown);
+    return SDB_OK;
+}
+
+int sdb_load_device(sdb_ctx* c, const float* X, long long n, int d) {
+    if (!c || !X) return SDB_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    int r = set_input_dims(c, n, d);
+    if (r) return r;
+    c->X = X;
+    return SDB_OK;
+}
+
+int sdb_generate(sdb_ctx* c, int kind, long long n, int d, unsigned long long seed, float sigma) {
+    if (!c) return SDB_EINVAL;
+    if (kind != 0) return fail(c, SDB_EINVAL, "unknown generator kind %d", kind);
+    HIPCHK(hipSetDevice(c->device));
+    int r = set_input_dims(c, n, d);
+    if (r) return r;
+    ENSURE(c->Xown, (size_t)n * d * 4);
+    LCHK(launch_gen_mixture(P_<float>(c->Xown), n, d, seed, sigma, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->X = P_<float>(c->Xown);
+    return SDB_OK;
+}
+
+int sdb_copy_input(sdb_ctx* c, long long row0, long long rows, float* out) {
+    if (!c || !out || !c->X) return SDB_EINVAL;
+    if (row0 < 0 || rows < 0 || row0 + rows > c->n) return fail(c, SDB_EINVAL, "row range outside input");
+    HIPCHK(hipMemcpy(out, c->X + (size_t)row0 * c->d, (size_t)rows * c->d * 4, hipMemcpyDeviceToHost));
+    return SDB_OK;
+}
+
+int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
+    if (!c || !hd3_bits) return SDB_EINVAL;
+    if (!c->X) return fail(c, SDB_ESTATE, "no input loaded");
+    HIPCHK(hipSetDevice(c->device));
+    const sdb_params& q = c->prm;
+    const long long n = c->n;
+    const int d = c->d, D = q.fhtDim, P = q.n_proj, K = q.topK, M = q.topM;
+    const int ldx = c->ldx;
+    const bool l12 = q.metric == SDB_L1 || q.metric == SDB_L2;
+    if (l12 && !Rh) return fail(c, SDB_EINVAL, "L1/L2 need the random feature matrix R");
+    c->L2 = 2 * P;
+    c->W = (M + 31) / 32;
+    c->S = 2 * K;
+    c->stats = sdb_stats{};
+    hipStream_t st = c->st;
+
+    HIPCHK(hipEventRecord(c->ev[0], st));
+    // ---- transform (src/Utilities.h:289-335)
+    ENSURE(c->Xn, (size_t)n * ldx * 4);
+    if (q.precision == SDB_BF16) ENSURE(c->Xb, (size_t)n * ldx * 2);
+    ENSURE(c->err_flag, 4);
+    HIPCHK(hipMemsetAsync(c->err_flag.p, 0, 4, st));
+    LCHK(launch_transform(c->X, n, d, ldx, q.metric, P_<float>(c->Xn),
+                          q.precision == SDB_BF16 ? P_<uint16_t>(c->Xb) : nullptr, P_<int>(c->err_flag), st));
+    int errf = 0;
+    HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(c->ev[1], st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
+    if (errf & 2) return fail(c, SDB_EZERO, "Error: There is an zero point !");
+
+    // ---- random structures
+    ENSURE(c->bits, (size_t)3 * D);
+    HIPCHK(hipMemcpyAsync(c->bits.p, hd3_bits, (size_t)3 * D, hipMemcpyHostToDevice, st));
+    if (l12) {
+        ENSURE(c->R, (size_t)(q.ker_n_features / 2) * d * 4);
+        HIPCHK(hipMemcpyAsync(c->R.p, Rh, (size_t)(q.ker_n_features / 2) * d * 4, hipMemcpyHostToDevice, st));
+    }
+
+    // ---- pass 1: projections, top-K, chunk extremes
+    int G = 1024;
+    if (n < 4LL * G) G = (int)std::max<long long>(1, (n + 3) / 4);
+    const int tie_cap = (int)std::min<long long>(n, 65536);
+    ENSURE(c->topK, (size_t)n * 2 * K * 4);
+    ENSURE(c->tie_idx, (size_t)tie_cap * 4);
+    ENSURE(c->tie_vals, (size_t)tie_cap * P * 4);
+    ENSURE(c->tie_count, 4);
+    ENSURE(c->cmax, (size_t)G * P * 4);
+    ENSURE(c->cmin, (size_t)G * P * 4);
+    ENSURE(c->thi, (size_t)P * 4);
+    ENSURE(c->tlo, (size_t)P * 4);
+    const int cap = 4096;
+    ENSURE(c->cand_cnt, (size_t)2 * P * 4);
+    ENSURE(c->cand_key, (size_t)2 * P * cap * 4);
+    ENSURE(c->cand_idx, (size_t)2 * P * cap * 4);
+    ENSURE(c->topM, (size_t)P * 2 * M * 4);
+    ENSURE(c->flags, (size_t)2 * P * 4);
+    HIPCHK(hipMemsetAsync(c->tie_count.p, 0, 4, st));
+    HIPCHK(hipMemsetAsync(c->cand_cnt.p, 0, (size_t)2 * P * 4, st));
+
+    IndexArgs ia{};
+    ia.Xn = P_<float>(c->Xn);
+    ia.n = n; ia.ldx = ldx; ia.d = d; ia.metric = q.metric; ia.ker = q.ker_n_features;
+    ia.kerL = q.ker_intervalSampling;
+    ia.R = l12 ? P_<float>(c->R) : nullptr;
+    ia.bits = P_<uint8_t>(c->bits);
+    ia.P = P; ia.K = K;
+    ia.topK = P_<int>(c->topK);
+    ia.tie_count = P_<int>(c->tie_count); ia.tie_cap = tie_cap;
+    ia.tie_idx = P_<int>(c->tie_idx); ia.tie_vals = P_<float>(c->tie_vals);
+    ia.chunk_max = P_<uint32_t>(c->cmax); ia.chunk_min = P_<uint32_t>(c->cmin);
+    ia.thr_hi = P_<uint32_t>(c->thi); ia.thr_lo = P_<uint32_t>(c->tlo);
+    ia.cand_cnt = P_<int>(c->cand_cnt); ia.cand_cap = cap;
+    ia.cand_key = P_<uint32_t>(c->cand_key); ia.cand_idx = P_<int>(c->cand_idx);
+    const int logD = ilog2(D);
+    LCHK(launch_index(ia, logD, 0, G, st));
+    int ntie = 0;
+    HIPCHK(hipMemcpyAsync(&ntie, c->tie_count.p, 4, hipMemcpyDeviceToHost, st));
+    LCHK(launch_topm_threshold(ia.chunk_max, ia.chunk_min, G, P, M, P_<uint32_t>(c->thi), P_<uint32_t>(c->tlo), st));
+    LCHK(launch_index(ia, logD, 1, G, st));
+    LCHK(launch_topm_select(ia.cand_key, ia.cand_idx, ia.cand_cnt, cap, P, M, P_<int>(c->topM), P_<int>(c->flags), st));
+    std::vector<int> hflags(2 * P);
+    HIPCHK(hipMemcpyAsync(hflags.data(), c->flags.p, (size_t)2 * P * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+
+    // ---- host replay of tied top-K points (reference heap semantics)
+    if (ntie > tie_cap) return fail(c, SDB_EUNSUPPORTED, "%d points with tied top-K (capacity %d)", ntie, tie_cap);
+    if (ntie > 0) {
+        std::vector<int> tidx(ntie);
+        std::vector<float> tv((size_t)ntie * P);
+        HIPCHK(hipMemcpy(tidx.data(), c->tie_idx.p, (size_t)ntie * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(tv.data(), c->tie_vals.p, (size_t)ntie * P * 4, hipMemcpyDeviceToHost));
+        std::vector<int> row(2 * K);
+        for (int t = 0; t < ntie; ++t) {
+            heap_select(tv.data() + (size_t)t * P, P, K, row.data(), row.data() + K);
+            HIPCHK(hipMemcpy(P_<int>(c->topK) + (size_t)tidx[t] * 2 * K, row.data(), (size_t)2 * K * 4, hipMemcpyHostToDevice));
+        }
+    }
+    c->stats.n_tie_points = ntie;
+
+    // ---- host replay of flagged top-M rows
+    std::vector<int> rows;
+    for (int r = 0; r < P; ++r)
+        if (hflags[2 * r] || hflags[2 * r + 1]) rows.push_back(r);
+    c->stats.n_tie_rows = (long long)rows.size();
+    if (!rows.empty()) {
+        const int batch = (int)std::max<long long>(1, std::min<long long>(16, (1ll << 31) / (n * 4)));
+        ENSURE(c->row_slot, (size_t)P * 4);
+        ENSURE(c->row_buf, (size_t)batch * n * 4);
+        std::vector<int> slot(P);
+        std::vector<float> buf((size_t)batch * n);
+        std::vector<int> out(2 * M);
+        for (size_t b0 = 0; b0 < rows.size(); b0 += batch) {
+            const size_t b1 = std::min(rows.size(), b0 + batch);
+            std::fill(slot.begin(), slot.end(), -1);
+            for (size_t i = b0; i < b1; ++i) slot[rows[i]] = (int)(i - b0);
+            HIPCHK(hipMemcpyAsync(c->row_slot.p, slot.data(), (size_t)P * 4, hipMemcpyHostToDevice, st));
+            ia.row_slot = P_<int>(c->row_slot);
+            ia.row_buf = P_<float>(c->row_buf);
+            LCHK(launch_index(ia, logD, 2, G, st));
+            HIPCHK(hipMemcpyAsync(buf.data(), c->row_buf.p, (b1 - b0) * n * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (size_t i = b0; i < b1; ++i) {
+                heap_select(buf.data() + (i - b0) * n, n, M, out.data(), out.data() + M);
+                HIPCHK(hipMemcpy(P_<int>(c->topM) + (size_t)rows[i] * 2 * M, out.data(), (size_t)2 * M * 4, hipMemcpyHostToDevice));
+            }
+        }
+    }
+
+    // ---- list structures, hubs, first-occurrence masks, groups
+    const int L2 = c->L2, W = c->W, S = c->S;
+    const int LM = L2 * M;
+    ENSURE(c->lists, (size_t)LM * 4);
+    ENSURE(c->is_hub, (size_t)n);
+    ENSURE(c->hub_flags, (size_t)n * 4);
+    ENSURE(c->hub_excl, (size_t)n * 4);
+    ENSURE(c->hub_id, (size_t)n * 4);
+    ENSURE(c->total, 16);
+    size_t stb = std::max(scan_tmp_bytes(n), scan_tmp_bytes((long long)std::max(L2, LM) + 1));
+    ENSURE(c->scan_tmp, stb);
+    HIPCHK(hipMemsetAsync(c->is_hub.p, 0, (size_t)n, st));
+    LCHK(launch_build_lists(P_<int>(c->topM), P, M, P_<int>(c->lists), P_<uint8_t>(c->is_hub), st));
+    LCHK(launch_is_hub_scan_prep(P_<uint8_t>(c->is_hub), P_<int>(c->hub_flags), n, st));
+    LCHK(scan_exclusive_i32(P_<int>(c->hub_flags), P_<int>(c->hub_excl), n, P_<int>(c->total), c->scan_tmp.p, c->scan_tmp.bytes, st));
+    LCHK(launch_hub_id(P_<uint8_t>(c->is_hub), P_<int>(c->hub_excl), P_<int>(c->hub_id), n, st));
+    int H = 0;
+    HIPCHK(hipMemcpyAsync(&H, c->total.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->H = H;
+    c->stats.n_hubs = H;
+    ENSURE(c->hub_cnt, (size_t)(H + 1) * 4);
+    ENSURE(c->hub_ptr, (size_t)(H + 1) * 4);
+    ENSURE(c->hub_cur, (size_t)(H + 1) * 4);
+    ENSURE(c->hub_mem, (size_t)LM * 4);
+    HIPCHK(hipMemsetAsync(c->hub_cnt.p, 0, (size_t)(H + 1) * 4, st));
+    HIPCHK(hipMemsetAsync(c->hub_cur.p, 0, (size_t)(H + 1) * 4, st));
+    LCHK(launch_hub_count(P_<int>(c->lists), LM, P_<int>(c->hub_id), P_<int>(c->hub_cnt), st));
+    LCHK(scan_exclusive_i32(P_<int>(c->hub_cnt), P_<int>(c->hub_ptr), H + 1, nullptr, c->scan_tmp.p, c->scan_tmp.bytes, st));
+    LCHK(launch_hub_fill(P_<int>(c->lists), LM, P_<int>(c->hub_id), P_<int>(c->hub_ptr), P_<int>(c->hub_cur), P_<int>(c->hub_mem), st));
+    const size_t tbl = (size_t)L2 * L2 * W * 4;
+    ENSURE(c->dup, tbl);
+    ENSURE(c->inv, tbl);
+    HIPCHK(hipMemsetAsync(c->dup.p, 0, tbl, st));
+    HIPCHK(hipMemsetAsync(c->inv.p, 0, tbl, st));
+    LCHK(launch_dupmask(P_<int>(c->hub_ptr), P_<int>(c->hub_mem), H, M, L2, W, P_<uint32_t>(c->dup), st));
+    LCHK(launch_invmask(P_<int>(c->lists), P_<int>(c->topK), P, K, M, L2, W, P_<uint32_t>(c->inv), st));
+    ENSURE(c->F, (size_t)n * S * W * 4);
+    ENSURE(c->grp_cnt, (size_t)(L2 + 1) * 4);
+    ENSURE(c->grp_ptr, (size_t)(L2 + 1) * 4);
+    ENSURE(c->grp_cur, (size_t)(L2 + 1) * 4);
+    ENSURE(c->grp, (size_t)n * S * 4);
+    HIPCHK(hipMemsetAsync(c->grp_cnt.p, 0, (size_t)(L2 + 1) * 4, st));
+    HIPCHK(hipMemsetAsync(c->grp_cur.p, 0, (size_t)(L2 + 1) * 4, st));
+    LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, L2, W, P_<uint32_t>(c->dup), P_<int>(c->hub_id),
+                           P_<int>(c->hub_ptr), P_<int>(c->hub_mem), P_<uint32_t>(c->F), P_<int>(c->grp_cnt), st));
+    LCHK(scan_exclusive_i32(P_<int>(c->grp_cnt), P_<int>(c->grp_ptr), L2 + 1, nullptr, c->scan_tmp.p, c->scan_tmp.bytes, st));
+    LCHK(launch_group_fill(P_<int>(c->topK), n, P, K, P_<int>(c->grp_ptr), P_<int>(c->grp_cur), P_<uint32_t>(c->grp), st));
+    c->h_grp_ptr.resize(L2 + 1);
+    HIPCHK(hipMemcpyAsync(c->h_grp_ptr.data(), c->grp_ptr.p, (size_t)(L2 + 1) * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(c->ev[2], st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->stats.t_transform_ms = elapsed(c->ev[0], c->ev[1]);
+    c->stats.t_index_ms = elapsed(c->ev[1], c->ev[2]);
+    c->indexed = true;
+    c->cored = false;
+    c->have_nb = false;
+    return SDB_OK;
+}
+
+int sdb_get_index(sdb_ctx* c, int* topK_out, int* topM_out) {
+    if (!c) return SDB_EINVAL;
+    if (!c->indexed) return fail(c, SDB_ESTATE, "build_index first");
+    HIPCHK(hipSetDevice(c->device));
+    if (topK_out) HIPCHK(hipMemcpy(topK_out, c->topK.p, (size_t)c->n * 2 * c->prm.topK * 4, hipMemcpyDeviceToHost));
+    if (topM_out) HIPCHK(hipMemcpy(topM_out, c->topM.p, (size_t)c->prm.n_proj * 2 * c->prm.topM * 4, hipMemcpyDeviceToHost));
+    return SDB_OK;
+}
+
+int sdb_get_transformed(sdb_ctx* c, float* X_out) {
+    if (!c || !X_out) return SDB_EINVAL;
+    if (!c->indexed) return fail(c, SDB_ESTATE, "build_index first");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpy2D(X_out, (size_t)c->d * 4, c->Xn.p, (size_t)c->ldx * 4, (size_t)c->d * 4, (size_t)c->n, hipMemcpyDeviceToHost));
+    return SDB_OK;
+}
+
+// Error band of the fast distance, derived from the standard bound
+// |fl(sum) - sum| <= gamma_k * sum |terms| (any summation order, one rounding of
+// at most 2^-23 relative per operation), with a safety factor of 2.
+static void compute_bands(const sdb_params& q, float eps, double* band, double* band32) {
+    const double u = std::ldexp(1.0, -23);
+    const double d = q.n_features;
+    const double Smax = 1.0 + 1e-5;  // sum |x_i y_i| <= ||x|| ||y|| for unit rows
+    const double b32 = 2.0 * ((d + 2.0) * u * Smax + 2.0 * u);
+    *band32 = b32;
+    switch (q.metric) {
+        case SDB_COSINE:
+            if (q.precision == SDB_BF16) {
+                const double ub = std::ldexp(1.0, -8);
+                *band = 2.0 * ((2.0 * ub + ub * ub) * Smax + (d + 2.0) * u * Smax * (1.0 + 2.0 * ub) + 2.0 * u);
+            } else {
+                *band = b32;
+            }
+            break;
+        case SDB_L2: *band = 2.0 * (double)eps * (d + 8.0) * u + 1e-30; break;
+        case SDB_L1: *band = 2.0 * (double)eps * (d + 4.0) * u + 1e-30; break;
+        case SDB_CHI2: *band = 2.0 * (d + 8.0) * u; break;
+        default: *band = 2.0 * (2.0 * d + 200.0) * u; break;
+    }
+}
+
+int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* degree_out) {
+    if (!c) return SDB_EINVAL;
+    if (!c->indexed) return fail(c, SDB_ESTATE, "build_index first");
+    HIPCHK(hipSetDevice(c->device));
+    const sdb_params& q = c->prm;
+    const long long n = c->n;
+    const int M = q.topM, W = c->W, S = c->S, L2 = c->L2;
+    hipStream_t st = c->st;
+    const bool mfma = q.metric == SDB_COSINE && M <= 128;
+    const int prec = mfma ? q.precision : SDB_FP32;
+
+    // tiles: contiguous ranges of one list's group
+    int QB;
+    DistArgs da{};
+    da.ldx = c->ldx; da.d = c->d; da.metric = q.metric;
+    da.M = M; da.W = W; da.S = S;
+    if (mfma) {
+        const int SE = prec == SDB_BF16 ? 32 : 16;
+        da.dk = (c->d + SE - 1) / SE * SE;
+        const int steps = (int)(((kLdsBytes - 4 * M) / M - 16) / 64);
+        if (steps <= 0) return fail(c, SDB_EUNSUPPORTED, "topM too large for the LDS tile");
+        da.kc = std::min(da.dk, steps * SE);
+        da.nchunk = (da.dk + da.kc - 1) / da.kc;
+        QB = da.nchunk == 1 ? 1024 : 256;
+    } else {
+        int kc = (kLdsBytes - 8 * M) / (4 * M + 128);
+        kc = std::min(kc, c->d);
+        if (kc <= 0) return fail(c, SDB_EUNSUPPORTED, "topM too large for the LDS tile");
+        da.kc = kc;
+        da.nchunk = (c->d + kc - 1) / kc;
+        da.dk = c->d;
+        QB = 256;
+    }
+    std::vector<int> tl, tb, te;
+    for (int l = 0; l < L2; ++l)
+        for (int b = c->h_grp_ptr[l]; b < c->h_grp_ptr[l + 1]; b += QB) {
+            tl.push_back(l);
+            tb.push_back(b);
+            te.push_back(std::min(b + QB, c->h_grp_ptr[l + 1]));
+        }
+    const int ntiles = (int)tl.size();
+    ENSURE(c->tiles, (size_t)3 * std::max(ntiles, 1) * 4);
+    int* dt = P_<int>(c->tiles);
+    HIPCHK(hipMemcpyAsync(dt, tl.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dt + ntiles, tb.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dt + 2 * ntiles, te.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice, st));
+
+    ENSURE(c->E, (size_t)n * S * W * 4);
+    ENSURE(c->colsum, (size_t)L2 * M * 4);
+    ENSURE(c->qcount, 4);
+    ENSURE(c->stats64, 64);
+    if (c->qcap == 0) c->qcap = 1 << 22;
+    ENSURE(c->queue, (size_t)c->qcap * 8);
+
+    double band = 0, band32 = 0;
+    compute_bands(q, eps, &band, &band32);
+    da.X = prec == SDB_BF16 ? (const void*)c->Xb.p : (const void*)c->Xn.p;
+    da.X32 = P_<float>(c->Xn);
+    da.grp = P_<uint32_t>(c->grp);
+    da.tile_list = dt; da.tile_beg = dt + ntiles; da.tile_end = dt + 2 * ntiles;
+    da.lists = P_<int>(c->lists);
+    da.F = P_<uint32_t>(c->F);
+    da.E = P_<uint32_t>(c->E);
+    da.colsum = P_<int>(c->colsum);
+    da.eps = eps;
+    da.band = band;
+
+    HIPCHK(hipEventRecord(c->ev[3], st));
+    int nq = 0;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        da.queue = (unsigned long long*)c->queue.p;
+        da.qcount = P_<int>(c->qcount);
+        da.qcap = (int)c->qcap;
+        HIPCHK(hipMemsetAsync(c->colsum.p, 0, (size_t)L2 * M * 4, st));
+        HIPCHK(hipMemsetAsync(c->qcount.p, 0, 4, st));
+        HIPCHK(hipEventRecord(c->ev[4], st));
+        if (mfma) LCHK(launch_dist_mfma(da, prec, ntiles, st));
+        else LCHK(launch_dist_valu(da, ntiles, st));
+        HIPCHK(hipEventRecord(c->ev[5], st));
+        HIPCHK(hipMemcpyAsync(&nq, c->qcount.p, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (nq <= c->qcap) break;
+        c->qcap = (long long)nq * 2;
+        if (c->qcap > (1ll << 30)) return fail(c, SDB_ENOMEM, "recheck queue of %d pairs is too large", nq);
+        ENSURE(c->queue, (size_t)c->qcap * 8);
+    }
+    if (nq > c->qcap) return fail(c, SDB_ENOMEM, "recheck queue overflow");
+    c->stats.t_dist_ms = elapsed(c->ev[4], c->ev[5]);
+    HIPCHK(hipMemsetAsync(c->stats64.p, 0, 64, st));
+    LCHK(launch_recheck_k(da, P_<int>(c->topK), q.topK, q.n_proj, prec, band32, nq, (unsigned long long*)c->stats64.p, st));
+
+    // degrees and core flags
+    ENSURE(c->degree, (size_t)n * 4);
+    ENSURE(c->core, (size_t)n);
+    ClusterArgs ca{};
+    ca.n = n; ca.S = S; ca.W = W; ca.M = M; ca.P = q.n_proj; ca.K = q.topK; ca.L2 = L2;
+    ca.topK = P_<int>(c->topK); ca.lists = P_<int>(c->lists); ca.E = P_<uint32_t>(c->E);
+    ca.colsum = P_<int>(c->colsum); ca.hub_id = P_<int>(c->hub_id); ca.hub_ptr = P_<int>(c->hub_ptr);
+    ca.hub_mem = P_<int>(c->hub_mem); ca.inv = P_<uint32_t>(c->inv); ca.minPts = minPts;
+    ca.degree = P_<int>(c->degree); ca.core = P_<uint8_t>(c->core);
+    ca.n_core = (unsigned long long*)c->stats64.p + 3;
+    LCHK(launch_degree(ca, st));
+    HIPCHK(hipEventRecord(c->ev[6], st));
+    unsigned long long hs[8] = {0};
+    HIPCHK(hipMemcpyAsync(hs, c->stats64.p, 64, hipMemcpyDeviceToHost, st));
+    if (core_out) HIPCHK(hipMemcpyAsync(core_out, c->core.p, (size_t)n, hipMemcpyDeviceToHost, st));
+    if (degree_out) HIPCHK(hipMemcpyAsync(degree_out, c->degree.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->stats.t_core_ms = elapsed(c->ev[3], c->ev[6]);
+    c->stats.n_pairs = (long long)n * S * M;
+    c->stats.n_queue = nq;
+    c->stats.n_recheck32 = (long long)hs[0];
+    c->stats.n_recheck64 = (long long)hs[1];
+    c->stats.n_recheck_hits = (long long)hs[2];
+    c->stats.n_core = (long long)hs[3];
+    c->stats.band = band;
+    c->stats.band32 = band32;
+    c->eps = eps;
+    c->minPts = minPts;
+    c->cored = true;
+    c->have_nb = false;
+    return SDB_OK;
+}
+
+static ClusterArgs cluster_args(sdb_ctx* c) {
+    const sdb_params& q = c->prm;
+    ClusterArgs ca{};
+    ca.n = c->n; ca.S = c->S; ca.W = c->W; ca.M = q.topM; ca.P = q.n_proj; ca.K = q.topK; ca.L2 = c->L2;
+    ca.topK = P_<int>(c->topK); ca.lists = P_<int>(c->lists); ca.E = P_<uint32_t>(c->E);
+    ca.colsum = P_<int>(c->colsum); ca.hub_id = P_<int>(c->hub_id); ca.hub_ptr = P_<int>(c->hub_ptr);
+    ca.hub_mem = P_<int>(c->hub_mem); ca.inv = P_<uint32_t>(c->inv); ca.minPts = c->minPts;
+    ca.degree = P_<int>(c->degree); ca.core = P_<uint8_t>(c->core);
+    ca.corehub = P_<uint32_t>(c->corehub); ca.parent = P_<int>(c->parent);
+    ca.border_min = P_<int>(c->border); ca.labels = P_<int>(c->labels); ca.adj = P_<uint32_t>(c->adj);
+    return ca;
+}
+
+int sdb_form_clusters(sdb_ctx* c, int* labels_out, int* n_clusters_out) {
+    if (!c) return SDB_EINVAL;
+    if (!c->cored) return fail(c, SDB_ESTATE, "find_core first");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->st;
+    const long long n = c->n;
+    const int M = c->prm.topM, W = c->W, L2 = c->L2;
+    ENSURE(c->corehub, (size_t)L2 * W * 4);
+    ENSURE(c->parent, (size_t)n * 4);
+    ENSURE(c->border, (size_t)n * 4);
+    ENSURE(c->labels, (size_t)n * 4);
+    ENSURE(c->adj, (size_t)L2 * M * W * 4);
+    ENSURE(c->rootf, (size_t)n * 4);
+    ENSURE(c->rank, (size_t)n * 4);
+    ClusterArgs ca = cluster_args(c);
+    HIPCHK(hipEventRecord(c->ev[0], st));
+    HIPCHK(hipMemsetAsync(c->adj.p, 0, (size_t)L2 * M * W * 4, st));
+    LCHK(launch_corehub(ca, st));
+    LCHK(launch_uf_init(ca, st));
+    LCHK(launch_uf_slots(ca, st));
+    LCHK(launch_uf_lists(ca, st));
+    LCHK(launch_uf_compress(ca, st));
+    LCHK(launch_root_flags(ca, P_<int>(c->rootf), st));
+    LCHK(scan_exclusive_i32(P_<int>(c->rootf), P_<int>(c->rank), n, P_<int>(c->total), c->scan_tmp.p, c->scan_tmp.bytes, st));
+    LCHK(launch_border(ca, st));
+    LCHK(launch_labels(ca, P_<int>(c->rank), st));
+    HIPCHK(hipEventRecord(c->ev[1], st));
+    int comps = 0;
+    HIPCHK(hipMemcpyAsync(&comps, c->total.p, 4, hipMemcpyDeviceToHost, st));
+    if (labels_out) HIPCHK(hipMemcpyAsync(labels_out, c->labels.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->stats.t_cluster_ms = elapsed(c->ev[0], c->ev[1]);
+    c->stats.n_components = comps;
+    if (n_clusters_out) *n_clusters_out = comps == 0 ? 2 : comps + 1;  // src/sDbscan.cpp:633
+    return SDB_OK;
+}
+
+static int build_neighbors(sdb_ctx* c) {
+    if (c->have_nb) return 0;
+    hipStream_t st = c->st;
+    const long long n = c->n;
+    ENSURE(c->nb_ptr, (size_t)(n + 1) * 8);
+    ENSURE(c->nb_cur, (size_t)n * 4);
+    // ptr = exclusive scan of the degrees (64-bit)
+    std::vector<int> deg(n);
+    HIPCHK(hipMemcpyAsync(deg.data(), c->degree.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<long long> ptr(n + 1, 0);
+    for (long long a = 0; a < n; ++a) ptr[a + 1] = ptr[a] + deg[a];
+    const long long tot = ptr[n];
+    HIPCHK(hipMemcpyAsync(c->nb_ptr.p, ptr.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(c->nb_cur.p, 0, (size_t)n * 4, st));
+    ENSURE(c->nb_idx, (size_t)std::max<long long>(tot, 1) * 4);
+    ENSURE(c->nb_dist, (size_t)std::max<long long>(tot, 1) * 4);
+    ENSURE(c->nb_sorted, (size_t)std::max<long long>(tot, 1) * 8);
+    ClusterArgs ca = cluster_args(c);
+    LCHK(launch_csr_fill(ca, P_<long long>(c->nb_ptr), P_<int>(c->nb_cur), P_<int>(c->nb_idx), st));
+    {
+        std::vector<int> cur(n);
+        HIPCHK(hipMemcpyAsync(cur.data(), c->nb_cur.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (long long a = 0; a < n; ++a)
+            if (cur[a] != deg[a])
+                return fail(c, SDB_EHIP, "neighbourhood of point %lld has %d entries, degree says %d", a, cur[a], deg[a]);
+    }
+    // sort every neighbourhood by index (segmented radix sort of the ids)
+    if (tot > 0) {
+        int* keys_out = (int*)c->nb_sorted.p;
+        size_t tb = 0;
+        hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, P_<int>(c->nb_idx), keys_out, (int)tot, (int)n,
+                                                   P_<long long>(c->nb_ptr), P_<long long>(c->nb_ptr) + 1, 0, 32, st);
+        DevBuf tmp;
+        if (ensure(c, tmp, tb)) return SDB_ENOMEM;
+        hipError_t e = hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, tb, P_<int>(c->nb_idx), keys_out, (int)tot, (int)n,
+                                                                  P_<long long>(c->nb_ptr), P_<long long>(c->nb_ptr) + 1, 0, 32, st);
+        HIPCHK(hipMemcpyAsync(c->nb_idx.p, keys_out, (size_t)tot * 4, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        hipFree(tmp.p);
+        if (e != hipSuccess) return fail(c, SDB_EHIP, "segmented sort failed: %s", hipGetErrorString(e));
+        LCHK(launch_csr_dist(P_<float>(c->Xn), c->ldx, c->d, c->prm.metric, n, P_<long long>(c->nb_ptr),
+                             P_<int>(c->nb_idx), P_<float>(c->nb_dist), st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    c->have_nb = true;
+    return 0;
+}
+
+int sdb_get_neighbors(sdb_ctx* c, long long* ptr, int* idx, float* dist) {
+    if (!c || !ptr) return SDB_EINVAL;
+    if (!c->cored) return fail(c, SDB_ESTATE, "find_core first");
+    HIPCHK(hipSetDevice(c->device));
+    int r = build_neighbors(c);
+    if (r) return r;
+    const long long n = c->n;
+    HIPCHK(hipMemcpy(ptr, c->nb_ptr.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToHost));
+    const long long tot = ptr[n];
+    if (idx && tot) HIPCHK(hipMemcpy(idx, c->nb_idx.p, (size_t)tot * 4, hipMemcpyDeviceToHost));
+    if (dist && tot) HIPCHK(hipMemcpy(dist, c->nb_dist.p, (size_t)tot * 4, hipMemcpyDeviceToHost));
+    return SDB_OK;
+}
+
+int sdb_core_dist(sdb_ctx* c, int minPts, float* core_dist_out) {
+    if (!c || !core_dist_out) return SDB_EINVAL;
+    if (!c->cored) return fail(c, SDB_ESTATE, "find_core first");
+    if (minPts < 2) return fail(c, SDB_EINVAL, "minPts must be at least 2 for core distances");
+    HIPCHK(hipSetDevice(c->device));
+    int r = build_neighbors(c);
+    if (r) return r;
+    hipStream_t st = c->st;
+    const long long n = c->n;
+    long long tot = 0;
+    HIPCHK(hipMemcpy(&tot, P_<long long>(c->nb_ptr) + n, 8, hipMemcpyDeviceToHost));
+    ENSURE(c->core_dist, (size_t)n * 4);
+    float* sorted = (float*)c->nb_sorted.p;
+    if (tot > 0) {
+        size_t tb = 0;
+        hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, P_<float>(c->nb_dist), sorted, (int)tot, (int)n,
+                                                   P_<long long>(c->nb_ptr), P_<long long>(c->nb_ptr) + 1, 0, 32, st);
+        DevBuf tmp;
+        if (ensure(c, tmp, tb)) return SDB_ENOMEM;
+        hipError_t e = hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, tb, P_<float>(c->nb_dist), sorted, (int)tot, (int)n,
+                                                                  P_<long long>(c->nb_ptr), P_<long long>(c->nb_ptr) + 1, 0, 32, st);
+        HIPCHK(hipStreamSynchronize(st));
+        hipFree(tmp.p);
+        if (e != hipSuccess) return fail(c, SDB_EHIP, "segmented sort failed: %s", hipGetErrorString(e));
+    }
+    LCHK(launch_kth(n, P_<long long>(c->nb_ptr), sorted, P_<uint8_t>(c->core), minPts - 2, P_<float>(c->core_dist), st));
+    HIPCHK(hipMemcpyAsync(core_dist_out, c->core_dist.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return SDB_OK;
+}
+
+int sdb_optics_reach(sdb_ctx* c, const int* ordering, float* reach_out) {
+    if (!c || !ordering || !reach_out) return SDB_EINVAL;
+    if (!c->have_nb || !c->core_dist.p) return fail(c, SDB_ESTATE, "core_dist first");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->st;
+    const long long n = c->n;
+    std::vector<int> pos(n, -1);
+    for (long long k = 0; k < n; ++k) {
+        int p = ordering[k];
+        if (p < 0 || p >= n || pos[p] >= 0) return fail(c, SDB_EINVAL, "ordering is not a permutation");
+        pos[p] = (int)k;
+    }
+    ENSURE(c->pos, (size_t)n * 4);
+    ENSURE(c->reach, (size_t)n * 4);
+    HIPCHK(hipMemcpyAsync(c->pos.p, pos.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+    LCHK(launch_reach(n, P_<long long>(c->nb_ptr), P_<int>(c->nb_idx), P_<float>(c->nb_dist), P_<uint8_t>(c->core),
+                      P_<float>(c->core_dist), P_<int>(c->pos), P_<uint32_t>(c->reach), st));
+    HIPCHK(hipMemcpyAsync(reach_out, c->reach.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return SDB_OK;
+}
+
+int sdb_get_stats(const sdb_ctx* c, sdb_stats* out) {
+    if (!c || !out) return SDB_EINVAL;
+    *out = c->stats;
+    return SDB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,284 @@
+// Degrees, core test and cluster labels of sDbscan on gfx950.
+//
+// Reference: the dedup + core test of rp_findCorePoints
+// (src/sDbscan.cpp:374-400) and formCluster (src/sDbscan.cpp:509-637).
+//
+// Neighbourhood of a point a in the reference: N(a) = O(a) u R(a) with
+// O(a) = {c in Cand(a) : hit(a,c)} and R(a) = {b : a in Cand(b), hit(a,b)}
+// (edges are pushed both ways, then deduplicated).  Here
+//   |O(a)| = popcount of a's first-occurrence hit masks E,
+//   |R(a)| = sum of the list-position hit counts (colsum) over a's list
+//            memberships (non-zero only for points that sit in a top-M list),
+//   |O(a) n R(a)| = hits c of a for which a is itself a candidate of c,
+//            found with the inv masks.
+// Labels follow formCluster exactly: clusters are the components of the
+// core graph numbered by their smallest core index, a border point takes the
+// smallest cluster id among its core neighbours, everything else is -1.
+#include <hipcub/hipcub.hpp>
+#include "sdb_kernels.hpp"
+
+namespace sdb {
+
+__device__ __forceinline__ int slot_list_c(const int* topK, long long a, int s, int K, int P) {
+    int k = s >> 1;
+    return (s & 1) ? P + topK[a * 2 * K + K + k] : topK[a * 2 * K + k];
+}
+
+__device__ int point_degree_impl(const ClusterArgs& c, long long a);
+__device__ __forceinline__ int point_degree(const ClusterArgs& c, long long a) { return point_degree_impl(c, a); }
+
+__global__ void k_degree(ClusterArgs c) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = a < c.n;
+    int deg = 0;
+    if (live) deg = point_degree(c, a);
+    const bool is_core = live && deg >= c.minPts - 1;
+    if (live) {
+        c.degree[a] = deg;
+        c.core[a] = is_core ? 1 : 0;
+    }
+    const unsigned long long b = __ballot(is_core);
+    if (lane_id() == 0 && b) atomicAdd(c.n_core, (unsigned long long)__popcll(b));
+}
+
+__device__ int point_degree_impl(const ClusterArgs& c, long long a) {
+    int own = 0;
+    const uint32_t* Ea = c.E + (size_t)a * c.S * c.W;
+    for (int q = 0; q < c.S * c.W; ++q) own += __popc(Ea[q]);
+    int rev = 0, mutual = 0;
+    const int h = c.hub_id[a];
+    if (h >= 0) {
+        const int hb = c.hub_ptr[h], he = c.hub_ptr[h + 1];
+        for (int q = hb; q < he; ++q) rev += c.colsum[c.hub_mem[q]];
+        for (int s = 0; s < c.S; ++s) {
+            const int l = slot_list_c(c.topK, a, s, c.K, c.P);
+            for (int w = 0; w < c.W; ++w) {
+                uint32_t e = Ea[s * c.W + w];
+                if (!e) continue;
+                uint32_t m = 0;
+                for (int q = hb; q < he; ++q) {
+                    const int L = c.hub_mem[q] / c.M;
+                    m |= c.inv[((size_t)l * c.L2 + L) * c.W + w];
+                }
+                mutual += __popc(e & m);
+            }
+        }
+    }
+    return own + rev - mutual;
+}
+
+// corehub[l][w]: bit i set when the point at position i of list l is core.
+__global__ void k_corehub(ClusterArgs c) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.L2 * c.W) return;
+    int l = t / c.W, w = t % c.W;
+    uint32_t m = 0;
+    for (int b = 0; b < 32; ++b) {
+        int i = w * 32 + b;
+        if (i < c.M && c.core[c.lists[(size_t)l * c.M + i]]) m |= 1u << b;
+    }
+    c.corehub[t] = m;
+}
+
+// Union-find with parent[x] <= x: roots are hooked under the smaller root, so
+// every component ends with its smallest member as root.
+__device__ __forceinline__ int uf_find(int* par, int x) {
+    int cur = load_relaxed(&par[x]);
+    if (cur == x) return x;
+    int prev = x;
+    while (true) {
+        int nxt = load_relaxed(&par[cur]);
+        if (nxt == cur) break;
+        __hip_atomic_store(&par[prev], nxt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        prev = cur;
+        cur = nxt;
+    }
+    return cur;
+}
+
+__device__ __forceinline__ void uf_union(int* par, int a, int b) {
+    int ra = uf_find(par, a), rb = uf_find(par, b);
+    while (ra != rb) {
+        if (ra < rb) { int t = ra; ra = rb; rb = t; }
+        int old = atomicCAS(&par[ra], ra, rb);
+        if (old == ra) return;
+        ra = uf_find(par, old);
+        rb = uf_find(par, rb);
+    }
+}
+
+__global__ void k_uf_init(ClusterArgs c) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= c.n) return;
+    c.parent[a] = (int)a;
+    c.border_min[a] = 0x7fffffff;
+}
+
+// Per (core point, slot): all core hits of the slot's list join the point.
+// The point is united with the first of them; the list-local star edges
+// (first -> all) are recorded in adj and united by k_uf_lists.
+__global__ void k_uf_slots(ClusterArgs c) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.n * c.S) return;
+    const long long a = t / c.S;
+    if (!c.core[a]) return;
+    const int s = (int)(t % c.S);
+    const int l = slot_list_c(c.topK, a, s, c.K, c.P);
+    uint32_t m[kMaxW];
+    int first = -1;
+    for (int w = 0; w < c.W; ++w) {
+        m[w] = c.E[(size_t)t * c.W + w] & c.corehub[(size_t)l * c.W + w];
+        if (first < 0 && m[w]) first = w * 32 + __ffs(m[w]) - 1;
+    }
+    if (first < 0) return;
+    uf_union(c.parent, (int)a, c.lists[(size_t)l * c.M + first]);
+    uint32_t* ad = c.adj + ((size_t)l * c.M + first) * c.W;
+    for (int w = 0; w < c.W; ++w)
+        if (m[w] && (load_relaxed((const int*)&ad[w]) & m[w]) != m[w]) atomicOr(&ad[w], m[w]);
+}
+
+__global__ void k_uf_lists(ClusterArgs c) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.L2 * c.M) return;
+    const int l = t / c.M;
+    const int f = c.lists[t];
+    for (int w = 0; w < c.W; ++w) {
+        uint32_t m = c.adj[(size_t)t * c.W + w];
+        while (m) {
+            int b = __ffs(m) - 1;
+            m &= m - 1;
+            uf_union(c.parent, f, c.lists[(size_t)l * c.M + w * 32 + b]);
+        }
+    }
+}
+
+__global__ void k_uf_compress(ClusterArgs c) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= c.n) return;
+    if (c.core[a]) c.parent[a] = uf_find(c.parent, (int)a);
+}
+
+__global__ void k_root_flags(ClusterArgs c, int* flags) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= c.n) return;
+    flags[a] = (c.core[a] && c.parent[a] == (int)a) ? 1 : 0;
+}
+
+// Border points: the smallest root among adjacent core points, from both
+// directions of every hit pair.
+__global__ void k_border(ClusterArgs c) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.n * c.S) return;
+    const long long a = t / c.S;
+    const int s = (int)(t % c.S);
+    const int l = slot_list_c(c.topK, a, s, c.K, c.P);
+    const bool ca = c.core[a] != 0;
+    int best = 0x7fffffff;
+    for (int w = 0; w < c.W; ++w) {
+        const uint32_t e = c.E[(size_t)t * c.W + w];
+        if (!e) continue;
+        const uint32_t ch = c.corehub[(size_t)l * c.W + w];
+        uint32_t m = ca ? (e & ~ch) : (e & ch);
+        while (m) {
+            const int b = __ffs(m) - 1;
+            m &= m - 1;
+            const int cp = c.lists[(size_t)l * c.M + w * 32 + b];
+            if (ca) {
+                const int r = c.parent[a];
+                if (load_relaxed(&c.border_min[cp]) > r) atomicMin(&c.border_min[cp], r);
+            } else {
+                const int r = c.parent[cp];
+                best = r < best ? r : best;
+            }
+        }
+    }
+    if (!ca && best != 0x7fffffff && load_relaxed(&c.border_min[a]) > best) atomicMin(&c.border_min[a], best);
+}
+
+__global__ void k_labels(ClusterArgs c, const int* rank) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= c.n) return;
+    int lab;
+    if (c.core[a]) lab = rank[c.parent[a]];
+    else lab = c.border_min[a] == 0x7fffffff ? -1 : rank[c.border_min[a]];
+    c.labels[a] = lab;
+}
+
+static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int launch_degree(const ClusterArgs& c, hipStream_t st) {
+    hipLaunchKernelGGL(k_degree, dim3(nblk(c.n, 256)), dim3(256), 0, st, c);
+    return (int)hipGetLastError();
+}
+int launch_corehub(const ClusterArgs& c, hipStream_t st) {
+    hipLaunchKernelGGL(k_corehub, dim3(nblk((long long)c.L2 * c.W, 256)), dim3(256), 0, st, c);
+    return (int)hipGetLastError();
+}
+int launch_uf_init(const ClusterArgs& c, hipStream_t st) {
+    hipLaunchKernelGGL(k_uf_init, dim3(nblk(c.n, 256)), dim3(256), 0, st, c);
+    return (int)hipGetLastError();
+}
+int launch_uf_slots(const ClusterArgs& c, hipStream_t st) {
+    hipLaunchKernelGGL(k_uf_slots, dim3(nblk(c.n * c.S, 256)), dim3(256), 0, st, c);
+    return (int)hipGetLastError();
+}
+int launch_uf_lists(const ClusterArgs& c, hipStream_t st) {
+    hipLaunchKernelGGL(k_uf_lists, dim3(nblk((long long)c.L2 * c.M, 256)), dim3(256), 0, st, c);
+    return (int)hipGetLastError();
+}
+int launch_uf_compress(const ClusterArgs& c, hipStream_t st) {
+    hipLaunchKernelGGL(k_uf_compress, dim3(nblk(c.n, 256)), dim3(256), 0, st, c);
+    return (int)hipGetLastError();
+}
+int launch_root_flags(const ClusterArgs& c, int* flags, hipStream_t st) {
+    hipLaunchKernelGGL(k_root_flags, dim3(nblk(c.n, 256)), dim3(256), 0, st, c, flags);
+    return (int)hipGetLastError();
+}
+int launch_border(const ClusterArgs& c, hipStream_t st) {
+    hipLaunchKernelGGL(k_border, dim3(nblk(c.n * c.S, 256)), dim3(256), 0, st, c);
+    return (int)hipGetLastError();
+}
+int launch_labels(const ClusterArgs& c, const int* rank, hipStream_t st) {
+    hipLaunchKernelGGL(k_labels, dim3(nblk(c.n, 256)), dim3(256), 0, st, c, rank);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive prefix sums (hipcub) and hub ids.
+size_t scan_tmp_bytes(long long n) {
+    size_t bytes = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int*)nullptr, (int*)nullptr, (int)n);
+    return bytes;
+}
+
+__global__ void k_total(const int* in, const int* excl, long long n, int* total) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *total = n > 0 ? excl[n - 1] + in[n - 1] : 0;
+}
+
+int scan_exclusive_i32(const int* in, int* out, long long n, int* total_out_dev, void* tmp,
+                       size_t tmp_bytes, hipStream_t st) {
+    if (n <= 0) return 0;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, (int)n, st);
+    if (e != hipSuccess) return (int)e;
+    if (total_out_dev) hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, st, in, out, n, total_out_dev);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_hub_flags(const uint8_t* is_hub, int* flags, long long n) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) flags[a] = is_hub[a] ? 1 : 0;
+}
+__global__ void k_hub_id(const uint8_t* is_hub, const int* excl, int* hub_id, long long n) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) hub_id[a] = is_hub[a] ? excl[a] : -1;
+}
+int launch_is_hub_scan_prep(const uint8_t* is_hub, int* flags, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_hub_flags, dim3(nblk(n, 256)), dim3(256), 0, st, is_hub, flags, n);
+    return (int)hipGetLastError();
+}
+int launch_hub_id(const uint8_t* is_hub, const int* excl, int* hub_id, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_hub_id, dim3(nblk(n, 256)), dim3(256), 0, st, is_hub, excl, hub_id, n);
+    return (int)hipGetLastError();
+}
+
+}  // namespace sdb

@@ -1,0 +1,431 @@
+// Candidate distances and the eps decision of sDbscan on gfx950.
+//
+// Reference: the candidate loop of rp_findCorePoints (src/sDbscan.cpp:243-359)
+// and computeDist (src/Utilities.h:212-287).  Instead of one point at a time,
+// work is grouped by top-M list: the (point, slot) entries that visit list l
+// form the query rows of a tile whose candidate rows are the M points of l,
+// kept in LDS.  Each pair is decided as "distance <= eps" exactly: a pair
+// whose fast distance lies within a derived error band of eps is queued and
+// re-decided by k_recheck (FP32, then FP64 in a fixed order), so every
+// precision makes the same decision.  Hits are stored only for the first
+// occurrence of a candidate among the point's slots (F masks), which is the
+// reference's per-point dedup (src/sDbscan.cpp:260-282).
+#include "sdb_kernels.hpp"
+
+namespace sdb {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned long long queue_entry(uint32_t ent, int l, int col) {
+    return ((unsigned long long)ent << 32) | ((unsigned long long)(uint32_t)l << 8) | (unsigned)col;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA tiles (cosine).  A = 16 query rows x k, B = k x 16 candidates, via
+// v_mfma_f32_16x16x32_bf16 (BF) or 4 x v_mfma_f32_16x16x4_f32.  Lane l loads
+// 16 bytes of query row (l & 15) and of candidate column (l & 15) at k-group
+// (l >> 4) per step; A and B use the same k assignment, so the sum over k is
+// the dot product whatever order the unit uses internally.
+template <bool BF, int QT, int NT, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
+    constexpr int SZ = BF ? 2 : 4;
+    constexpr int SE = 64 / SZ;  // elements per 64-byte k-step
+    constexpr int EW = (NT + 1) / 2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int M = a.M;
+    const size_t RSB = (size_t)a.kc * SZ + 16;
+    int* col_lds = (int*)(smem + (size_t)M * RSB);
+    const int tid = threadIdx.x;
+    const int lane = lane_id(), w = tid >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int l = a.tile_list[blockIdx.x];
+    const int beg = a.tile_beg[blockIdx.x], end = a.tile_end[blockIdx.x];
+    const int* Lm = a.lists + (size_t)l * M;
+    const char* Xc = (const char*)a.X;
+    const double lo = (double)a.eps - a.band, hi = (double)a.eps + a.band;
+
+    for (int i = tid; i < M; i += WAVES * 64) col_lds[i] = 0;
+    int colacc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) colacc[t] = 0;
+
+    const int rows_per_set = WAVES * QT * 16;
+    const int nsets = (end - beg + rows_per_set - 1) / rows_per_set;
+    for (int set = 0; set < nsets; ++set) {
+        uint32_t ent[QT];
+        long long pa[QT];
+        bool valid[QT];
+#pragma unroll
+        for (int q = 0; q < QT; ++q) {
+            int rr = beg + set * rows_per_set + (w * QT + q) * 16 + r16;
+            valid[q] = rr < end;
+            ent[q] = a.grp[valid[q] ? rr : beg];
+            pa[q] = (long long)(ent[q] / (uint32_t)a.S);
+        }
+        f32x4_t acc[QT][NT];
+#pragma unroll
+        for (int q = 0; q < QT; ++q)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[q][t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+
+        for (int c = 0; c < a.nchunk; ++c) {
+            const int k0 = c * a.kc;
+            const int kn = min(a.kc, a.dk - k0);
+            if (a.nchunk > 1 || set == 0) {
+                __syncthreads();
+                const int vpr = kn * SZ / 16;
+                for (int v = tid; v < M * vpr; v += WAVES * 64) {
+                    int row = v / vpr, cv = v - row * vpr;
+                    const uint4* src = (const uint4*)(Xc + ((size_t)Lm[row] * a.ldx + k0) * SZ) + cv;
+                    *(uint4*)(smem + row * RSB + (size_t)cv * 16) = *src;
+                }
+                __syncthreads();
+            }
+            const int nsteps = kn / SE;
+            const char* arow[QT];
+#pragma unroll
+            for (int q = 0; q < QT; ++q) arow[q] = Xc + ((size_t)pa[q] * a.ldx + k0) * SZ + g * 16;
+            uint4 af[QT];
+#pragma unroll
+            for (int q = 0; q < QT; ++q) af[q] = *(const uint4*)(arow[q]);
+            for (int st = 0; st < nsteps; ++st) {
+                uint4 an[QT];
+                const int stn = st + 1 < nsteps ? st + 1 : st;
+#pragma unroll
+                for (int q = 0; q < QT; ++q) an[q] = *(const uint4*)(arow[q] + (size_t)stn * 64);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    int row = t * 16 + r16;
+                    row = row < M ? row : M - 1;
+                    uint4 bfr = *(const uint4*)(smem + row * RSB + (size_t)st * 64 + g * 16);
+#pragma unroll
+                    for (int q = 0; q < QT; ++q) {
+                        if constexpr (BF) {
+                            bf16x8_t av = __builtin_bit_cast(bf16x8_t, af[q]);
+                            bf16x8_t bv = __builtin_bit_cast(bf16x8_t, bfr);
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[q][t], 0, 0, 0);
+                        } else {
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[q].x), __uint_as_float(bfr.x), acc[q][t], 0, 0, 0);
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[q].y), __uint_as_float(bfr.y), acc[q][t], 0, 0, 0);
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[q].z), __uint_as_float(bfr.z), acc[q][t], 0, 0, 0);
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[q].w), __uint_as_float(bfr.w), acc[q][t], 0, 0, 0);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < QT; ++q) af[q] = an[q];
+            }
+        }
+
+        // epilogue: lane (r16, g) holds rows 4g..4g+3 of column t*16 + r16
+#pragma unroll
+        for (int q = 0; q < QT; ++q) {
+            uint32_t Fw[EW], Ew[EW];
+#pragma unroll
+            for (int x = 0; x < EW; ++x) {
+                Fw[x] = x < a.W ? a.F[(size_t)ent[q] * a.W + x] : 0u;
+                Ew[x] = 0u;
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int col = t * 16 + r16;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = 4 * g + i;
+                    const uint32_t fword = __shfl(Fw[t >> 1], row, kWave);
+                    const int vr = __shfl((int)valid[q], row, kWave);
+                    const bool act = vr && col < M && ((fword >> (col & 31)) & 1u);
+                    const float dd = 1.0f - acc[q][t][i];
+                    const double dv = (double)dd;
+                    const bool hit = act && dv < lo;
+                    const bool unc = act && !(dv < lo) && !(dv > hi);
+                    const unsigned long long bh = __ballot(hit);
+                    if ((r16 & 3) == i)
+                        Ew[t >> 1] |= (uint32_t)((bh >> (16 * (r16 >> 2))) & 0xffffull) << (16 * (t & 1));
+                    colacc[t] += hit ? 1 : 0;
+                    const uint32_t er = __shfl(ent[q], row, kWave);
+                    if (unc) {
+                        int pos = atomicAdd(a.qcount, 1);
+                        if (pos < a.qcap) a.queue[pos] = queue_entry(er, l, col);
+                    }
+                }
+            }
+            if (g == 0 && valid[q]) {
+#pragma unroll
+                for (int x = 0; x < EW; ++x)
+                    if (x < a.W) a.E[(size_t)ent[q] * a.W + x] = Ew[x];
+            }
+        }
+    }
+    // column hit counts -> colsum (reverse degrees of the list's points)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        int v = colacc[t];
+        v += __shfl_xor(v, 16, kWave);
+        v += __shfl_xor(v, 32, kWave);
+        const int col = t * 16 + r16;
+        if (g == 0 && col < M && v) atomicAdd(&col_lds[col], v);
+    }
+    __syncthreads();
+    for (int i = tid; i < M; i += WAVES * 64) {
+        int v = col_lds[i];
+        if (v) atomicAdd(&a.colsum[(size_t)l * M + i], v);
+    }
+}
+
+template <bool BF, int NT>
+static int launch_mfma_nt(const DistArgs& a, int ntiles, hipStream_t st) {
+    constexpr int QT = 2, WAVES = 8;
+    const size_t sm = (size_t)a.M * ((size_t)a.kc * (BF ? 2 : 4) + 16) + (size_t)a.M * 4;
+    if (sm > (size_t)kLdsBytes) return -2;
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL((k_dist_mfma<BF, QT, NT, WAVES>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
+    return (int)hipGetLastError();
+}
+
+template <bool BF>
+static int launch_mfma_bf(const DistArgs& a, int ntiles, hipStream_t st) {
+    const int NT = (a.M + 15) / 16;
+    switch (NT) {
+        case 1: return launch_mfma_nt<BF, 1>(a, ntiles, st);
+        case 2: return launch_mfma_nt<BF, 2>(a, ntiles, st);
+        case 3: return launch_mfma_nt<BF, 3>(a, ntiles, st);
+        case 4: return launch_mfma_nt<BF, 4>(a, ntiles, st);
+        case 5: return launch_mfma_nt<BF, 5>(a, ntiles, st);
+        case 6: return launch_mfma_nt<BF, 6>(a, ntiles, st);
+        case 7: return launch_mfma_nt<BF, 7>(a, ntiles, st);
+        case 8: return launch_mfma_nt<BF, 8>(a, ntiles, st);
+        default: return -3;
+    }
+}
+
+int launch_dist_mfma(const DistArgs& a, int precision, int ntiles, hipStream_t st) {
+    if (ntiles <= 0) return 0;
+    return precision == kBF16 ? launch_mfma_bf<true>(a, ntiles, st)
+                              : launch_mfma_bf<false>(a, ntiles, st);
+}
+
+// ---------------------------------------------------------------------------
+// VALU tiles for every metric (src/Utilities.h:212-287).  Candidate rows (fp32)
+// in LDS with an odd row stride; each wave takes QV queries at a time, lane j
+// owns candidates j, j + 64, ...; per-lane sums run sequentially over k.
+template <int WAVES, int QV, int CM>
+__global__ __launch_bounds__(WAVES * 64) void k_dist_valu(DistArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int M = a.M;
+    const int RS = a.kc + 1;
+    float* cand = (float*)smem;
+    float* qbuf = cand + (size_t)M * RS;  // WAVES x QV x kc
+    int* col_lds = (int*)(qbuf + (size_t)WAVES * QV * a.kc);
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const int l = a.tile_list[blockIdx.x];
+    const int beg = a.tile_beg[blockIdx.x], end = a.tile_end[blockIdx.x];
+    const int* Lm = a.lists + (size_t)l * M;
+    const int metric = a.metric;
+    const double lo = (double)a.eps - a.band, hi = (double)a.eps + a.band;
+    float* qb = qbuf + (size_t)w * QV * a.kc;
+
+    for (int i = tid; i < M; i += WAVES * 64) col_lds[i] = 0;
+    const int rows_per_set = WAVES * QV;
+    const int nsets = (end - beg + rows_per_set - 1) / rows_per_set;
+    for (int set = 0; set < nsets; ++set) {
+        uint32_t ent[QV];
+        long long pa[QV];
+        bool valid[QV];
+#pragma unroll
+        for (int q = 0; q < QV; ++q) {
+            int rr = beg + set * rows_per_set + w * QV + q;
+            valid[q] = rr < end;
+            ent[q] = a.grp[valid[q] ? rr : beg];
+            pa[q] = (long long)(ent[q] / (uint32_t)a.S);
+        }
+        float acc[QV][CM];
+#pragma unroll
+        for (int q = 0; q < QV; ++q)
+#pragma unroll
+            for (int c = 0; c < CM; ++c) acc[q][c] = 0.0f;
+        for (int ch = 0; ch < a.nchunk; ++ch) {
+            const int k0 = ch * a.kc;
+            const int kn = min(a.kc, a.d - k0);
+            if (a.nchunk > 1 || set == 0) {
+                __syncthreads();
+                for (int v = tid; v < M * kn; v += WAVES * 64) {
+                    int row = v / kn, k = v - row * kn;
+                    cand[(size_t)row * RS + k] = a.X32[(size_t)Lm[row] * a.ldx + k0 + k];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < QV; ++q)
+                for (int k = lane; k < kn; k += 64) qb[q * a.kc + k] = a.X32[(size_t)pa[q] * a.ldx + k0 + k];
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+                int ci = lane + 64 * c;
+                int row = ci < M ? ci : M - 1;
+                const float* cr = cand + (size_t)row * RS;
+                for (int k = 0; k < kn; ++k) {
+                    const float y = cr[k];
+#pragma unroll
+                    for (int q = 0; q < QV; ++q) {
+                        const float x = qb[q * a.kc + k];
+                        if (metric == kL2) { float t = x - y; acc[q][c] = acc[q][c] + t * t; }
+                        else if (metric == kL1) { acc[q][c] = acc[q][c] + fabsf(x - y); }
+                        else if (metric == kChi2) { if (x > 0.0f && y > 0.0f) acc[q][c] = acc[q][c] + x * y / (x + y); }
+                        else if (metric == kJS) {
+                            const float xe = x + 1e-6f, ye = y + 1e-6f, s = xe + ye;
+                            acc[q][c] = acc[q][c] + xe * (logf(s / xe) * 1.4426950408889634f) + ye * (logf(s / ye) * 1.4426950408889634f);
+                        } else { acc[q][c] = acc[q][c] + x * y; }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < QV; ++q) {
+            uint32_t Fw[2 * CM];
+#pragma unroll
+            for (int x = 0; x < 2 * CM; ++x) Fw[x] = (valid[q] && x < a.W) ? a.F[(size_t)ent[q] * a.W + x] : 0u;
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+                const int ci = lane + 64 * c;
+                float dist;
+                const float s = acc[q][c];
+                if (metric == kL2) dist = sqrtf(s);
+                else if (metric == kL1) dist = s;
+                else if (metric == kChi2) dist = 1.0f - 2.0f * s;
+                else if (metric == kJS) dist = 1.0f - s * 0.5f;
+                else dist = 1.0f - s;
+                const uint32_t fw = Fw[2 * c + (lane >> 5)];
+                const bool act = valid[q] && ci < M && ((fw >> (lane & 31)) & 1u);
+                const double dv = (double)dist;
+                const bool hit = act && dv < lo;
+                const bool unc = act && !(dv < lo) && !(dv > hi);
+                const unsigned long long bh = __ballot(hit);
+                if (valid[q] && lane == 0) {
+                    if (2 * c < a.W) a.E[(size_t)ent[q] * a.W + 2 * c] = (uint32_t)bh;
+                    if (2 * c + 1 < a.W) a.E[(size_t)ent[q] * a.W + 2 * c + 1] = (uint32_t)(bh >> 32);
+                }
+                if (hit) atomicAdd(&col_lds[ci], 1);
+                if (unc) {
+                    int pos = atomicAdd(a.qcount, 1);
+                    if (pos < a.qcap) a.queue[pos] = queue_entry(ent[q], l, ci);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < M; i += WAVES * 64) {
+        int v = col_lds[i];
+        if (v) atomicAdd(&a.colsum[(size_t)l * M + i], v);
+    }
+}
+
+template <int CM>
+static int launch_valu_cm(const DistArgs& a, int ntiles, hipStream_t st) {
+    constexpr int WAVES = 8, QV = 4;
+    const size_t sm = ((size_t)a.M * (a.kc + 1) + (size_t)WAVES * QV * a.kc) * 4 + (size_t)a.M * 4;
+    if (sm > (size_t)kLdsBytes) return -2;
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_valu<WAVES, QV, CM>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL((k_dist_valu<WAVES, QV, CM>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
+    return (int)hipGetLastError();
+}
+
+int launch_dist_valu(const DistArgs& a, int ntiles, hipStream_t st) {
+    if (ntiles <= 0) return 0;
+    const int CM = (a.M + 63) / 64;
+    switch (CM) {
+        case 1: return launch_valu_cm<1>(a, ntiles, st);
+        case 2: return launch_valu_cm<2>(a, ntiles, st);
+        case 3: return launch_valu_cm<3>(a, ntiles, st);
+        case 4: return launch_valu_cm<4>(a, ntiles, st);
+        default: return -3;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary recheck, one wavefront per queued pair.  Sums run per lane over
+// k = lane, lane + 64, ... and combine through an xor butterfly (offsets
+// 32..1).  The BF16 path first re-decides with the FP32 value when it lies
+// outside the FP32 band; everything else is decided with the FP64 value.
+// stats[0]: FP32 rechecks, stats[1]: FP64 rechecks, stats[2]: hits added.
+__device__ __forceinline__ double dist64_wave(const float* x, const float* y, int d, int metric, int lane) {
+    double s = 0.0;
+    for (int i = lane; i < d; i += 64) {
+        const double xv = x[i], yv = y[i];
+        if (metric == kCosine) s = s + xv * yv;
+        else if (metric == kL1) s = s + fabs(xv - yv);
+        else if (metric == kL2) { double t = xv - yv; s = s + t * t; }
+        else if (metric == kChi2) { if (x[i] > 0.0f && y[i] > 0.0f) s = s + (xv * yv) / (xv + yv); }
+        else {
+            const double xe = xv + 1e-6, ye = yv + 1e-6, t = xe + ye, ln2 = 0.69314718055994530942;
+            s = s + (xe * (log(t / xe) / ln2) + ye * (log(t / ye) / ln2));
+        }
+    }
+    s = wave_sum_f64_tree(s);
+    if (metric == kCosine) return 1.0 - s;
+    if (metric == kL1) return s;
+    if (metric == kL2) return sqrt(s);
+    if (metric == kChi2) return 1.0 - 2.0 * s;
+    return 1.0 - s / 2.0;
+}
+
+__device__ __forceinline__ float dot32_wave(const float* x, const float* y, int d, int lane) {
+    float s = 0.0f;
+    for (int i = lane; i < d; i += 64) s = fmaf(x[i], y[i], s);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s = s + __shfl_xor(s, m, kWave);
+    return s;
+}
+
+__global__ __launch_bounds__(256) void k_recheck(DistArgs a, const int* topK, int K, int P,
+                                                 int precision, double band32, int n_entries,
+                                                 unsigned long long* stats) {
+    const int lane = lane_id();
+    const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long e = wv; e < n_entries; e += nwv) {
+        const unsigned long long qe = a.queue[e];
+        const uint32_t ent = (uint32_t)(qe >> 32);
+        const int l = (int)((qe >> 8) & 0xffffffu);
+        const int col = (int)(qe & 0xffu);
+        const long long pa = ent / (uint32_t)a.S;
+        const int c = a.lists[(size_t)l * a.M + col];
+        const float* x = a.X32 + (size_t)pa * a.ldx;
+        const float* y = a.X32 + (size_t)c * a.ldx;
+        bool hit;
+        bool decided = false;
+        if (precision == kBF16 && a.metric == kCosine) {
+            const float d32 = 1.0f - dot32_wave(x, y, a.d, lane);
+            const double dv = (double)d32;
+            if (lane == 0) atomicAdd(&stats[0], 1ull);
+            if (dv < (double)a.eps - band32) { hit = true; decided = true; }
+            else if (dv > (double)a.eps + band32) { hit = false; decided = true; }
+        }
+        if (!decided) {
+            const double d64 = dist64_wave(x, y, a.d, a.metric, lane);
+            if (lane == 0) atomicAdd(&stats[1], 1ull);
+            hit = d64 <= (double)a.eps;
+        }
+        if (hit && lane == 0) {
+            atomicOr(&a.E[(size_t)ent * a.W + (col >> 5)], 1u << (col & 31));
+            atomicAdd(&a.colsum[(size_t)l * a.M + col], 1);
+            atomicAdd(&stats[2], 1ull);
+        }
+    }
+}
+
+int launch_recheck_k(const DistArgs& a, const int* topK, int K, int P, int precision,
+                     double band32, int n_entries, unsigned long long* stats, hipStream_t st) {
+    if (n_entries <= 0) return 0;
+    long long blocks = ((long long)n_entries + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_recheck, dim3((unsigned)blocks), dim3(256), 0, st, a, topK, K, P, precision, band32, n_entries, stats);
+    return (int)hipGetLastError();
+}
+
+}  // namespace sdb

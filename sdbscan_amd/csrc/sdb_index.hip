@@ -1,0 +1,724 @@
+// Index construction of sDbscan on gfx950: transform, kernel embedding,
+// HD3 Walsh-Hadamard projection, per-point top-K and per-random-vector top-M.
+//
+// Reference: rp_parIndex (src/sDbscan.cpp:19-209), transformData
+// (src/Utilities.h:289-335), embeddings (src/sDbscan.cpp:53-68,
+// src/Utilities.h:132-208), FFHT (src/fht_avx.c:1013-1256).
+#include "sdb_kernels.hpp"
+
+namespace sdb {
+
+// ---------------------------------------------------------------------------
+// transformData (src/Utilities.h:289-335).  One wavefront per point.  Cosine:
+// x / ||x|| with ||x||^2 accumulated in FP64 per lane (stride 64) and combined
+// by an xor butterfly; Chi2/JS: nonnegativity check and x / sum(x).  Writes
+// the FP32 rows (row stride ldx, zero padded) and, when requested, the BF16
+// copy used by the BF16 distance tiles.
+// err bit 1: negative entry (Chi2/JS); err bit 2: zero point.
+__global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, long long n, int d,
+                                                   int ldx, int metric, float* __restrict__ Xn,
+                                                   uint16_t* __restrict__ Xb, int* __restrict__ err) {
+    const int lane = lane_id();
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const long long nwaves = (long long)gridDim.x * blockDim.x / kWave;
+    for (long long p = wave; p < n; p += nwaves) {
+        const float* x = X + p * d;
+        float scale = 1.0f;
+        bool scaled = false;
+        if (metric == kCosine) {
+            double acc = 0.0;
+            for (int i = lane; i < d; i += kWave) { double v = x[i]; acc = acc + v * v; }
+            double t = wave_sum_f64_tree(acc);
+            scale = (float)sqrt(t);
+            scaled = true;
+            if (!(scale > 0.0f) && lane == 0) atomicOr(err, 2);
+        } else if (metric == kChi2 || metric == kJS) {
+            double acc = 0.0;
+            int neg = 0;
+            for (int i = lane; i < d; i += kWave) { float v = x[i]; neg |= v < 0.0f; acc = acc + (double)v; }
+            if (__any(neg) && lane == 0) atomicOr(err, 1);
+            double t = wave_sum_f64_tree(acc);
+            scale = (float)t;
+            scaled = true;
+            if (!(scale > 0.0f) && lane == 0) atomicOr(err, 2);
+        }
+        float* o = Xn + p * ldx;
+        uint16_t* ob = Xb ? Xb + p * ldx : nullptr;
+        for (int i = lane; i < ldx; i += kWave) {
+            float v = 0.0f;
+            if (i < d) v = scaled ? x[i] / scale : x[i];
+            o[i] = v;
+            if (ob) ob[i] = (uint16_t)f32_to_bf16_rne(v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One FFHT level on a wavefront-resident vector.  Element e of the D-vector
+// lives in lane (e & 63), register j = e >> 6.  Strides < 64 exchange across
+// lanes (lane ^ s); strides >= 64 pair registers j and j + s/64.  Butterfly
+// forms follow the FFHT kernels (src/fht_avx.c:33-56, 1013-1251): stride 1
+// computes lo = a - (0 - b), hi = a + (0 - b); strides 2 and 4 compute
+// lo = a + b, hi = a + (0 - b); wider strides compute a + b, a - b.
+template <int VPL>
+__device__ __forceinline__ void fwht_cross(float (&v)[VPL], int s, int lane) {
+    const bool upper = (lane & s) != 0;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) {
+        float mine = v[j];
+        float other = __shfl_xor(mine, s, kWave);
+        float a = upper ? other : mine;   // lower element
+        float b = upper ? mine : other;   // upper element
+        float r;
+        if (s == 1) {
+            float t = 0.0f - b;
+            r = upper ? a + t : a - t;
+        } else if (s == 2 || s == 4) {
+            float t = 0.0f - b;
+            r = upper ? a + t : a + b;
+        } else {
+            r = upper ? a - b : a + b;
+        }
+        v[j] = r;
+    }
+}
+
+template <int VPL>
+__device__ __forceinline__ void fwht_inreg(float (&v)[VPL], int js) {
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) {
+        if ((j & js) == 0) {
+            float a = v[j], b = v[j + js];
+            v[j] = a + b;
+            v[j + js] = a - b;
+        }
+    }
+}
+
+template <int LOGD>
+__device__ __forceinline__ void ffht_wave(float (&v)[(1 << LOGD) / 64], int lane) {
+    constexpr int VPL = (1 << LOGD) / 64;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) fwht_cross<VPL>(v, s, lane);
+#pragma unroll
+    for (int js = 1; js < VPL; js <<= 1) fwht_inreg<VPL>(v, js);
+}
+
+// Embedding of point p into the wave registers (element e = lane + 64 j).
+// Cosine: identity; L1/L2: [cos(Rx), sin(Rx)] with Rx a sequential fmaf chain
+// over the input dims; Chi2/JS: the deterministic sampled features.
+template <int VPL>
+__device__ __forceinline__ void load_embedding(const IndexArgs& a, long long p, int lane,
+                                               float (&v)[VPL]) {
+    const float* x = a.Xn + p * a.ldx;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) v[j] = 0.0f;
+    if (a.metric == kCosine) {
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            int e = lane + 64 * j;
+            if (e < a.d) v[j] = x[e];
+        }
+    } else if (a.metric == kL1 || a.metric == kL2) {
+        const int half = a.ker / 2;
+        // element e < half holds cos(R_e x), half <= e < 2 half holds
+        // sin(R_{e-half} x)
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            int e = lane + 64 * j;
+            if (e < 2 * half) {
+                int r = e < half ? e : e - half;
+                float acc = 0.0f;
+                for (int c = 0; c < a.d; ++c) acc = fmaf(a.R[(size_t)c * half + r], x[c], acc);
+                v[j] = e < half ? cosf(acc) : sinf(acc);
+            }
+        }
+    } else {
+        const int d = a.d;
+        const int comps = ((a.ker / d) - 1) / 2;
+        const float L = a.kerL;
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            int e = lane + 64 * j;
+            if (e >= a.ker) continue;
+            if (e < d) {
+                float xv = x[e];
+                if (xv > 0.0f) {
+                    if (a.metric == kChi2) v[j] = sqrtf(xv * L);
+                    else v[j] = (float)sqrt((double)(xv * L) * 2.0 / log(4.0));
+                }
+            } else {
+                int t = e - d;
+                int i = t / (2 * d) + 1;
+                if (i > comps) continue;
+                int rem = t % (2 * d);
+                int jj = rem < d ? rem : rem - d;
+                float xv = x[jj];
+                if (xv > 0.0f) {
+                    float f, ph;
+                    if (a.metric == kChi2) {
+                        f = (float)sqrt(2.0 * xv * L / cosh(M_PI * i * L));
+                        ph = (float)i * L * logf(xv);
+                    } else {
+                        const float iL = (float)i * L;
+                        float kap = (float)(2.0 / (log(4.0) * (double)(1.0f + 4.0f * iL * iL) * cosh(M_PI * i * L)));
+                        f = (float)sqrt(2.0 * xv * L * kap);
+                        ph = iL * logf(xv);
+                    }
+                    v[j] = rem < d ? f * cosf(ph) : f * sinf(ph);
+                }
+            }
+        }
+    }
+}
+
+template <int VPL>
+__device__ __forceinline__ void hd3_project(const IndexArgs& a, long long p, int lane,
+                                            float (&v)[VPL], const uint32_t (&sg)[3][(VPL + 31) / 32]) {
+    load_embedding<VPL>(a, p, lane, v);
+    constexpr int LOGD = (VPL == 1 ? 6 : VPL == 2 ? 7 : VPL == 4 ? 8 : VPL == 8 ? 9 : VPL == 16 ? 10 : VPL == 32 ? 11 : VPL == 64 ? 12 : 13);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            // (2 * bit - 1) multiplier of src/sDbscan.cpp:80-83: bit 0 negates
+            uint32_t bit = (sg[r][j >> 5] >> (j & 31)) & 1u;
+            v[j] = __uint_as_float(__float_as_uint(v[j]) ^ ((bit ^ 1u) << 31));
+        }
+        ffht_wave<LOGD>(v, lane);
+    }
+}
+
+// Wave-level selection of the K largest keys (value desc, index asc) among
+// the first P elements.  T0 = K-th largest lane maximum bounds the K-th
+// largest element from below, so only elements whose value is >= value(T0)
+// are ranked (ties with the K-th value included).  Winners are written to
+// out[0..K); returns true when the (K+1)-th largest value equals the K-th
+// (a tie across the selection boundary).
+template <int VPL>
+__device__ bool wave_topk(const uint32_t (&key)[VPL], int P, int K, int lane, uint64_t* lds,
+                          int* out) {
+    uint64_t lm = 0;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) {
+        int e = lane + 64 * j;
+        if (e < P) {
+            uint64_t k64 = ((uint64_t)key[j] << 32) | (uint32_t)(0xffffffffu - (uint32_t)e);
+            lm = k64 > lm ? k64 : lm;
+        }
+    }
+    uint64_t cur = lm, t0 = 0;
+    const int rounds = K < 64 ? K : 64;
+    for (int k = 0; k < rounds; ++k) {
+        uint64_t b = wave_max_u64(cur);
+        t0 = b;
+        if (cur == b) cur = 0;
+    }
+    if (K > 64) t0 = 0;  // more winners than lanes: rank every element
+    const uint32_t tv = (uint32_t)(t0 >> 32);
+    // compact candidates (value >= value(T0)) into lds
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) {
+        int e = lane + 64 * j;
+        cnt += (e < P && key[j] >= tv) ? 1 : 0;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        int o = __shfl_up(incl, m, kWave);
+        if (lane >= m) incl += o;
+    }
+    const int total = __shfl(incl, 63, kWave);
+    int pos = incl - cnt;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) {
+        int e = lane + 64 * j;
+        if (e < P && key[j] >= tv)
+            lds[pos++] = ((uint64_t)key[j] << 32) | (uint32_t)(0xffffffffu - (uint32_t)e);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    uint64_t prev = 0;
+    bool tie = false;
+    for (int k = 0; k <= K && k < total; ++k) {
+        uint64_t best = 0;
+        int bi = -1;
+        for (int i = lane; i < total; i += 64) {
+            uint64_t c = lds[i];
+            if (c > best) { best = c; bi = i; }
+        }
+        uint64_t w = wave_max_u64(best);
+        if (k < K) {
+            if (best == w && bi >= 0) lds[bi] = 0;
+            if (lane == 0) out[k] = (int)(0xffffffffu - (uint32_t)w);
+            prev = w;
+        } else {
+            tie = (uint32_t)(w >> 32) == (uint32_t)(prev >> 32);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    return tie;
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1 of the index (MODE 0): embedding + HD3 + per-point top-K
+// (src/sDbscan.cpp:97-143), per-chunk row maxima / minima of the projections
+// for the top-M threshold, and capture of points whose top-K has a tie at the
+// boundary (replayed on the host with the reference heap).
+// Pass 2 (MODE 1): recompute the projections and append every value beyond
+// the per-row thresholds to the top-M candidate lists.
+// MODE 2: recompute and store whole projection rows listed in row_slot.
+//
+// Work split: grid of G workgroups x 4 waves; wave w of group g handles the
+// points p = (4 g + w) + t * 4 G, so chunk g is an interleaved subset.
+template <int LOGD, int MODE>
+__global__ __launch_bounds__(256) void k_index(IndexArgs a) {
+    constexpr int D = 1 << LOGD;
+    constexpr int VPL = D / 64;
+    constexpr int NW = (VPL + 31) / 32;
+    const int lane = lane_id();
+    const int wib = threadIdx.x / kWave;
+    const long long gw = (long long)blockIdx.x * 4 + wib;
+    const long long nw = (long long)gridDim.x * 4;
+
+    extern __shared__ __align__(16) unsigned char smem[];
+
+    uint32_t sg[3][NW];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            uint32_t m = 0;
+            for (int b = 0; b < 32 && w * 32 + b < VPL; ++b)
+                m |= (uint32_t)a.bits[r * D + lane + 64 * (w * 32 + b)] << b;
+            sg[r][w] = m;
+        }
+
+    float v[VPL];
+    if (MODE == 0) {
+        uint64_t* lds = (uint64_t*)smem + (size_t)wib * D;        // top-K candidates
+        uint32_t* red = (uint32_t*)(smem + (size_t)4 * D * 8);    // [2][P] chunk max/min keys
+        for (int i = threadIdx.x; i < a.P; i += blockDim.x) { red[i] = 0u; red[a.P + i] = 0xffffffffu; }
+        __syncthreads();
+        uint32_t cmax[VPL], cmin[VPL];
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) { cmax[j] = 0u; cmin[j] = 0xffffffffu; }
+        for (long long p = gw; p < a.n; p += nw) {
+            hd3_project<VPL>(a, p, lane, v, sg);
+            uint32_t kc[VPL], kf[VPL];
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) {
+                kc[j] = ord_key(v[j]);
+                kf[j] = ord_key(-v[j]);
+                int e = lane + 64 * j;
+                if (e < a.P) {
+                    cmax[j] = kc[j] > cmax[j] ? kc[j] : cmax[j];
+                    cmin[j] = kc[j] < cmin[j] ? kc[j] : cmin[j];
+                }
+            }
+            int* tk = a.topK + p * 2 * a.K;
+            bool t1 = wave_topk<VPL>(kc, a.P, a.K, lane, lds, tk);
+            bool t2 = wave_topk<VPL>(kf, a.P, a.K, lane, lds, tk + a.K);
+            if (t1 || t2) {
+                int slot = 0;
+                if (lane == 0) slot = atomicAdd(a.tie_count, 1);
+                slot = __shfl(slot, 0, kWave);
+                if (slot < a.tie_cap) {
+                    if (lane == 0) a.tie_idx[slot] = (int)p;
+                    float* tv = a.tie_vals + (size_t)slot * a.P;
+#pragma unroll
+                    for (int j = 0; j < VPL; ++j) {
+                        int e = lane + 64 * j;
+                        if (e < a.P) tv[e] = v[j];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            int e = lane + 64 * j;
+            if (e < a.P) { atomicMax(&red[e], cmax[j]); atomicMin(&red[a.P + e], cmin[j]); }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < a.P; i += blockDim.x) {
+            a.chunk_max[(size_t)blockIdx.x * a.P + i] = red[i];
+            a.chunk_min[(size_t)blockIdx.x * a.P + i] = red[a.P + i];
+        }
+    } else if (MODE == 1) {
+        uint32_t thi[VPL], tlo[VPL];
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            int e = lane + 64 * j;
+            thi[j] = e < a.P ? a.thr_hi[e] : 0xffffffffu;
+            tlo[j] = e < a.P ? a.thr_lo[e] : 0u;
+        }
+        for (long long p = gw; p < a.n; p += nw) {
+            hd3_project<VPL>(a, p, lane, v, sg);
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) {
+                int e = lane + 64 * j;
+                if (e >= a.P) continue;
+                uint32_t k = ord_key(v[j]);
+                if (k >= thi[j]) {
+                    int pos = atomicAdd(&a.cand_cnt[2 * e], 1);
+                    if (pos < a.cand_cap) {
+                        size_t o = ((size_t)(2 * e) * a.cand_cap + pos);
+                        a.cand_key[o] = k;
+                        a.cand_idx[o] = (int)p;
+                    }
+                }
+                if (k <= tlo[j]) {
+                    int pos = atomicAdd(&a.cand_cnt[2 * e + 1], 1);
+                    if (pos < a.cand_cap) {
+                        size_t o = ((size_t)(2 * e + 1) * a.cand_cap + pos);
+                        a.cand_key[o] = k;
+                        a.cand_idx[o] = (int)p;
+                    }
+                }
+            }
+        }
+    } else {
+        for (long long p = gw; p < a.n; p += nw) {
+            hd3_project<VPL>(a, p, lane, v, sg);
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) {
+                int e = lane + 64 * j;
+                if (e < a.P) {
+                    int s = a.row_slot[e];
+                    if (s >= 0) a.row_buf[(size_t)s * a.n + p] = v[j];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-row top-M thresholds: the (M+1)-th largest chunk maximum bounds the
+// (M+1)-th largest projection value of the row from below (at least M+1
+// chunks hold a value that large), likewise for minima.  One workgroup per
+// row; rank by counting over the G chunk keys.
+__global__ __launch_bounds__(256) void k_topm_threshold(const uint32_t* __restrict__ cmax,
+                                                        const uint32_t* __restrict__ cmin, int G,
+                                                        int P, int M, uint32_t* __restrict__ thi,
+                                                        uint32_t* __restrict__ tlo) {
+    const int r = blockIdx.x;
+    __shared__ uint32_t smax[4096], smin[4096];
+    __shared__ uint32_t res_hi, res_lo;
+    if (threadIdx.x == 0) { res_hi = 0u; res_lo = 0xffffffffu; }
+    for (int i = threadIdx.x; i < G; i += blockDim.x) {
+        smax[i] = cmax[(size_t)i * P + r];
+        smin[i] = cmin[(size_t)i * P + r];
+    }
+    __syncthreads();
+    const int want = M + 1;
+    if (G < want) {  // too few chunks: every value is a candidate
+        if (threadIdx.x == 0) { thi[r] = 0u; tlo[r] = 0xffffffffu; }
+        return;
+    }
+    for (int i = threadIdx.x; i < G; i += blockDim.x) {
+        uint32_t x = smax[i], y = smin[i];
+        int gx = 0, gy = 0;
+        for (int j = 0; j < G; ++j) {
+            uint32_t u = smax[j], w = smin[j];
+            gx += (u > x || (u == x && j < i)) ? 1 : 0;
+            gy += (w < y || (w == y && j < i)) ? 1 : 0;
+        }
+        if (gx == want - 1) res_hi = x;
+        if (gy == want - 1) res_lo = y;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) { thi[r] = res_hi; tlo[r] = res_lo; }
+}
+
+// Exact top-M per row and side from the candidate lists: bitonic sort of the
+// (value, index) keys in LDS, descending value then ascending index (the far
+// side sorts the complemented key, so the smallest values come first).
+// flags[row*2+side]: 1 = tie across the boundary, 2 = candidate overflow;
+// flagged rows are replayed on the host.
+__global__ __launch_bounds__(1024) void k_topm_select(const uint32_t* __restrict__ cand_key,
+                                                      const int* __restrict__ cand_idx,
+                                                      const int* __restrict__ cand_cnt, int cap,
+                                                      int P, int M, int* __restrict__ topM,
+                                                      int* __restrict__ flags) {
+    const int rs = blockIdx.x;  // row * 2 + side
+    const int r = rs >> 1, side = rs & 1;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint64_t* s = (uint64_t*)smem;
+    const int cnt = cand_cnt[rs];
+    if (cnt > cap) {
+        if (threadIdx.x == 0) flags[rs] = 2;
+        return;
+    }
+    int sz = 1;
+    while (sz < cnt) sz <<= 1;
+    for (int i = threadIdx.x; i < sz; i += blockDim.x) {
+        uint64_t k = 0;
+        if (i < cnt) {
+            uint32_t kv = cand_key[(size_t)rs * cap + i];
+            if (side) kv = ~kv;
+            k = ((uint64_t)kv << 32) | (uint32_t)(0xffffffffu - (uint32_t)cand_idx[(size_t)rs * cap + i]);
+        }
+        s[i] = k;
+    }
+    __syncthreads();
+    for (int k = 2; k <= sz; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < sz; i += blockDim.x) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t x = s[i], y = s[ixj];
+                    bool desc = (i & k) == 0;
+                    if (desc ? (x < y) : (x > y)) { s[i] = y; s[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int m = threadIdx.x; m < M; m += blockDim.x)
+        topM[(size_t)r * 2 * M + side * M + m] = m < cnt ? (int)(0xffffffffu - (uint32_t)s[m]) : -1;
+    if (threadIdx.x == 0) {
+        int f = 0;
+        if (cnt < M) f = 2;
+        else if (cnt > M && (uint32_t)(s[M - 1] >> 32) == (uint32_t)(s[M] >> 32)) f = 1;
+        flags[rs] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Candidate-list structures.  A "list" is a top-M list: list l < P is the
+// close list of random vector l, list P + r the far list of r.  Point a's
+// slot s (0 <= s < 2K) refers to list slot_list(a, s): slot 2k is the close
+// list of its k-th closest vector, slot 2k+1 the far list of its k-th
+// furthest vector (the reference's loop order, src/sDbscan.cpp:262-354).
+__device__ __forceinline__ int slot_list(const int* topK, long long a, int s, int K, int P) {
+    int k = s >> 1;
+    return (s & 1) ? P + topK[a * 2 * K + K + k] : topK[a * 2 * K + k];
+}
+
+// lists[l * M + i] = point id of position i of list l (from matrix_topM).
+__global__ void k_build_lists(const int* __restrict__ topM, int P, int M, int* __restrict__ lists,
+                              uint8_t* __restrict__ is_hub) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * P * M) return;
+    int l = t / M, i = t % M;
+    int r = l < P ? l : l - P;
+    int side = l < P ? 0 : 1;
+    int pid = topM[(size_t)r * 2 * M + side * M + i];
+    lists[t] = pid;
+    is_hub[pid] = 1;
+}
+
+// Per hub membership count (hub id from the exclusive scan of is_hub).
+__global__ void k_hub_count(const int* __restrict__ lists, int LM, const int* __restrict__ hub_id,
+                            int* __restrict__ hub_cnt) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= LM) return;
+    atomicAdd(&hub_cnt[hub_id[lists[t]]], 1);
+}
+
+__global__ void k_hub_fill(const int* __restrict__ lists, int LM, const int* __restrict__ hub_id,
+                           const int* __restrict__ hub_ptr, int* __restrict__ hub_cur,
+                           int* __restrict__ hub_mem) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= LM) return;
+    int h = hub_id[lists[t]];
+    int pos = atomicAdd(&hub_cur[h], 1);
+    hub_mem[hub_ptr[h] + pos] = t;  // t = l * M + i
+}
+
+// dup[l1][l2] bit i set when position i of list l2 holds a point that is also
+// in list l1 (l1 != l2).  Used for first-occurrence masks.
+__global__ void k_dupmask(const int* __restrict__ hub_ptr, const int* __restrict__ hub_mem, int H,
+                          int M, int L2, int W, uint32_t* __restrict__ dup) {
+    int h = blockIdx.x * blockDim.y + threadIdx.y;
+    if (h >= H) return;
+    int b = hub_ptr[h], e = hub_ptr[h + 1];
+    int t = e - b;
+    for (int q = threadIdx.x; q < t * t; q += blockDim.x) {
+        int x = q / t, y = q % t;
+        if (x == y) continue;
+        int mx = hub_mem[b + x], my = hub_mem[b + y];
+        int l1 = mx / M, l2 = my / M, i2 = my % M;
+        atomicOr(&dup[((size_t)l1 * L2 + l2) * W + (i2 >> 5)], 1u << (i2 & 31));
+    }
+}
+
+// inv[l][L] bit i set when list L is one of the 2K lists of the point at
+// position i of list l.  Used for the mutual-pair correction of hub degrees.
+__global__ void k_invmask(const int* __restrict__ lists, const int* __restrict__ topK, int P, int K,
+                          int M, int L2, int W, uint32_t* __restrict__ inv) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long tot = (long long)L2 * M * 2 * K;
+    if (t >= tot) return;
+    int s = (int)(t % (2 * K));
+    long long li = t / (2 * K);
+    int l = (int)(li / M), i = (int)(li % M);
+    int c = lists[li];
+    int L = slot_list(topK, c, s, K, P);
+    atomicOr(&inv[((size_t)l * L2 + L) * W + (i >> 5)], 1u << (i & 31));
+}
+
+// First-occurrence mask per (point, slot): bit i of F[a][s] is set when the
+// point at position i of the slot's list is not a itself and does not occur
+// in an earlier slot of a (the reference's per-point dedup bitset,
+// src/sDbscan.cpp:260-282).  Also counts slot entries per list for grouping.
+__global__ void k_first_mask(const int* __restrict__ topK, long long n, int P, int K, int M, int L2,
+                             int W, const uint32_t* __restrict__ dup,
+                             const int* __restrict__ hub_id, const int* __restrict__ hub_ptr,
+                             const int* __restrict__ hub_mem, uint32_t* __restrict__ F,
+                             int* __restrict__ grp_cnt) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int S = 2 * K;
+    if (t >= n * S) return;
+    long long a = t / S;
+    int s = (int)(t % S);
+    int l = slot_list(topK, a, s, K, P);
+    uint32_t m[kMaxW];
+    for (int w = 0; w < W; ++w) {
+        int lo = w * 32;
+        int nb = M - lo;
+        m[w] = nb >= 32 ? 0xffffffffu : (nb > 0 ? ((1u << nb) - 1u) : 0u);
+    }
+    for (int s2 = 0; s2 < s; ++s2) {
+        int l2 = slot_list(topK, a, s2, K, P);
+        const uint32_t* dp = dup + ((size_t)l2 * L2 + l) * W;
+        for (int w = 0; w < W; ++w) m[w] &= ~dp[w];
+    }
+    int h = hub_id[a];
+    if (h >= 0) {
+        for (int q = hub_ptr[h]; q < hub_ptr[h + 1]; ++q) {
+            int li = hub_mem[q];
+            if (li / M == l) { int i = li % M; m[i >> 5] &= ~(1u << (i & 31)); }
+        }
+    }
+    for (int w = 0; w < W; ++w) F[t * W + w] = m[w];
+    atomicAdd(&grp_cnt[l], 1);
+}
+
+// Group (point, slot) entries by list: grp[grp_ptr[l] + k] = a * 2K + s.
+__global__ void k_group_fill(const int* __restrict__ topK, long long n, int P, int K,
+                             const int* __restrict__ grp_ptr, int* __restrict__ grp_cur,
+                             uint32_t* __restrict__ grp) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int S = 2 * K;
+    if (t >= n * S) return;
+    long long a = t / S;
+    int s = (int)(t % S);
+    int l = slot_list(topK, a, s, K, P);
+    int pos = atomicAdd(&grp_cur[l], 1);
+    grp[grp_ptr[l] + pos] = (uint32_t)t;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
+                     uint16_t* Xb, int* err, hipStream_t st) {
+    long long blocks = (n + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_transform, dim3((unsigned)blocks), dim3(256), 0, st, X, n, d, ldx, metric, Xn, Xb, err);
+    return (int)hipGetLastError();
+}
+
+template <int LOGD, int MODE>
+static int launch_index_t(const IndexArgs& a, int grid, hipStream_t st) {
+    constexpr int D = 1 << LOGD;
+    size_t sm = 0;
+    if (MODE == 0) sm = (size_t)4 * D * 8 + (size_t)2 * a.P * 4;
+    if (MODE == 0 && sm > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL((k_index<LOGD, MODE>), dim3(grid), dim3(256), sm, st, a);
+    return (int)hipGetLastError();
+}
+
+int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st) {
+#define SDB_CASE(LD)                                                  \
+    case LD:                                                          \
+        if (mode == 0) return launch_index_t<LD, 0>(a, grid, st);     \
+        if (mode == 1) return launch_index_t<LD, 1>(a, grid, st);     \
+        return launch_index_t<LD, 2>(a, grid, st);
+    switch (logD) {
+        SDB_CASE(6)
+        SDB_CASE(7)
+        SDB_CASE(8)
+        SDB_CASE(9)
+        SDB_CASE(10)
+        SDB_CASE(11)
+        SDB_CASE(12)
+        default: return -1;
+    }
+#undef SDB_CASE
+}
+
+int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,
+                          uint32_t* thi, uint32_t* tlo, hipStream_t st) {
+    hipLaunchKernelGGL(k_topm_threshold, dim3(P), dim3(256), 0, st, cmax, cmin, G, P, M, thi, tlo);
+    return (int)hipGetLastError();
+}
+
+int launch_topm_select(const uint32_t* key, const int* idx, const int* cnt, int cap, int P, int M,
+                       int* topM, int* flags, hipStream_t st) {
+    size_t sm = 0;
+    int sz = 1;
+    while (sz < cap) sz <<= 1;
+    sm = (size_t)sz * 8;
+    if (sm > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_topm_select,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(k_topm_select, dim3(2 * P), dim3(1024), sm, st, key, idx, cnt, cap, P, M, topM, flags);
+    return (int)hipGetLastError();
+}
+
+int launch_build_lists(const int* topM, int P, int M, int* lists, uint8_t* is_hub, hipStream_t st) {
+    int tot = 2 * P * M;
+    hipLaunchKernelGGL(k_build_lists, dim3((tot + 255) / 256), dim3(256), 0, st, topM, P, M, lists, is_hub);
+    return (int)hipGetLastError();
+}
+
+int launch_hub_count(const int* lists, int LM, const int* hub_id, int* hub_cnt, hipStream_t st) {
+    hipLaunchKernelGGL(k_hub_count, dim3((LM + 255) / 256), dim3(256), 0, st, lists, LM, hub_id, hub_cnt);
+    return (int)hipGetLastError();
+}
+
+int launch_hub_fill(const int* lists, int LM, const int* hub_id, const int* hub_ptr, int* hub_cur,
+                    int* hub_mem, hipStream_t st) {
+    hipLaunchKernelGGL(k_hub_fill, dim3((LM + 255) / 256), dim3(256), 0, st, lists, LM, hub_id, hub_ptr, hub_cur, hub_mem);
+    return (int)hipGetLastError();
+}
+
+int launch_dupmask(const int* hub_ptr, const int* hub_mem, int H, int M, int L2, int W,
+                   uint32_t* dup, hipStream_t st) {
+    dim3 blk(64, 4);
+    hipLaunchKernelGGL(k_dupmask, dim3((H + 3) / 4), blk, 0, st, hub_ptr, hub_mem, H, M, L2, W, dup);
+    return (int)hipGetLastError();
+}
+
+int launch_invmask(const int* lists, const int* topK, int P, int K, int M, int L2, int W,
+                   uint32_t* inv, hipStream_t st) {
+    long long tot = (long long)L2 * M * 2 * K;
+    hipLaunchKernelGGL(k_invmask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, lists, topK, P, K, M, L2, W, inv);
+    return (int)hipGetLastError();
+}
+
+int launch_first_mask(const int* topK, long long n, int P, int K, int M, int L2, int W,
+                      const uint32_t* dup, const int* hub_id, const int* hub_ptr,
+                      const int* hub_mem, uint32_t* F, int* grp_cnt, hipStream_t st) {
+    long long tot = n * 2 * K;
+    hipLaunchKernelGGL(k_first_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, topK, n, P, K, M, L2, W, dup, hub_id, hub_ptr, hub_mem, F, grp_cnt);
+    return (int)hipGetLastError();
+}
+
+int launch_group_fill(const int* topK, long long n, int P, int K, const int* grp_ptr, int* grp_cur,
+                      uint32_t* grp, hipStream_t st) {
+    long long tot = n * 2 * K;
+    hipLaunchKernelGGL(k_group_fill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, topK, n, P, K, grp_ptr, grp_cur, grp);
+    return (int)hipGetLastError();
+}
+
+}  // namespace sdb

@@ -1,0 +1,124 @@
+// Kernel argument structs and host launchers of the sDbscan HIP path.
+#pragma once
+#include "sdb_common.hpp"
+
+namespace sdb {
+
+constexpr int kMaxW = 8;          // top-M lists up to 256 points (8 x 32-bit words)
+constexpr int kLdsBytes = 163840;  // LDS per workgroup on gfx950
+
+struct IndexArgs {
+    const float* Xn;  // n x ldx transformed rows
+    long long n;
+    int ldx, d, metric, ker;
+    float kerL;        // ker_intervalSampling
+    const float* R;    // (ker/2) x d column-major (L1/L2), else null
+    const uint8_t* bits;  // 3 x D sign bits
+    int P, K;
+    int* topK;         // n x 2K (mode 0 output)
+    int* tie_count;
+    int tie_cap;
+    int* tie_idx;
+    float* tie_vals;   // tie_cap x P
+    uint32_t* chunk_max;  // G x P keys
+    uint32_t* chunk_min;
+    const uint32_t* thr_hi;  // P keys
+    const uint32_t* thr_lo;
+    int* cand_cnt;     // 2P (row * 2 + side)
+    int cand_cap;
+    uint32_t* cand_key;
+    int* cand_idx;
+    const int* row_slot;  // P (mode 2)
+    float* row_buf;       // slots x n (mode 2)
+};
+
+struct DistArgs {
+    const void* X;       // rows (fp32 or bf16), stride ldx elements
+    const float* X32;    // fp32 rows (recheck / VALU path)
+    int ldx, d, metric;
+    const uint32_t* grp;  // entries a * S + s grouped by list
+    const int* tile_list; // per workgroup: list id
+    const int* tile_beg;  // per workgroup: first group position
+    const int* tile_end;  // per workgroup: one past last group position
+    const int* lists;     // L2 x M point ids
+    int M, W, S;
+    const uint32_t* F;    // first-occurrence masks (n*S x W)
+    uint32_t* E;          // hit masks (n*S x W)
+    int* colsum;          // L2 x M
+    unsigned long long* queue;  // (entry << 32) | column
+    int* qcount;
+    int qcap;
+    float eps;
+    double band;          // |d_hat - eps| < band -> recheck
+    int kc;               // dims per LDS chunk
+    int nchunk;
+    int dk;               // dims covered by the tiles (multiple of the k-step)
+};
+
+// index
+int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
+                     uint16_t* Xb, int* err, hipStream_t st);
+int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st);
+int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,
+                          uint32_t* thi, uint32_t* tlo, hipStream_t st);
+int launch_topm_select(const uint32_t* key, const int* idx, const int* cnt, int cap, int P, int M,
+                       int* topM, int* flags, hipStream_t st);
+int launch_build_lists(const int* topM, int P, int M, int* lists, uint8_t* is_hub, hipStream_t st);
+int launch_hub_count(const int* lists, int LM, const int* hub_id, int* hub_cnt, hipStream_t st);
+int launch_hub_fill(const int* lists, int LM, const int* hub_id, const int* hub_ptr, int* hub_cur,
+                    int* hub_mem, hipStream_t st);
+int launch_dupmask(const int* hub_ptr, const int* hub_mem, int H, int M, int L2, int W,
+                   uint32_t* dup, hipStream_t st);
+int launch_invmask(const int* lists, const int* topK, int P, int K, int M, int L2, int W,
+                   uint32_t* inv, hipStream_t st);
+int launch_first_mask(const int* topK, long long n, int P, int K, int M, int L2, int W,
+                      const uint32_t* dup, const int* hub_id, const int* hub_ptr,
+                      const int* hub_mem, uint32_t* F, int* grp_cnt, hipStream_t st);
+int launch_group_fill(const int* topK, long long n, int P, int K, const int* grp_ptr, int* grp_cur,
+                      uint32_t* grp, hipStream_t st);
+
+// distances
+int launch_dist_mfma(const DistArgs& a, int precision, int ntiles, hipStream_t st);
+int launch_dist_valu(const DistArgs& a, int ntiles, hipStream_t st);
+
+// scan helpers
+int scan_exclusive_i32(const int* in, int* out, long long n, int* total_out_dev, void* tmp,
+                       size_t tmp_bytes, hipStream_t st);
+size_t scan_tmp_bytes(long long n);
+int launch_is_hub_scan_prep(const uint8_t* is_hub, int* flags, long long n, hipStream_t st);
+int launch_hub_id(const uint8_t* is_hub, const int* excl, int* hub_id, long long n, hipStream_t st);
+
+// degrees, cores, clusters
+struct ClusterArgs {
+    long long n;
+    int S, W, M, P, K, L2;
+    const int* topK;
+    const int* lists;
+    const uint32_t* E;
+    const int* colsum;
+    const int* hub_id;
+    const int* hub_ptr;
+    const int* hub_mem;
+    const uint32_t* inv;
+    int minPts;
+    int* degree;
+    uint8_t* core;
+    uint32_t* corehub;   // L2 x W: list positions holding core points
+    int* parent;
+    int* root_rank;
+    int* border_min;
+    int* labels;
+    uint32_t* adj;       // L2 x M x W list-local co-hit adjacency
+    unsigned long long* n_core;
+};
+int launch_degree(const ClusterArgs& c, hipStream_t st);
+int launch_corehub(const ClusterArgs& c, hipStream_t st);
+int launch_uf_init(const ClusterArgs& c, hipStream_t st);
+int launch_uf_slots(const ClusterArgs& c, hipStream_t st);
+int launch_uf_lists(const ClusterArgs& c, hipStream_t st);
+int launch_uf_compress(const ClusterArgs& c, hipStream_t st);
+int launch_root_flags(const ClusterArgs& c, int* flags, hipStream_t st);
+int launch_border(const ClusterArgs& c, hipStream_t st);
+int launch_labels(const ClusterArgs& c, const int* rank, hipStream_t st);
+
+}  // namespace sdb

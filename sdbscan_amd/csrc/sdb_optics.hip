@@ -1,0 +1,198 @@
+// sOPTICS support and synthetic workloads on gfx950.
+//
+// Reference: rp_findCoreDist (src/sDbscan.cpp:1314-1474) keeps every
+// deduplicated (neighbour, distance) pair and takes the (minPts-1)-th smallest
+// distance as core distance; formOptics_scikit (src/sDbscan.cpp:1588-1703)
+// sets reach(j) = min over core points i processed before j with j in N(i)
+// of max(core_dist(i), d(i,j)), and -2^31 for points no core point reaches.
+#include "sdb_kernels.hpp"
+
+namespace sdb {
+
+__device__ __forceinline__ int slot_list_o(const int* topK, long long a, int s, int K, int P) {
+    int k = s >> 1;
+    return (s & 1) ? P + topK[a * 2 * K + K + k] : topK[a * 2 * K + k];
+}
+
+// Neighbourhood CSR: every hit (a -> c) lands in a's row; it lands in c's
+// row too unless a is itself a candidate of c (then c's own hit covers it).
+__global__ void k_csr_fill(ClusterArgs c, const long long* ptr, int* cursor, int* idx) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.n * c.S) return;
+    const long long a = t / c.S;
+    const int s = (int)(t % c.S);
+    const int l = slot_list_o(c.topK, a, s, c.K, c.P);
+    const int h = c.hub_id[a];
+    for (int w = 0; w < c.W; ++w) {
+        uint32_t e = c.E[(size_t)t * c.W + w];
+        if (!e) continue;
+        uint32_t mm = 0;
+        if (h >= 0)
+            for (int q = c.hub_ptr[h]; q < c.hub_ptr[h + 1]; ++q)
+                mm |= c.inv[((size_t)l * c.L2 + c.hub_mem[q] / c.M) * c.W + w];
+        while (e) {
+            const int b = __ffs(e) - 1;
+            e &= e - 1;
+            const int cp = c.lists[(size_t)l * c.M + w * 32 + b];
+            int pos = atomicAdd(&cursor[a], 1);
+            if (ptr[a] + pos < ptr[a + 1]) idx[ptr[a] + pos] = cp;
+            if (!((mm >> b) & 1u)) {
+                int pos2 = atomicAdd(&cursor[cp], 1);
+                if (ptr[cp] + pos2 < ptr[cp + 1]) idx[ptr[cp] + pos2] = (int)a;
+            }
+        }
+    }
+}
+
+// FP32 distance of every CSR entry, one wavefront per point; sums run per
+// lane with stride 64 and combine through an xor butterfly, so d(a,c) and
+// d(c,a) are bitwise equal.
+__global__ __launch_bounds__(256) void k_csr_dist(const float* X, int ldx, int d, int metric,
+                                                  long long n, const long long* ptr, const int* idx,
+                                                  float* dist) {
+    const int lane = lane_id();
+    const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long a = wv; a < n; a += nwv) {
+        const float* x = X + (size_t)a * ldx;
+        for (long long j = ptr[a]; j < ptr[a + 1]; ++j) {
+            const float* y = X + (size_t)idx[j] * ldx;
+            float s = 0.0f;
+            for (int i = lane; i < d; i += 64) {
+                const float xv = x[i], yv = y[i];
+                if (metric == kCosine) s = fmaf(xv, yv, s);
+                else if (metric == kL1) s = s + fabsf(xv - yv);
+                else if (metric == kL2) { float t = xv - yv; s = fmaf(t, t, s); }
+                else if (metric == kChi2) { if (xv > 0.0f && yv > 0.0f) s = s + xv * yv / (xv + yv); }
+                else {
+                    const float xe = xv + 1e-6f, ye = yv + 1e-6f, u = xe + ye;
+                    s = s + xe * (logf(u / xe) * 1.4426950408889634f) + ye * (logf(u / ye) * 1.4426950408889634f);
+                }
+            }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) s = s + __shfl_xor(s, m, kWave);
+            float r;
+            if (metric == kCosine) r = 1.0f - s;
+            else if (metric == kL1) r = s;
+            else if (metric == kL2) r = sqrtf(s);
+            else if (metric == kChi2) r = 1.0f - 2.0f * s;
+            else r = 1.0f - s * 0.5f;
+            if (lane == 0) dist[j] = r;
+        }
+    }
+}
+
+__global__ void k_kth(long long n, const long long* ptr, const float* sorted, const uint8_t* core,
+                      int k, float* out) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    float v = 0.0f;
+    if (core[a] && ptr[a + 1] - ptr[a] > k) v = sorted[ptr[a] + k];
+    out[a] = v;
+}
+
+__global__ void k_reach(long long n, const long long* ptr, const int* idx, const float* dist,
+                        const uint8_t* core, const float* core_dist, const int* pos,
+                        uint32_t* reach_key) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !core[i]) return;
+    const int pi = pos[i];
+    const float cd = core_dist[i];
+    for (long long j = ptr[i]; j < ptr[i + 1]; ++j) {
+        const int q = idx[j];
+        if (pos[q] <= pi) continue;
+        const float r = fmaxf(cd, dist[j]);
+        atomicMin(&reach_key[q], ord_key(r));
+    }
+}
+
+__global__ void k_reach_final(long long n, uint32_t* reach_key) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const uint32_t k = reach_key[a];
+    const float v = k == 0xffffffffu ? -2147483648.0f : key_to_float(k);
+    reach_key[a] = __float_as_uint(v);
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic mixture (10 components): centres uniform in [0,1]^d with 80% of
+// the coordinates zeroed, point = centre + N(0, sigma^2) clipped to [0,1],
+// L2-normalised.  Counter-based (splitmix64) so any row can be regenerated.
+__device__ __forceinline__ uint64_t smix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float u24(uint64_t h) { return (float)(h >> 40) * (1.0f / 16777216.0f); }
+
+__global__ __launch_bounds__(256) void k_gen_mixture(float* X, long long n, int d, uint64_t seed,
+                                                     float sigma) {
+    const int lane = lane_id();
+    const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long p = wv; p < n; p += nwv) {
+        const int comp = (int)(smix(seed * 0x5851F42D4C957F2Dull + (uint64_t)p) >> 33) % 10;
+        float* x = X + (size_t)p * d;
+        float ss = 0.0f;
+        for (int i = lane; i < d; i += 64) {
+            const uint64_t hc = smix(seed ^ smix(0xC0FFEEull + (uint64_t)comp * 1000003ull + (uint64_t)i));
+            const float cen = u24(smix(hc)) < 0.8f ? 0.0f : u24(hc);
+            const uint64_t h1 = smix(seed ^ smix(((uint64_t)p * (uint64_t)d + (uint64_t)i) * 2ull + 1ull));
+            const uint64_t h2 = smix(h1);
+            const float u1 = ((float)(h1 >> 40) + 1.0f) * (1.0f / 16777216.0f);
+            const float u2 = u24(h2);
+            const float z = sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+            float v = cen + sigma * z;
+            v = fminf(fmaxf(v, 0.0f), 1.0f);
+            x[i] = v;
+            ss = fmaf(v, v, ss);
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) ss = ss + __shfl_xor(ss, m, kWave);
+        const float nr = sqrtf(ss);
+        if (nr > 0.0f)
+            for (int i = lane; i < d; i += 64) x[i] = x[i] / nr;
+    }
+}
+
+static inline unsigned nb(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int launch_gen_mixture(float* X, long long n, int d, unsigned long long seed, float sigma,
+                       hipStream_t st) {
+    long long blocks = (n + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_gen_mixture, dim3((unsigned)blocks), dim3(256), 0, st, X, n, d, (uint64_t)seed, sigma);
+    return (int)hipGetLastError();
+}
+
+int launch_csr_fill(const ClusterArgs& c, const long long* ptr, int* cursor, int* idx, hipStream_t st) {
+    hipLaunchKernelGGL(k_csr_fill, dim3(nb(c.n * c.S, 256)), dim3(256), 0, st, c, ptr, cursor, idx);
+    return (int)hipGetLastError();
+}
+
+int launch_csr_dist(const float* X, int ldx, int d, int metric, long long n, const long long* ptr,
+                    const int* idx, float* dist, hipStream_t st) {
+    long long blocks = (n + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_csr_dist, dim3((unsigned)blocks), dim3(256), 0, st, X, ldx, d, metric, n, ptr, idx, dist);
+    return (int)hipGetLastError();
+}
+
+int launch_kth(long long n, const long long* ptr, const float* sorted, const uint8_t* core, int k,
+               float* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_kth, dim3(nb(n, 256)), dim3(256), 0, st, n, ptr, sorted, core, k, out);
+    return (int)hipGetLastError();
+}
+
+int launch_reach(long long n, const long long* ptr, const int* idx, const float* dist,
+                 const uint8_t* core, const float* core_dist, const int* pos, uint32_t* reach_key,
+                 hipStream_t st) {
+    hipError_t e = hipMemsetAsync(reach_key, 0xff, (size_t)n * 4, st);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_reach, dim3(nb(n, 256)), dim3(256), 0, st, n, ptr, idx, dist, core, core_dist, pos, reach_key);
+    hipLaunchKernelGGL(k_reach_final, dim3(nb(n, 256)), dim3(256), 0, st, n, reach_key);
+    return (int)hipGetLastError();
+}
+
+}  // namespace sdb
