@@ -1,0 +1,48 @@
+"""The C ABI library and the drop-in module load and export their interface
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import sdbscan_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "sdbscan_amd.h")).read()
+    return sorted(set(re.findall(r"\b(sdb_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points():
+    syms = declared_symbols()
+    for s in ("sdb_create", "sdb_build_index", "sdb_find_core", "sdb_form_clusters",
+              "sdb_core_dist", "sdb_optics_reach", "sdb_get_index"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(sdbscan_amd.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_ctypes_signatures_cover_the_header():
+    assert set(declared_symbols()) == set(sdbscan_amd._SIGS)
+
+
+def test_dropin_module_surface():
+    m = sdbscan_amd.module()
+    cls = m.sDbscan
+    for name in ("set_params", "set_sngParams", "set_kernel_params", "clear", "set_topM",
+                 "set_topK", "set_threads", "set_clusterNoise", "fit_sDbscan", "test_sDbscan",
+                 "fit_sOptics", "labels_", "n_clusters_", "ordering_", "reachability_"):
+        assert hasattr(cls, name), name
+    doc = cls.set_params.__doc__
+    for kw, val in (("n_proj", "1024"), ("topK", "10"), ("topM", "50"), ("distance", "'Cosine'"),
+                    ("ker_n_features", "1024"), ("ker_sigma", "1.0"), ("clusterNoise", "0"),
+                    ("verbose", "False"), ("n_threads", "-1"), ("random_seed", "-1")):
+        assert re.search(rf"\b{kw}: [^=,]*= {re.escape(val)}", doc), kw
+    obj = cls(100, 8)
+    obj.set_params(n_proj=64, topK=3, topM=10, random_seed=1)
+    assert obj.labels_ == [] and obj.n_clusters_ == 0

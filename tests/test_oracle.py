@@ -1,0 +1,252 @@
+"""Pins of the oracle (CPU) against golden vectors, closed forms and sklearn.
+
+The reference ships no tests or fixtures for this path; these pins come from
+the reference's own compiled FFHT (tests/golden/fwht_golden.npz), the
+published generator algorithms, closed forms and special cases that reduce
+to exact DBSCAN / OPTICS.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sdbscan_amd import datagen
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+COSINE, L1, L2, CHI2, JS = 0, 1, 2, 3, 4
+
+
+def bits_eq(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+# ---------------------------------------------------------------- FFHT
+def test_ffht_matches_reference_fht_float_bitwise():
+    g = np.load(os.path.join(GOLD, "fwht_golden.npz"))
+    for log_n in range(1, 14):
+        for a, b in zip(g[f"in_{log_n}"], g[f"out_{log_n}"]):
+            assert bits_eq(O.fht(a, log_n), b), log_n
+
+
+def test_golden_signed_zero_pattern_distinguishes_textbook_butterfly():
+    # the restated FFHT forms matter: a textbook u+v/u-v transform differs in
+    # the sign of exact zeros on the all-signed-zero inputs
+    import ctypes
+    g = np.load(os.path.join(GOLD, "fwht_golden.npz"))
+    lib = O.lib()
+    differs = 0
+    for log_n in (3, 5, 8, 10):
+        a, b = g[f"in_{log_n}"][-1], g[f"out_{log_n}"][-1]
+        t = np.ascontiguousarray(a.copy())
+        lib.orc_fwht_textbook(t.ctypes.data, log_n)
+        differs += not bits_eq(t, b)
+        assert np.array_equal(t, b)  # equal as values (+0 == -0)
+    assert differs > 0
+
+
+def test_hd3_chain_matches_reference():
+    g = np.load(os.path.join(GOLD, "fwht_golden.npz"))
+    bits = g["hd3_bits_seed42"]
+    E = np.zeros((len(g["hd3_in"]), 784), np.float32)
+    E[:] = g["hd3_in"]
+    proj = O.project(E, bits, 1024, 1024)
+    assert bits_eq(proj.T, g["hd3_out"])
+
+
+def test_fwht_closed_form_sylvester():
+    for log_n in (3, 6, 10):
+        n = 1 << log_n
+        for i in (0, 1, n // 3, n - 1):
+            e = np.zeros(n, np.float32)
+            e[i] = 1.0
+            h = O.fht(e, log_n)
+            j = np.arange(n)
+            want = np.where(np.array([bin(i & x).count("1") % 2 for x in j]) == 0, 1.0, -1.0)
+            assert np.array_equal(h, want.astype(np.float32))
+        x = np.random.default_rng(log_n).standard_normal(n).astype(np.float32)
+        back = O.fht(O.fht(x, log_n), log_n) / n  # H H = n I
+        assert np.allclose(back, x, atol=1e-4)
+
+
+# ------------------------------------------------------------------ RNG
+def minstd0_bits(seed, nbits):
+    """Published algorithms: minstd_rand0 (a=16807, m=2^31-1) feeding
+    libstdc++'s uniform_int_distribution downscaling for the range [0, 1]."""
+    m = 2147483647
+    x = seed % m
+    if x == 0:
+        x = 1
+    urng_range = (m - 1) - 1  # max() - min()
+    scaling = urng_range // 2
+    past = 2 * scaling
+    out = []
+    while len(out) < nbits:
+        while True:
+            x = (16807 * x) % m
+            r = x - 1
+            if r < past:
+                break
+        out.append(r // scaling)
+    return np.array(out, np.uint8)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 42, 12345])
+def test_hd3_bits_match_published_generator(seed):
+    assert np.array_equal(O.hd3_bits(3000, seed), minstd0_bits(seed, 3000))
+
+
+def test_rng_known_answers():
+    kat = np.load(os.path.join(GOLD, "rng_kat.npz"))
+    for seed in (0, 42):
+        assert np.array_equal(O.hd3_bits(4096, seed), kat[f"bits_{seed}"])
+        assert bits_eq(O.gauss(256, 4, 0.0, 0.5, seed), kat[f"gauss_{seed}"])
+        assert bits_eq(O.cauchy(256, 4, 0.0, 0.5, seed), kat[f"cauchy_{seed}"])
+    g = O.gauss(512, 64, 0.0, 2.0, 3)
+    assert abs(g.std() - 2.0) < 0.05 and abs(g.mean()) < 0.05
+
+
+# ------------------------------------------------------------- transform
+def test_transform_cosine_and_chi2():
+    X = np.abs(np.random.default_rng(1).standard_normal((50, 33))).astype(np.float32) * 5
+    Xc, rc = O.transform(X, COSINE)
+    assert rc == 0
+    assert np.allclose(np.linalg.norm(Xc.astype(np.float64), axis=1), 1.0, atol=2e-7)
+    Xs, rc = O.transform(X, CHI2)
+    assert rc == 0 and np.allclose(Xs.astype(np.float64).sum(1), 1.0, atol=1e-6)
+    Xn = X.copy()
+    Xn[3, 4] = -1e-3
+    assert O.transform(Xn, CHI2)[1] == 1
+    assert O.transform(Xn, JS)[1] == 1
+    Xz = X.copy()
+    Xz[7] = 0
+    assert O.transform(Xz, CHI2)[1] == 2
+
+
+# ---------------------------------------------------------- selections
+def test_topk_topm_are_extreme_sets():
+    rng = np.random.default_rng(5)
+    proj = rng.standard_normal((64, 500)).astype(np.float32)
+    tk = O.topk(proj, 7)
+    for p in range(500):
+        col = proj[:, p]
+        assert set(tk[p, :7]) == set(np.argsort(-col)[:7])
+        assert set(tk[p, 7:]) == set(np.argsort(col)[:7])
+        assert np.all(np.diff(col[tk[p, :7]]) <= 0)  # close: descending
+        assert np.all(np.diff(col[tk[p, 7:]]) >= 0)  # far: ascending
+    tm = O.topm(proj, 20)
+    for r in range(64):
+        assert set(tm[r, :20]) == set(np.argsort(-proj[r])[:20])
+        assert set(tm[r, 20:]) == set(np.argsort(proj[r])[:20])
+
+
+def test_heap_select_with_ties_keeps_extremes():
+    v = np.array([3, 1, 3, 2, 3, 0, 3, 1], np.float32)
+    c, f = O.heap_select(v, 3)
+    assert sorted(v[c]) == [3, 3, 3]
+    assert sorted(v[f]) == [0, 1, 1]
+
+
+# --------------------------------------------- exact DBSCAN special case
+def full_candidate_index(n, P=64, K=2):
+    """topM = n makes every list hold every point, so Cand(a) = all others."""
+    topK = np.tile(np.concatenate([np.arange(K), np.arange(K, 2 * K)]), (n, 1)).astype(np.int32)
+    row = np.arange(n, dtype=np.int32)
+    topM = np.tile(np.concatenate([row, row]), (P, 1)).astype(np.int32)
+    return topK, topM
+
+
+def test_find_core_full_candidates_is_exact_dbscan():
+    from sklearn.cluster import DBSCAN
+    X, _ = datagen.mixture(400, 40, k=5, sigma=0.25, seed=3)
+    Xt, _ = O.transform(X, COSINE)
+    topK, topM = full_candidate_index(400)
+    eps, minPts = 0.12, 8
+    core, deg, (ptr, idx, dd), near, nd = O.find_core(Xt, COSINE, topK, topM, eps, minPts, mode=1)
+    D = 1.0 - Xt.astype(np.float64) @ Xt.astype(np.float64).T
+    np.fill_diagonal(D, np.inf)
+    assert np.array_equal(deg, (D <= eps).sum(1))
+    sk = DBSCAN(eps=eps, min_samples=minPts, metric="precomputed").fit(np.where(np.isinf(D), 0, D))
+    sk_core = np.zeros(400, np.uint8)
+    sk_core[sk.core_sample_indices_] = 1
+    assert np.array_equal(core, sk_core)
+    lab, nc = O.form_cluster(core, ptr, idx)
+    assert np.array_equal(lab, sk.labels_)
+    assert nc == (lab.max() + 2 if lab.max() >= 0 else 2)
+
+
+def rlabel(n, core, edges):
+    """Closed form of formCluster: components of the core graph numbered by
+    min core index; border = min adjacent cluster id; else -1."""
+    par = list(range(n))
+
+    def f(x):
+        while par[x] != x:
+            par[x] = par[par[x]]
+            x = par[x]
+        return x
+    for a, b in edges:
+        if core[a] and core[b]:
+            ra, rb = f(a), f(b)
+            if ra != rb:
+                par[max(ra, rb)] = min(ra, rb)
+    roots = sorted({f(a) for a in range(n) if core[a]})
+    cid = {r: i for i, r in enumerate(roots)}
+    lab = [-1] * n
+    for a in range(n):
+        if core[a]:
+            lab[a] = cid[f(a)]
+    for a, b in edges:
+        for x, y in ((a, b), (b, a)):
+            if core[x] and not core[y]:
+                c = cid[f(x)]
+                lab[y] = c if lab[y] == -1 else min(lab[y], c)
+    return np.array(lab, np.int32), (len(roots) + 1 if roots else 2)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_form_cluster_closed_form(seed):
+    rng = np.random.default_rng(seed)
+    n = 300
+    edges = {(int(a), int(b)) for a, b in rng.integers(0, n, (500, 2)) if a != b}
+    edges |= {(b, a) for a, b in edges}
+    edges = sorted(edges)
+    core = (rng.random(n) < 0.4).astype(np.uint8)
+    ptr = np.zeros(n + 1, np.int32)
+    for a, _ in edges:
+        ptr[a + 1] += 1
+    ptr = np.cumsum(ptr).astype(np.int32)
+    idx = np.array([b for _, b in edges], np.int32)
+    lab, nc = O.form_cluster(core, ptr, idx)
+    want, wnc = rlabel(n, core, edges)
+    assert np.array_equal(lab, want) and nc == wnc
+
+
+def test_optics_full_candidates_matches_sklearn_core_distances():
+    from sklearn.cluster import OPTICS
+    X, _ = datagen.mixture(300, 30, k=4, sigma=0.3, seed=9)
+    Xt, _ = O.transform(X, COSINE)
+    topK, topM = full_candidate_index(300)
+    eps, minPts = 0.2, 6
+    core, deg, (ptr, idx, dd), _, _ = O.find_core(Xt, COSINE, topK, topM, eps, minPts, mode=0)
+    oc, cd = O.core_dist(minPts, ptr, dd)
+    assert np.array_equal(oc, core)
+    D = np.clip(1.0 - Xt.astype(np.float64) @ Xt.astype(np.float64).T, 0, None)
+    sk = OPTICS(min_samples=minPts, max_eps=eps, metric="precomputed").fit(D)
+    m = core.astype(bool)
+    assert m.sum() > 50
+    assert np.allclose(cd[m], sk.core_distances_[m], atol=2e-6)
+    order, reach = O.optics(core, cd, ptr, idx, dd)
+    assert sorted(order) == list(range(300))
+    # a start point of an expansion has no reachability, others are reached
+    # from a core point placed before them
+    pos = np.empty(300, int)
+    pos[order] = np.arange(300)
+    for j in range(300):
+        preds = [(max(cd[i], dd[k]))
+                 for i in range(300) if core[i] and pos[i] < pos[j]
+                 for k in range(ptr[i], ptr[i + 1]) if idx[k] == j]
+        if preds:
+            assert reach[j] == np.float32(min(preds))
+        else:
+            assert reach[j] == np.float32(-2147483648.0)
