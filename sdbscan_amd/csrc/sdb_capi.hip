@@ -56,7 +56,7 @@ struct sdb_ctx {
     // index
     DevBuf bits, R, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
         cand_idx, topM, flags, lists, is_hub, hub_flags, hub_excl, hub_id, hub_cnt, hub_ptr,
-        hub_cur, hub_mem, dup, inv, F, grp_cnt, grp_ptr, grp_cur, grp, scan_tmp, total, err_flag,
+        hub_cur, hub_mem, inv, multi, F, wgcnt, wgoff, grp_ptr, grp, scan_tmp, total, err_flag,
         row_slot, row_buf;
     bool indexed = false;
     int H = 0, L2 = 0, W = 0, S = 0;
@@ -186,7 +186,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->tie_count, &c->cmax, &c->cmin, &c->thi, &c->tlo, &c->cand_cnt, &c->cand_key,
                      &c->cand_idx, &c->topM, &c->flags, &c->lists, &c->is_hub, &c->hub_flags,
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
-                     &c->dup, &c->inv, &c->F, &c->grp_cnt, &c->grp_ptr, &c->grp_cur, &c->grp,
+                     &c->inv, &c->multi, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
                      &c->scan_tmp, &c->total, &c->err_flag, &c->row_slot, &c->row_buf, &c->E,
                      &c->colsum, &c->queue, &c->qcount, &c->tiles, &c->degree, &c->core,
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
@@ -420,7 +420,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ENSURE(c->hub_excl, (size_t)n * 4);
     ENSURE(c->hub_id, (size_t)n * 4);
     ENSURE(c->total, 16);
-    size_t stb = std::max(scan_tmp_bytes(n), scan_tmp_bytes((long long)std::max(L2, LM) + 1));
+    size_t stb = std::max(scan_tmp_bytes(n), scan_tmp_bytes((long long)std::max(L2 * 1024, LM) + 1));
     ENSURE(c->scan_tmp, stb);
     HIPCHK(hipMemsetAsync(c->is_hub.p, 0, (size_t)n, st));
     LCHK(launch_build_lists(P_<int>(c->topM), P, M, P_<int>(c->lists), P_<uint8_t>(c->is_hub), st));
@@ -442,23 +442,23 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     LCHK(scan_exclusive_i32(P_<int>(c->hub_cnt), P_<int>(c->hub_ptr), H + 1, nullptr, c->scan_tmp.p, c->scan_tmp.bytes, st));
     LCHK(launch_hub_fill(P_<int>(c->lists), LM, P_<int>(c->hub_id), P_<int>(c->hub_ptr), P_<int>(c->hub_cur), P_<int>(c->hub_mem), st));
     const size_t tbl = (size_t)L2 * L2 * W * 4;
-    ENSURE(c->dup, tbl);
     ENSURE(c->inv, tbl);
-    HIPCHK(hipMemsetAsync(c->dup.p, 0, tbl, st));
     HIPCHK(hipMemsetAsync(c->inv.p, 0, tbl, st));
-    LCHK(launch_dupmask(P_<int>(c->hub_ptr), P_<int>(c->hub_mem), H, M, L2, W, P_<uint32_t>(c->dup), st));
     LCHK(launch_invmask(P_<int>(c->lists), P_<int>(c->topK), P, K, M, L2, W, P_<uint32_t>(c->inv), st));
+    ENSURE(c->multi, (size_t)L2 * W * 4);
+    LCHK(launch_multi(P_<int>(c->lists), P_<int>(c->hub_id), P_<int>(c->hub_ptr), L2, M, W, P_<uint32_t>(c->multi), st));
     ENSURE(c->F, (size_t)n * S * W * 4);
-    ENSURE(c->grp_cnt, (size_t)(L2 + 1) * 4);
+    LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, W, P_<uint32_t>(c->multi), P_<int>(c->lists),
+                           P_<int>(c->hub_id), P_<int>(c->hub_ptr), P_<int>(c->hub_mem), P_<uint32_t>(c->F), st));
+    // group the (point, slot) entries by list
+    const int Gg = (int)std::max<long long>(1, std::min<long long>(1024, (n * S + 4095) / 4096));
+    ENSURE(c->wgcnt, (size_t)L2 * Gg * 4);
+    ENSURE(c->wgoff, (size_t)L2 * Gg * 4);
     ENSURE(c->grp_ptr, (size_t)(L2 + 1) * 4);
-    ENSURE(c->grp_cur, (size_t)(L2 + 1) * 4);
     ENSURE(c->grp, (size_t)n * S * 4);
-    HIPCHK(hipMemsetAsync(c->grp_cnt.p, 0, (size_t)(L2 + 1) * 4, st));
-    HIPCHK(hipMemsetAsync(c->grp_cur.p, 0, (size_t)(L2 + 1) * 4, st));
-    LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, L2, W, P_<uint32_t>(c->dup), P_<int>(c->hub_id),
-                           P_<int>(c->hub_ptr), P_<int>(c->hub_mem), P_<uint32_t>(c->F), P_<int>(c->grp_cnt), st));
-    LCHK(scan_exclusive_i32(P_<int>(c->grp_cnt), P_<int>(c->grp_ptr), L2 + 1, nullptr, c->scan_tmp.p, c->scan_tmp.bytes, st));
-    LCHK(launch_group_fill(P_<int>(c->topK), n, P, K, P_<int>(c->grp_ptr), P_<int>(c->grp_cur), P_<uint32_t>(c->grp), st));
+    LCHK(launch_group_count(P_<int>(c->topK), n, P, K, L2, Gg, P_<int>(c->wgcnt), st));
+    LCHK(scan_exclusive_i32(P_<int>(c->wgcnt), P_<int>(c->wgoff), (long long)L2 * Gg, nullptr, c->scan_tmp.p, c->scan_tmp.bytes, st));
+    LCHK(launch_group_scatter(P_<int>(c->topK), n, P, K, L2, Gg, P_<int>(c->wgoff), P_<uint32_t>(c->grp), P_<int>(c->grp_ptr), st));
     c->h_grp_ptr.resize(L2 + 1);
     HIPCHK(hipMemcpyAsync(c->h_grp_ptr.data(), c->grp_ptr.p, (size_t)(L2 + 1) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(c->ev[2], st));
@@ -671,6 +671,7 @@ int sdb_form_clusters(sdb_ctx* c, int* labels_out, int* n_clusters_out) {
     LCHK(launch_uf_init(ca, st));
     LCHK(launch_uf_slots(ca, st));
     LCHK(launch_uf_lists(ca, st));
+    LCHK(launch_uf_points(ca, st));
     LCHK(launch_uf_compress(ca, st));
     LCHK(launch_root_flags(ca, P_<int>(c->rootf), st));
     LCHK(scan_exclusive_i32(P_<int>(c->rootf), P_<int>(c->rank), n, P_<int>(c->total), c->scan_tmp.p, c->scan_tmp.bytes, st));

@@ -114,10 +114,11 @@ __global__ void k_uf_init(ClusterArgs c) {
     c.border_min[a] = 0x7fffffff;
 }
 
-// Per (core point, slot): all core hits of the slot's list join the point.
-// The point is united with the first of them; the list-local star edges
-// (first -> all) are recorded in adj and united by k_uf_lists.
-__global__ void k_uf_slots(ClusterArgs c) {
+// Per (core point, slot): all core hits of the slot's list are connected
+// through the point.  adj[l][first] collects them (star edges from the first
+// hit) for the list-local unions of k_uf_lists; k_uf_points then joins each
+// point to the first hit of each slot.
+__global__ void k_uf_adj(ClusterArgs c) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= c.n * c.S) return;
     const long long a = t / c.S;
@@ -131,10 +132,32 @@ __global__ void k_uf_slots(ClusterArgs c) {
         if (first < 0 && m[w]) first = w * 32 + __ffs(m[w]) - 1;
     }
     if (first < 0) return;
-    uf_union(c.parent, (int)a, c.lists[(size_t)l * c.M + first]);
+    // only the hits after the first one add edges
+    m[first >> 5] &= ~(1u << (first & 31));
     uint32_t* ad = c.adj + ((size_t)l * c.M + first) * c.W;
     for (int w = 0; w < c.W; ++w)
         if (m[w] && (load_relaxed((const int*)&ad[w]) & m[w]) != m[w]) atomicOr(&ad[w], m[w]);
+}
+
+__global__ void k_uf_points(ClusterArgs c) {
+    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= c.n || !c.core[a]) return;
+    int ra = uf_find(c.parent, (int)a);
+    for (int s = 0; s < c.S; ++s) {
+        const int l = slot_list_c(c.topK, a, s, c.K, c.P);
+        const size_t t = (size_t)a * c.S + s;
+        int first = -1;
+        for (int w = 0; w < c.W && first < 0; ++w) {
+            const uint32_t m = c.E[t * c.W + w] & c.corehub[(size_t)l * c.W + w];
+            if (m) first = w * 32 + __ffs(m) - 1;
+        }
+        if (first < 0) continue;
+        const int x = c.lists[(size_t)l * c.M + first];
+        if (uf_find(c.parent, x) != ra) {
+            uf_union(c.parent, (int)a, x);
+            ra = uf_find(c.parent, (int)a);
+        }
+    }
 }
 
 __global__ void k_uf_lists(ClusterArgs c) {
@@ -219,7 +242,11 @@ int launch_uf_init(const ClusterArgs& c, hipStream_t st) {
     return (int)hipGetLastError();
 }
 int launch_uf_slots(const ClusterArgs& c, hipStream_t st) {
-    hipLaunchKernelGGL(k_uf_slots, dim3(nblk(c.n * c.S, 256)), dim3(256), 0, st, c);
+    hipLaunchKernelGGL(k_uf_adj, dim3(nblk(c.n * c.S, 256)), dim3(256), 0, st, c);
+    return (int)hipGetLastError();
+}
+int launch_uf_points(const ClusterArgs& c, hipStream_t st) {
+    hipLaunchKernelGGL(k_uf_points, dim3(nblk(c.n, 256)), dim3(256), 0, st, c);
     return (int)hipGetLastError();
 }
 int launch_uf_lists(const ClusterArgs& c, hipStream_t st) {

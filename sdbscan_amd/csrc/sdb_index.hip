@@ -54,32 +54,44 @@ __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, 
 }
 
 // ---------------------------------------------------------------------------
-// One FFHT level on a wavefront-resident vector.  Element e of the D-vector
-// lives in lane (e & 63), register j = e >> 6.  Strides < 64 exchange across
-// lanes (lane ^ s); strides >= 64 pair registers j and j + s/64.  Butterfly
-// forms follow the FFHT kernels (src/fht_avx.c:33-56, 1013-1251): stride 1
-// computes lo = a - (0 - b), hi = a + (0 - b); strides 2 and 4 compute
-// lo = a + b, hi = a + (0 - b); wider strides compute a + b, a - b.
-template <int VPL>
-__device__ __forceinline__ void fwht_cross(float (&v)[VPL], int s, int lane) {
-    const bool upper = (lane & s) != 0;
+// Cross-lane exchange with lane ^ S without the LDS crossbar: DPP quad
+// permutes for S = 1, 2; two DPP row permutes (half-mirror / mirror, then a
+// quad or half-mirror permute) for S = 4, 8; the gfx950 permlane swaps for
+// S = 16, 32.
+template <int S>
+__device__ __forceinline__ uint32_t xchg(uint32_t v, int lane) {
+    const int iv = (int)v;
+    if constexpr (S == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, iv, 0xB1, 0xF, 0xF, false);
+    if constexpr (S == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, iv, 0x4E, 0xF, 0xF, false);
+    if constexpr (S == 4) {
+        const int h = __builtin_amdgcn_update_dpp(0, iv, 0x141, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, h, 0x1B, 0xF, 0xF, false);
+    }
+    if constexpr (S == 8) {
+        const int m = __builtin_amdgcn_update_dpp(0, iv, 0x140, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, m, 0x141, 0xF, 0xF, false);
+    }
+    if constexpr (S == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16) ? p[0] : p[1];
+    }
+    if constexpr (S == 32) {
+        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (lane & 32) ? p[0] : p[1];
+    }
+    return v;
+}
+
+// One Walsh-Hadamard level across lanes (stride S < 64): the lower element
+// becomes a + b, the upper a - b, computed as fma(+-1, mine, partner) with an
+// exact product, i.e. one rounding of partner +- mine.
+template <int S, int VPL>
+__device__ __forceinline__ void fwht_cross(float (&v)[VPL], int lane) {
+    const float sg = (lane & S) ? -1.0f : 1.0f;
 #pragma unroll
     for (int j = 0; j < VPL; ++j) {
-        float mine = v[j];
-        float other = __shfl_xor(mine, s, kWave);
-        float a = upper ? other : mine;   // lower element
-        float b = upper ? mine : other;   // upper element
-        float r;
-        if (s == 1) {
-            float t = 0.0f - b;
-            r = upper ? a + t : a - t;
-        } else if (s == 2 || s == 4) {
-            float t = 0.0f - b;
-            r = upper ? a + t : a + b;
-        } else {
-            r = upper ? a - b : a + b;
-        }
-        v[j] = r;
+        const float o = __uint_as_float(xchg<S>(__float_as_uint(v[j]), lane));
+        v[j] = fmaf(sg, v[j], o);
     }
 }
 
@@ -95,11 +107,20 @@ __device__ __forceinline__ void fwht_inreg(float (&v)[VPL], int js) {
     }
 }
 
+// Unnormalised FWHT with increasing strides (the level order of FFHT,
+// src/fht_avx.c:1013-1251).  Element e lives in lane (e & 63), register e >> 6.
+// Outputs equal FFHT's as IEEE values; FFHT's (0 - v) butterflies at strides
+// 1, 2, 4 can only change the sign of an exact zero, which no comparison of
+// the index distinguishes.
 template <int LOGD>
 __device__ __forceinline__ void ffht_wave(float (&v)[(1 << LOGD) / 64], int lane) {
     constexpr int VPL = (1 << LOGD) / 64;
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) fwht_cross<VPL>(v, s, lane);
+    fwht_cross<1, VPL>(v, lane);
+    fwht_cross<2, VPL>(v, lane);
+    fwht_cross<4, VPL>(v, lane);
+    fwht_cross<8, VPL>(v, lane);
+    fwht_cross<16, VPL>(v, lane);
+    fwht_cross<32, VPL>(v, lane);
 #pragma unroll
     for (int js = 1; js < VPL; js <<= 1) fwht_inreg<VPL>(v, js);
 }
@@ -189,6 +210,92 @@ __device__ __forceinline__ void hd3_project(const IndexArgs& a, long long p, int
     }
 }
 
+// Bitonic sorting network across the 64 lanes of a wavefront, descending:
+// stage k merges blocks of size k; lanes with (lane & k) == 0 sort their
+// block descending, so after the last stage lane 0 holds the largest key.
+template <int J>
+__device__ __forceinline__ void bitonic_u32(uint32_t& x, int lane, int k) {
+    const uint32_t p = xchg<J>(x, lane);
+    const bool keep_max = ((lane & J) == 0) == ((lane & k) == 0);
+    x = keep_max ? (p > x ? p : x) : (p < x ? p : x);
+}
+template <int J>
+__device__ __forceinline__ void bitonic_u64(uint32_t& h, uint32_t& lo, int lane, int k) {
+    const uint32_t ph = xchg<J>(h, lane), pl = xchg<J>(lo, lane);
+    const bool pg = ph > h || (ph == h && pl > lo);
+    const bool keep_max = ((lane & J) == 0) == ((lane & k) == 0);
+    const bool take = keep_max ? pg : !pg && (ph != h || pl != lo);
+    h = take ? ph : h;
+    lo = take ? pl : lo;
+}
+template <int K2, int J>
+__device__ __forceinline__ void bitonic_pass_u32(uint32_t& x, int lane) {
+    bitonic_u32<J>(x, lane, K2);
+    if constexpr (J > 1) bitonic_pass_u32<K2, J / 2>(x, lane);
+}
+template <int K2>
+__device__ __forceinline__ void bitonic_sort_u32(uint32_t& x, int lane) {
+    if constexpr (K2 > 2) bitonic_sort_u32<K2 / 2>(x, lane);
+    bitonic_pass_u32<K2, K2 / 2>(x, lane);
+}
+template <int K2, int J>
+__device__ __forceinline__ void bitonic_pass_u64(uint32_t& h, uint32_t& lo, int lane) {
+    bitonic_u64<J>(h, lo, lane, K2);
+    if constexpr (J > 1) bitonic_pass_u64<K2, J / 2>(h, lo, lane);
+}
+template <int K2>
+__device__ __forceinline__ void bitonic_sort_u64(uint32_t& h, uint32_t& lo, int lane) {
+    if constexpr (K2 > 2) bitonic_sort_u64<K2 / 2>(h, lo, lane);
+    bitonic_pass_u64<K2, K2 / 2>(h, lo, lane);
+}
+
+__device__ __forceinline__ int lanes_below(unsigned long long b) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
+// K largest keys (value desc, index asc) of the first P elements, K <= 64.
+// T0 = K-th largest lane maximum bounds the K-th largest value from below, so
+// every element with value >= T0 (ties with the K-th value included) is a
+// candidate.  Up to 64 candidates are compacted with ballots and sorted with
+// one 64-lane bitonic network; more candidates take the general path.
+// Returns true when the (K+1)-th largest value equals the K-th.
+template <int VPL>
+__device__ bool wave_topk_general(const uint32_t (&key)[VPL], int P, int K, int lane, uint64_t* lds,
+                                  int* out);
+
+template <int VPL>
+__device__ bool wave_topk(const uint32_t (&key)[VPL], int P, int K, int lane, uint64_t* lds,
+                          int* out) {
+    uint32_t lm = 0;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j)
+        if (lane + 64 * j < P) lm = key[j] > lm ? key[j] : lm;
+    bitonic_sort_u32<64>(lm, lane);
+    const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)lm, K - 1);
+    uint32_t* sh = (uint32_t*)lds;
+    int total = 0;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) {
+        const int e = lane + 64 * j;
+        const bool c = e < P && key[j] >= t0;
+        const unsigned long long b = __ballot(c);
+        const int pos = total + lanes_below(b);
+        if (c && pos < 64) { sh[pos] = key[j]; sh[64 + pos] = 0xffffffffu - (uint32_t)e; }
+        total += __popcll(b);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (total > 64) return wave_topk_general<VPL>(key, P, K, lane, lds, out);
+    uint32_t h = lane < total ? sh[lane] : 0u;
+    uint32_t lo = lane < total ? sh[64 + lane] : 0u;
+    __builtin_amdgcn_wave_barrier();
+    bitonic_sort_u64<64>(h, lo, lane);
+    if (lane < K) out[lane] = (int)(0xffffffffu - lo);
+    if (total <= K) return false;
+    const uint32_t vk = (uint32_t)__builtin_amdgcn_readlane((int)h, K - 1);
+    const uint32_t vn = (uint32_t)__builtin_amdgcn_readlane((int)h, K);
+    return vk == vn;
+}
+
 // Wave-level selection of the K largest keys (value desc, index asc) among
 // the first P elements.  T0 = K-th largest lane maximum bounds the K-th
 // largest element from below, so only elements whose value is >= value(T0)
@@ -196,8 +303,8 @@ __device__ __forceinline__ void hd3_project(const IndexArgs& a, long long p, int
 // out[0..K); returns true when the (K+1)-th largest value equals the K-th
 // (a tie across the selection boundary).
 template <int VPL>
-__device__ bool wave_topk(const uint32_t (&key)[VPL], int P, int K, int lane, uint64_t* lds,
-                          int* out) {
+__device__ bool wave_topk_general(const uint32_t (&key)[VPL], int P, int K, int lane, uint64_t* lds,
+                                  int* out) {
     uint64_t lm = 0;
 #pragma unroll
     for (int j = 0; j < VPL; ++j) {
@@ -527,23 +634,6 @@ __global__ void k_hub_fill(const int* __restrict__ lists, int LM, const int* __r
     hub_mem[hub_ptr[h] + pos] = t;  // t = l * M + i
 }
 
-// dup[l1][l2] bit i set when position i of list l2 holds a point that is also
-// in list l1 (l1 != l2).  Used for first-occurrence masks.
-__global__ void k_dupmask(const int* __restrict__ hub_ptr, const int* __restrict__ hub_mem, int H,
-                          int M, int L2, int W, uint32_t* __restrict__ dup) {
-    int h = blockIdx.x * blockDim.y + threadIdx.y;
-    if (h >= H) return;
-    int b = hub_ptr[h], e = hub_ptr[h + 1];
-    int t = e - b;
-    for (int q = threadIdx.x; q < t * t; q += blockDim.x) {
-        int x = q / t, y = q % t;
-        if (x == y) continue;
-        int mx = hub_mem[b + x], my = hub_mem[b + y];
-        int l1 = mx / M, l2 = my / M, i2 = my % M;
-        atomicOr(&dup[((size_t)l1 * L2 + l2) * W + (i2 >> 5)], 1u << (i2 & 31));
-    }
-}
-
 // inv[l][L] bit i set when list L is one of the 2K lists of the point at
 // position i of list l.  Used for the mutual-pair correction of hub degrees.
 __global__ void k_invmask(const int* __restrict__ lists, const int* __restrict__ topK, int P, int K,
@@ -559,15 +649,33 @@ __global__ void k_invmask(const int* __restrict__ lists, const int* __restrict__
     atomicOr(&inv[((size_t)l * L2 + L) * W + (i >> 5)], 1u << (i & 31));
 }
 
+// multi[l][w] bit i: the point at position i of list l occurs in at least two
+// lists, so it can repeat among a point's slots.
+__global__ void k_multi(const int* __restrict__ lists, const int* __restrict__ hub_id,
+                        const int* __restrict__ hub_ptr, int L2, int M, int W,
+                        uint32_t* __restrict__ multi) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L2 * W) return;
+    int l = t / W, w = t % W;
+    uint32_t m = 0;
+    for (int b = 0; b < 32; ++b) {
+        int i = w * 32 + b;
+        if (i >= M) break;
+        int h = hub_id[lists[(size_t)l * M + i]];
+        if (hub_ptr[h + 1] - hub_ptr[h] >= 2) m |= 1u << b;
+    }
+    multi[t] = m;
+}
+
 // First-occurrence mask per (point, slot): bit i of F[a][s] is set when the
 // point at position i of the slot's list is not a itself and does not occur
 // in an earlier slot of a (the reference's per-point dedup bitset,
-// src/sDbscan.cpp:260-282).  Also counts slot entries per list for grouping.
-__global__ void k_first_mask(const int* __restrict__ topK, long long n, int P, int K, int M, int L2,
-                             int W, const uint32_t* __restrict__ dup,
+// src/sDbscan.cpp:260-282).  Only multi positions can repeat; each is checked
+// against the lists of the earlier slots through its hub memberships.
+__global__ void k_first_mask(const int* __restrict__ topK, long long n, int P, int K, int M, int W,
+                             const uint32_t* __restrict__ multi, const int* __restrict__ lists,
                              const int* __restrict__ hub_id, const int* __restrict__ hub_ptr,
-                             const int* __restrict__ hub_mem, uint32_t* __restrict__ F,
-                             int* __restrict__ grp_cnt) {
+                             const int* __restrict__ hub_mem, uint32_t* __restrict__ F) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int S = 2 * K;
     if (t >= n * S) return;
@@ -576,14 +684,27 @@ __global__ void k_first_mask(const int* __restrict__ topK, long long n, int P, i
     int l = slot_list(topK, a, s, K, P);
     uint32_t m[kMaxW];
     for (int w = 0; w < W; ++w) {
-        int lo = w * 32;
-        int nb = M - lo;
-        m[w] = nb >= 32 ? 0xffffffffu : (nb > 0 ? ((1u << nb) - 1u) : 0u);
+        int nb = M - w * 32;
+        m[w] = nb >= 32 ? 0xffffffffu : ((1u << nb) - 1u);
     }
-    for (int s2 = 0; s2 < s; ++s2) {
-        int l2 = slot_list(topK, a, s2, K, P);
-        const uint32_t* dp = dup + ((size_t)l2 * L2 + l) * W;
-        for (int w = 0; w < W; ++w) m[w] &= ~dp[w];
+    if (s > 0) {
+        for (int w = 0; w < W; ++w) {
+            uint32_t mw = multi[(size_t)l * W + w];
+            while (mw) {
+                const int b = __ffs(mw) - 1;
+                mw &= mw - 1;
+                const int i = w * 32 + b;
+                const int h = hub_id[lists[(size_t)l * M + i]];
+                bool dup = false;
+                for (int q = hub_ptr[h]; q < hub_ptr[h + 1] && !dup; ++q) {
+                    const int L = hub_mem[q] / M;
+                    if (L == l) continue;
+                    for (int s2 = 0; s2 < s; ++s2)
+                        if (slot_list(topK, a, s2, K, P) == L) { dup = true; break; }
+                }
+                if (dup) m[w] &= ~(1u << b);
+            }
+        }
     }
     int h = hub_id[a];
     if (h >= 0) {
@@ -593,21 +714,51 @@ __global__ void k_first_mask(const int* __restrict__ topK, long long n, int P, i
         }
     }
     for (int w = 0; w < W; ++w) F[t * W + w] = m[w];
-    atomicAdd(&grp_cnt[l], 1);
 }
 
-// Group (point, slot) entries by list: grp[grp_ptr[l] + k] = a * 2K + s.
-__global__ void k_group_fill(const int* __restrict__ topK, long long n, int P, int K,
-                             const int* __restrict__ grp_ptr, int* __restrict__ grp_cur,
-                             uint32_t* __restrict__ grp) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int S = 2 * K;
-    if (t >= n * S) return;
-    long long a = t / S;
-    int s = (int)(t % S);
-    int l = slot_list(topK, a, s, K, P);
-    int pos = atomicAdd(&grp_cur[l], 1);
-    grp[grp_ptr[l] + pos] = (uint32_t)t;
+// Grouping of the (point, slot) entries by list without contended global
+// atomics: each workgroup counts its contiguous range of entries per list in
+// LDS (k_group_count), the counts are scanned list-major, and the workgroup
+// scatters its entries from LDS cursors (k_group_scatter).
+__global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ topK, long long n, int P,
+                                                     int K, int L2, int G, int* __restrict__ wgcnt) {
+    extern __shared__ int hist[];
+    const long long tot = n * 2 * K;
+    const long long per = (tot + G - 1) / G;
+    const long long t0 = (long long)blockIdx.x * per, t1 = t0 + per < tot ? t0 + per : tot;
+    for (int i = threadIdx.x; i < L2; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+        const long long a = t / (2 * K);
+        const int s = (int)(t % (2 * K));
+        atomicAdd(&hist[slot_list(topK, a, s, K, P)], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < L2; i += blockDim.x) wgcnt[(size_t)i * G + blockIdx.x] = hist[i];
+}
+
+__global__ __launch_bounds__(256) void k_group_scatter(const int* __restrict__ topK, long long n, int P,
+                                                       int K, int L2, int G, const int* __restrict__ wgoff,
+                                                       uint32_t* __restrict__ grp) {
+    extern __shared__ int cur[];
+    const long long tot = n * 2 * K;
+    const long long per = (tot + G - 1) / G;
+    const long long t0 = (long long)blockIdx.x * per, t1 = t0 + per < tot ? t0 + per : tot;
+    for (int i = threadIdx.x; i < L2; i += blockDim.x) cur[i] = wgoff[(size_t)i * G + blockIdx.x];
+    __syncthreads();
+    for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+        const long long a = t / (2 * K);
+        const int s = (int)(t % (2 * K));
+        const int pos = atomicAdd(&cur[slot_list(topK, a, s, K, P)], 1);
+        grp[pos] = (uint32_t)t;
+    }
+}
+
+__global__ void k_group_ptr(const int* __restrict__ wgoff, int L2, int G, long long tot,
+                            int* __restrict__ grp_ptr) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < L2) grp_ptr[l] = wgoff[(size_t)l * G];
+    if (l == L2) grp_ptr[L2] = (int)tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -692,13 +843,6 @@ int launch_hub_fill(const int* lists, int LM, const int* hub_id, const int* hub_
     return (int)hipGetLastError();
 }
 
-int launch_dupmask(const int* hub_ptr, const int* hub_mem, int H, int M, int L2, int W,
-                   uint32_t* dup, hipStream_t st) {
-    dim3 blk(64, 4);
-    hipLaunchKernelGGL(k_dupmask, dim3((H + 3) / 4), blk, 0, st, hub_ptr, hub_mem, H, M, L2, W, dup);
-    return (int)hipGetLastError();
-}
-
 int launch_invmask(const int* lists, const int* topK, int P, int K, int M, int L2, int W,
                    uint32_t* inv, hipStream_t st) {
     long long tot = (long long)L2 * M * 2 * K;
@@ -706,18 +850,29 @@ int launch_invmask(const int* lists, const int* topK, int P, int K, int M, int L
     return (int)hipGetLastError();
 }
 
-int launch_first_mask(const int* topK, long long n, int P, int K, int M, int L2, int W,
-                      const uint32_t* dup, const int* hub_id, const int* hub_ptr,
-                      const int* hub_mem, uint32_t* F, int* grp_cnt, hipStream_t st) {
-    long long tot = n * 2 * K;
-    hipLaunchKernelGGL(k_first_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, topK, n, P, K, M, L2, W, dup, hub_id, hub_ptr, hub_mem, F, grp_cnt);
+int launch_multi(const int* lists, const int* hub_id, const int* hub_ptr, int L2, int M, int W,
+                 uint32_t* multi, hipStream_t st) {
+    hipLaunchKernelGGL(k_multi, dim3((L2 * W + 255) / 256), dim3(256), 0, st, lists, hub_id, hub_ptr, L2, M, W, multi);
     return (int)hipGetLastError();
 }
 
-int launch_group_fill(const int* topK, long long n, int P, int K, const int* grp_ptr, int* grp_cur,
-                      uint32_t* grp, hipStream_t st) {
+int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const uint32_t* multi,
+                      const int* lists, const int* hub_id, const int* hub_ptr, const int* hub_mem,
+                      uint32_t* F, hipStream_t st) {
     long long tot = n * 2 * K;
-    hipLaunchKernelGGL(k_group_fill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, topK, n, P, K, grp_ptr, grp_cur, grp);
+    hipLaunchKernelGGL(k_first_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, topK, n, P, K, M, W, multi, lists, hub_id, hub_ptr, hub_mem, F);
+    return (int)hipGetLastError();
+}
+
+int launch_group_count(const int* topK, long long n, int P, int K, int L2, int G, int* wgcnt, hipStream_t st) {
+    hipLaunchKernelGGL(k_group_count, dim3(G), dim3(256), (size_t)L2 * 4, st, topK, n, P, K, L2, G, wgcnt);
+    return (int)hipGetLastError();
+}
+
+int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, int G, const int* wgoff,
+                         uint32_t* grp, int* grp_ptr, hipStream_t st) {
+    hipLaunchKernelGGL(k_group_scatter, dim3(G), dim3(256), (size_t)L2 * 4, st, topK, n, P, K, L2, G, wgoff, grp);
+    hipLaunchKernelGGL(k_group_ptr, dim3((L2 + 1 + 255) / 256), dim3(256), 0, st, wgoff, L2, G, n * 2 * K, grp_ptr);
     return (int)hipGetLastError();
 }
 

@@ -67,15 +67,16 @@ int launch_build_lists(const int* topM, int P, int M, int* lists, uint8_t* is_hu
 int launch_hub_count(const int* lists, int LM, const int* hub_id, int* hub_cnt, hipStream_t st);
 int launch_hub_fill(const int* lists, int LM, const int* hub_id, const int* hub_ptr, int* hub_cur,
                     int* hub_mem, hipStream_t st);
-int launch_dupmask(const int* hub_ptr, const int* hub_mem, int H, int M, int L2, int W,
-                   uint32_t* dup, hipStream_t st);
 int launch_invmask(const int* lists, const int* topK, int P, int K, int M, int L2, int W,
                    uint32_t* inv, hipStream_t st);
-int launch_first_mask(const int* topK, long long n, int P, int K, int M, int L2, int W,
-                      const uint32_t* dup, const int* hub_id, const int* hub_ptr,
-                      const int* hub_mem, uint32_t* F, int* grp_cnt, hipStream_t st);
-int launch_group_fill(const int* topK, long long n, int P, int K, const int* grp_ptr, int* grp_cur,
-                      uint32_t* grp, hipStream_t st);
+int launch_multi(const int* lists, const int* hub_id, const int* hub_ptr, int L2, int M, int W,
+                 uint32_t* multi, hipStream_t st);
+int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const uint32_t* multi,
+                      const int* lists, const int* hub_id, const int* hub_ptr, const int* hub_mem,
+                      uint32_t* F, hipStream_t st);
+int launch_group_count(const int* topK, long long n, int P, int K, int L2, int G, int* wgcnt, hipStream_t st);
+int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, int G, const int* wgoff,
+                         uint32_t* grp, int* grp_ptr, hipStream_t st);
 
 // distances
 int launch_dist_mfma(const DistArgs& a, int precision, int ntiles, hipStream_t st);
@@ -116,6 +117,7 @@ int launch_corehub(const ClusterArgs& c, hipStream_t st);
 int launch_uf_init(const ClusterArgs& c, hipStream_t st);
 int launch_uf_slots(const ClusterArgs& c, hipStream_t st);
 int launch_uf_lists(const ClusterArgs& c, hipStream_t st);
+int launch_uf_points(const ClusterArgs& c, hipStream_t st);
 int launch_uf_compress(const ClusterArgs& c, hipStream_t st);
 int launch_root_flags(const ClusterArgs& c, int* flags, hipStream_t st);
 int launch_border(const ClusterArgs& c, hipStream_t st);
