@@ -54,7 +54,7 @@ struct sdb_ctx {
     // transformed
     DevBuf Xn, Xb;
     // index
-    DevBuf bits, R, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
+    DevBuf bits, R, thr_any, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
         cand_idx, topM, flags, lists, is_hub, hub_flags, hub_excl, hub_id, hub_cnt, hub_ptr,
         hub_cur, hub_mem, inv, multi, F, wgcnt, wgoff, grp_ptr, grp, scan_tmp, total, err_flag,
         row_slot, row_buf;
@@ -182,7 +182,7 @@ void sdb_destroy(sdb_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
-    DevBuf* all[] = {&c->Xown, &c->Xn, &c->Xb, &c->bits, &c->R, &c->topK, &c->tie_idx, &c->tie_vals,
+    DevBuf* all[] = {&c->Xown, &c->Xn, &c->Xb, &c->bits, &c->R, &c->thr_any, &c->topK, &c->tie_idx, &c->tie_vals,
                      &c->tie_count, &c->cmax, &c->cmin, &c->thi, &c->tlo, &c->cand_cnt, &c->cand_key,
                      &c->cand_idx, &c->topM, &c->flags, &c->lists, &c->is_hub, &c->hub_flags,
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
@@ -299,40 +299,56 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     hipStream_t st = c->st;
 
     HIPCHK(hipEventRecord(c->ev[0], st));
-    // ---- transform (src/Utilities.h:289-335)
+    const bool cos = q.metric == SDB_COSINE;
+    // ---- transform (src/Utilities.h:289-335); fused into the index passes for Cosine
     ENSURE(c->Xn, (size_t)n * ldx * 4);
     if (q.precision == SDB_BF16) ENSURE(c->Xb, (size_t)n * ldx * 2);
     ENSURE(c->err_flag, 4);
     HIPCHK(hipMemsetAsync(c->err_flag.p, 0, 4, st));
-    LCHK(launch_transform(c->X, n, d, ldx, q.metric, P_<float>(c->Xn),
-                          q.precision == SDB_BF16 ? P_<uint16_t>(c->Xb) : nullptr, P_<int>(c->err_flag), st));
     int errf = 0;
-    HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    if (!cos) {
+        LCHK(launch_transform(c->X, n, d, ldx, q.metric, P_<float>(c->Xn),
+                              q.precision == SDB_BF16 ? P_<uint16_t>(c->Xb) : nullptr, P_<int>(c->err_flag), st));
+        HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
+        if (errf & 2) return fail(c, SDB_EZERO, "Error: There is an zero point !");
+    }
     HIPCHK(hipEventRecord(c->ev[1], st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
-    if (errf & 2) return fail(c, SDB_EZERO, "Error: There is an zero point !");
 
-    // ---- random structures
-    ENSURE(c->bits, (size_t)3 * D);
-    HIPCHK(hipMemcpyAsync(c->bits.p, hd3_bits, (size_t)3 * D, hipMemcpyHostToDevice, st));
+    // ---- random structures: per-lane sign masks of the 3 x D sign bits
+    const int VPL = D / 64, NWs = (VPL + 31) / 32;
+    std::vector<uint32_t> smask((size_t)3 * NWs * 64, 0u);
+    for (int r = 0; r < 3; ++r)
+        for (int w = 0; w < NWs; ++w)
+            for (int lane = 0; lane < 64; ++lane) {
+                uint32_t m = 0;
+                for (int b = 0; b < 32 && w * 32 + b < VPL; ++b)
+                    m |= (uint32_t)(hd3_bits[(size_t)r * D + lane + 64 * (w * 32 + b)] & 1u) << b;
+                smask[((size_t)r * NWs + w) * 64 + lane] = m;
+            }
+    ENSURE(c->bits, smask.size() * 4);
+    HIPCHK(hipMemcpyAsync(c->bits.p, smask.data(), smask.size() * 4, hipMemcpyHostToDevice, st));
     if (l12) {
         ENSURE(c->R, (size_t)(q.ker_n_features / 2) * d * 4);
         HIPCHK(hipMemcpyAsync(c->R.p, Rh, (size_t)(q.ker_n_features / 2) * d * 4, hipMemcpyHostToDevice, st));
     }
 
-    // ---- pass 1: projections, top-K, chunk extremes
-    int G = 1024;
-    if (n < 4LL * G) G = (int)std::max<long long>(1, (n + 3) / 4);
+    // ---- sample pass: top-M thresholds from every stride-th point
+    const long long stride = n >= 65536 ? 8 : 1;
+    const long long npts = (n + stride - 1) / stride;
+    const int Gs = (int)std::max<long long>(1, std::min<long long>(1024, (npts + 3) / 4));
+    const int Gf = (int)std::max<long long>(1, std::min<long long>(2048, (n + 3) / 4));
     const int tie_cap = (int)std::min<long long>(n, 65536);
     ENSURE(c->topK, (size_t)n * 2 * K * 4);
     ENSURE(c->tie_idx, (size_t)tie_cap * 4);
     ENSURE(c->tie_vals, (size_t)tie_cap * P * 4);
     ENSURE(c->tie_count, 4);
-    ENSURE(c->cmax, (size_t)G * P * 4);
-    ENSURE(c->cmin, (size_t)G * P * 4);
+    ENSURE(c->cmax, (size_t)Gs * P * 4);
+    ENSURE(c->cmin, (size_t)Gs * P * 4);
     ENSURE(c->thi, (size_t)P * 4);
     ENSURE(c->tlo, (size_t)P * 4);
+    ENSURE(c->thr_any, 8);
     const int cap = 4096;
     ENSURE(c->cand_cnt, (size_t)2 * P * 4);
     ENSURE(c->cand_key, (size_t)2 * P * cap * 4);
@@ -343,11 +359,17 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     HIPCHK(hipMemsetAsync(c->cand_cnt.p, 0, (size_t)2 * P * 4, st));
 
     IndexArgs ia{};
+    ia.X = c->X;
     ia.Xn = P_<float>(c->Xn);
+    ia.Xn_out = P_<float>(c->Xn);
+    ia.Xb_out = q.precision == SDB_BF16 ? P_<uint16_t>(c->Xb) : nullptr;
+    ia.err = P_<int>(c->err_flag);
+    ia.signmask = P_<uint32_t>(c->bits);
+    ia.sample_stride = stride;
+    ia.thr_any = P_<uint32_t>(c->thr_any);
     ia.n = n; ia.ldx = ldx; ia.d = d; ia.metric = q.metric; ia.ker = q.ker_n_features;
     ia.kerL = q.ker_intervalSampling;
     ia.R = l12 ? P_<float>(c->R) : nullptr;
-    ia.bits = P_<uint8_t>(c->bits);
     ia.P = P; ia.K = K;
     ia.topK = P_<int>(c->topK);
     ia.tie_count = P_<int>(c->tie_count); ia.tie_cap = tie_cap;
@@ -357,15 +379,19 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ia.cand_cnt = P_<int>(c->cand_cnt); ia.cand_cap = cap;
     ia.cand_key = P_<uint32_t>(c->cand_key); ia.cand_idx = P_<int>(c->cand_idx);
     const int logD = ilog2(D);
-    LCHK(launch_index(ia, logD, 0, G, st));
+    LCHK(launch_index(ia, logD, 0, Gs, st));
+    LCHK(launch_topm_threshold(ia.chunk_max, ia.chunk_min, Gs, P, M, P_<uint32_t>(c->thi), P_<uint32_t>(c->tlo), st));
+    LCHK(launch_thr_any(P_<uint32_t>(c->thi), P_<uint32_t>(c->tlo), P, P_<uint32_t>(c->thr_any), st));
+    // ---- full pass: (transform,) projections, top-K, top-M candidates
+    LCHK(launch_index(ia, logD, 1, Gf, st));
     int ntie = 0;
     HIPCHK(hipMemcpyAsync(&ntie, c->tie_count.p, 4, hipMemcpyDeviceToHost, st));
-    LCHK(launch_topm_threshold(ia.chunk_max, ia.chunk_min, G, P, M, P_<uint32_t>(c->thi), P_<uint32_t>(c->tlo), st));
-    LCHK(launch_index(ia, logD, 1, G, st));
+    HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
     LCHK(launch_topm_select(ia.cand_key, ia.cand_idx, ia.cand_cnt, cap, P, M, P_<int>(c->topM), P_<int>(c->flags), st));
     std::vector<int> hflags(2 * P);
     HIPCHK(hipMemcpyAsync(hflags.data(), c->flags.p, (size_t)2 * P * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (errf & 2) return fail(c, SDB_EZERO, "Error: There is an zero point !");
 
     // ---- host replay of tied top-K points (reference heap semantics)
     if (ntie > tie_cap) return fail(c, SDB_EUNSUPPORTED, "%d points with tied top-K (capacity %d)", ntie, tie_cap);
@@ -401,7 +427,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
             HIPCHK(hipMemcpyAsync(c->row_slot.p, slot.data(), (size_t)P * 4, hipMemcpyHostToDevice, st));
             ia.row_slot = P_<int>(c->row_slot);
             ia.row_buf = P_<float>(c->row_buf);
-            LCHK(launch_index(ia, logD, 2, G, st));
+            LCHK(launch_index(ia, logD, 2, Gf, st));
             HIPCHK(hipMemcpyAsync(buf.data(), c->row_buf.p, (b1 - b0) * n * 4, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             for (size_t i = b0; i < b1; ++i) {

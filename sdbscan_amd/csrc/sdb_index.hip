@@ -125,85 +125,69 @@ __device__ __forceinline__ void ffht_wave(float (&v)[(1 << LOGD) / 64], int lane
     for (int js = 1; js < VPL; js <<= 1) fwht_inreg<VPL>(v, js);
 }
 
-// Embedding of point p into the wave registers (element e = lane + 64 j).
-// Cosine: identity; L1/L2: [cos(Rx), sin(Rx)] with Rx a sequential fmaf chain
-// over the input dims; Chi2/JS: the deterministic sampled features.
-template <int VPL>
-__device__ __forceinline__ void load_embedding(const IndexArgs& a, long long p, int lane,
-                                               float (&v)[VPL]) {
+// Embedding of a transformed point for the metrics other than cosine, staged
+// through the wave's LDS slice (element e at emb[e]); loops are not unrolled so
+// the transcendental code is emitted once.  L1/L2: [cos(Rx), sin(Rx)] with Rx
+// a sequential fmaf chain over the input dims (src/sDbscan.cpp:59-64);
+// Chi2/JS: the deterministic sampled features (src/Utilities.h:132-208).
+template <int D>
+__device__ __noinline__ void embed_lds(const IndexArgs& a, long long p, int lane, float* emb) {
     const float* x = a.Xn + p * a.ldx;
-#pragma unroll
-    for (int j = 0; j < VPL; ++j) v[j] = 0.0f;
-    if (a.metric == kCosine) {
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) {
-            int e = lane + 64 * j;
-            if (e < a.d) v[j] = x[e];
-        }
-    } else if (a.metric == kL1 || a.metric == kL2) {
-        const int half = a.ker / 2;
-        // element e < half holds cos(R_e x), half <= e < 2 half holds
-        // sin(R_{e-half} x)
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) {
-            int e = lane + 64 * j;
+    const int d = a.d;
+    const float L = a.kerL;
+    for (int e = lane; e < D; e += 64) {
+        float v = 0.0f;
+        if (a.metric == kL1 || a.metric == kL2) {
+            const int half = a.ker / 2;
             if (e < 2 * half) {
-                int r = e < half ? e : e - half;
+                const int r = e < half ? e : e - half;
                 float acc = 0.0f;
-                for (int c = 0; c < a.d; ++c) acc = fmaf(a.R[(size_t)c * half + r], x[c], acc);
-                v[j] = e < half ? cosf(acc) : sinf(acc);
+                for (int c = 0; c < d; ++c) acc = fmaf(a.R[(size_t)c * half + r], x[c], acc);
+                v = e < half ? cosf(acc) : sinf(acc);
             }
-        }
-    } else {
-        const int d = a.d;
-        const int comps = ((a.ker / d) - 1) / 2;
-        const float L = a.kerL;
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) {
-            int e = lane + 64 * j;
-            if (e >= a.ker) continue;
+        } else if (e < a.ker) {
+            const int comps = ((a.ker / d) - 1) / 2;
             if (e < d) {
-                float xv = x[e];
+                const float xv = x[e];
                 if (xv > 0.0f) {
-                    if (a.metric == kChi2) v[j] = sqrtf(xv * L);
-                    else v[j] = (float)sqrt((double)(xv * L) * 2.0 / log(4.0));
+                    if (a.metric == kChi2) v = sqrtf(xv * L);
+                    else v = (float)sqrt((double)(xv * L) * 2.0 / log(4.0));
                 }
             } else {
-                int t = e - d;
-                int i = t / (2 * d) + 1;
-                if (i > comps) continue;
-                int rem = t % (2 * d);
-                int jj = rem < d ? rem : rem - d;
-                float xv = x[jj];
-                if (xv > 0.0f) {
+                const int t = e - d;
+                const int i = t / (2 * d) + 1;
+                const int rem = t % (2 * d);
+                const float xv = x[rem < d ? rem : rem - d];
+                if (i <= comps && xv > 0.0f) {
                     float f, ph;
                     if (a.metric == kChi2) {
                         f = (float)sqrt(2.0 * xv * L / cosh(M_PI * i * L));
                         ph = (float)i * L * logf(xv);
                     } else {
                         const float iL = (float)i * L;
-                        float kap = (float)(2.0 / (log(4.0) * (double)(1.0f + 4.0f * iL * iL) * cosh(M_PI * i * L)));
+                        const float kap = (float)(2.0 / (log(4.0) * (double)(1.0f + 4.0f * iL * iL) * cosh(M_PI * i * L)));
                         f = (float)sqrt(2.0 * xv * L * kap);
                         ph = iL * logf(xv);
                     }
-                    v[j] = rem < d ? f * cosf(ph) : f * sinf(ph);
+                    v = rem < d ? f * cosf(ph) : f * sinf(ph);
                 }
             }
         }
+        emb[e] = v;
     }
 }
 
-template <int VPL>
-__device__ __forceinline__ void hd3_project(const IndexArgs& a, long long p, int lane,
-                                            float (&v)[VPL], const uint32_t (&sg)[3][(VPL + 31) / 32]) {
-    load_embedding<VPL>(a, p, lane, v);
-    constexpr int LOGD = (VPL == 1 ? 6 : VPL == 2 ? 7 : VPL == 4 ? 8 : VPL == 8 ? 9 : VPL == 16 ? 10 : VPL == 32 ? 11 : VPL == 64 ? 12 : 13);
+// HD3 rotation in registers (src/sDbscan.cpp:77-88): 3 x {sign flip, FWHT};
+// the (2 bit - 1) multiplier negates where the bit is 0.
+template <int LOGD>
+__device__ __forceinline__ void hd3_rotate(float (&v)[(1 << LOGD) / 64], int lane,
+                                           const uint32_t (&sg)[3][((1 << LOGD) / 64 + 31) / 32]) {
+    constexpr int VPL = (1 << LOGD) / 64;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
         for (int j = 0; j < VPL; ++j) {
-            // (2 * bit - 1) multiplier of src/sDbscan.cpp:80-83: bit 0 negates
-            uint32_t bit = (sg[r][j >> 5] >> (j & 31)) & 1u;
+            const uint32_t bit = (sg[r][j >> 5] >> (j & 31)) & 1u;
             v[j] = __uint_as_float(__float_as_uint(v[j]) ^ ((bit ^ 1u) << 31));
         }
         ffht_wave<LOGD>(v, lane);
@@ -260,12 +244,15 @@ __device__ __forceinline__ int lanes_below(unsigned long long b) {
 // one 64-lane bitonic network; more candidates take the general path.
 // Returns true when the (K+1)-th largest value equals the K-th.
 template <int VPL>
-__device__ bool wave_topk_general(const uint32_t (&key)[VPL], int P, int K, int lane, uint64_t* lds,
-                                  int* out);
+__device__ __forceinline__ bool wave_topk_general(const float (&v)[VPL], bool neg, int P, int K, int lane,
+                                               int* out);
 
 template <int VPL>
-__device__ bool wave_topk(const uint32_t (&key)[VPL], int P, int K, int lane, uint64_t* lds,
-                          int* out) {
+__device__ __forceinline__ bool wave_topk(const float (&v)[VPL], bool neg, int P, int K, int lane,
+                                          uint64_t* lds, int* out) {
+    uint32_t key[VPL];
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) key[j] = ord_key(neg ? -v[j] : v[j]);
     uint32_t lm = 0;
 #pragma unroll
     for (int j = 0; j < VPL; ++j)
@@ -284,7 +271,7 @@ __device__ bool wave_topk(const uint32_t (&key)[VPL], int P, int K, int lane, ui
         total += __popcll(b);
     }
     __builtin_amdgcn_wave_barrier();
-    if (total > 64) return wave_topk_general<VPL>(key, P, K, lane, lds, out);
+    if (total > 64) return wave_topk_general<VPL>(v, neg, P, K, lane, out);
     uint32_t h = lane < total ? sh[lane] : 0u;
     uint32_t lo = lane < total ? sh[64 + lane] : 0u;
     __builtin_amdgcn_wave_barrier();
@@ -303,83 +290,71 @@ __device__ bool wave_topk(const uint32_t (&key)[VPL], int P, int K, int lane, ui
 // out[0..K); returns true when the (K+1)-th largest value equals the K-th
 // (a tie across the selection boundary).
 template <int VPL>
-__device__ bool wave_topk_general(const uint32_t (&key)[VPL], int P, int K, int lane, uint64_t* lds,
-                                  int* out) {
-    uint64_t lm = 0;
+__device__ __forceinline__ bool wave_topk_general(const float (&v)[VPL], bool neg, int P, int K, int lane,
+                                               int* out) {
+    // K + 1 rounds of a wave-wide argmax over the register keys (value desc,
+    // index asc); the winner of a round is masked out in its lane.
+    uint32_t key[VPL];
 #pragma unroll
-    for (int j = 0; j < VPL; ++j) {
-        int e = lane + 64 * j;
-        if (e < P) {
-            uint64_t k64 = ((uint64_t)key[j] << 32) | (uint32_t)(0xffffffffu - (uint32_t)e);
-            lm = k64 > lm ? k64 : lm;
-        }
-    }
-    uint64_t cur = lm, t0 = 0;
-    const int rounds = K < 64 ? K : 64;
-    for (int k = 0; k < rounds; ++k) {
-        uint64_t b = wave_max_u64(cur);
-        t0 = b;
-        if (cur == b) cur = 0;
-    }
-    if (K > 64) t0 = 0;  // more winners than lanes: rank every element
-    const uint32_t tv = (uint32_t)(t0 >> 32);
-    // compact candidates (value >= value(T0)) into lds
-    int cnt = 0;
-#pragma unroll
-    for (int j = 0; j < VPL; ++j) {
-        int e = lane + 64 * j;
-        cnt += (e < P && key[j] >= tv) ? 1 : 0;
-    }
-    int incl = cnt;
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        int o = __shfl_up(incl, m, kWave);
-        if (lane >= m) incl += o;
-    }
-    const int total = __shfl(incl, 63, kWave);
-    int pos = incl - cnt;
-#pragma unroll
-    for (int j = 0; j < VPL; ++j) {
-        int e = lane + 64 * j;
-        if (e < P && key[j] >= tv)
-            lds[pos++] = ((uint64_t)key[j] << 32) | (uint32_t)(0xffffffffu - (uint32_t)e);
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (int j = 0; j < VPL; ++j) key[j] = ord_key(neg ? -v[j] : v[j]);
+    uint64_t removed = 0;
     uint64_t prev = 0;
     bool tie = false;
-    for (int k = 0; k <= K && k < total; ++k) {
+    for (int k = 0; k <= K; ++k) {
         uint64_t best = 0;
-        int bi = -1;
-        for (int i = lane; i < total; i += 64) {
-            uint64_t c = lds[i];
-            if (c > best) { best = c; bi = i; }
+        int jb = -1;
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            const int e = lane + 64 * j;
+            if (e < P && !((removed >> j) & 1ull)) {
+                const uint64_t c = ((uint64_t)key[j] << 32) | (uint32_t)(0xffffffffu - (uint32_t)e);
+                if (c > best) { best = c; jb = j; }
+            }
         }
-        uint64_t w = wave_max_u64(best);
+        const uint64_t w = wave_max_u64(best);
+        if (w == 0) break;
         if (k < K) {
-            if (best == w && bi >= 0) lds[bi] = 0;
+            if (best == w && jb >= 0) removed |= 1ull << jb;
             if (lane == 0) out[k] = (int)(0xffffffffu - (uint32_t)w);
             prev = w;
         } else {
             tie = (uint32_t)(w >> 32) == (uint32_t)(prev >> 32);
         }
-        __builtin_amdgcn_wave_barrier();
     }
     return tie;
 }
 
 // ---------------------------------------------------------------------------
-// Pass 1 of the index (MODE 0): embedding + HD3 + per-point top-K
-// (src/sDbscan.cpp:97-143), per-chunk row maxima / minima of the projections
-// for the top-M threshold, and capture of points whose top-K has a tie at the
-// boundary (replayed on the host with the reference heap).
-// Pass 2 (MODE 1): recompute the projections and append every value beyond
-// the per-row thresholds to the top-M candidate lists.
-// MODE 2: recompute and store whole projection rows listed in row_slot.
-//
-// Work split: grid of G workgroups x 4 waves; wave w of group g handles the
-// points p = (4 g + w) + t * 4 G, so chunk g is an interleaved subset.
-template <int LOGD, int MODE>
+// Index passes over the points, one wavefront per point:
+//   kSample: every sample_stride-th point; HD3 projections and per-chunk row
+//            maxima / minima for the top-M thresholds.
+//   kFull:   every point; (cosine: transformData fused in) HD3 projections,
+//            per-point top-K (src/sDbscan.cpp:97-143) with capture of points
+//            whose top-K is tied at the boundary, and the values beyond the
+//            per-row thresholds appended to the top-M candidate lists.
+//   kRows:   recompute and store whole projection rows listed in row_slot
+//            (host replay of tied top-M rows).
+// Grid: G workgroups x 4 waves; wave w of group g takes the points
+// q = (4 g + w) + t * 4 G, so chunk g is an interleaved subset.
+enum { kSample = 0, kFull = 1, kRows = 2 };
+
+__device__ __forceinline__ double wave_sum_f64(double v, int lane) {
+    uint32_t lo = (uint32_t)__double_as_longlong(v);
+    uint32_t hi = (uint32_t)((unsigned long long)__double_as_longlong(v) >> 32);
+#define SDB_SUM_STEP(S)                                                                   \
+    {                                                                                     \
+        const uint32_t plo = xchg<S>(lo, lane), phi = xchg<S>(hi, lane);                  \
+        const double o = __longlong_as_double((long long)(((unsigned long long)phi << 32) | plo)); \
+        v = v + o;                                                                        \
+        lo = (uint32_t)__double_as_longlong(v);                                           \
+        hi = (uint32_t)((unsigned long long)__double_as_longlong(v) >> 32);               \
+    }
+    SDB_SUM_STEP(32) SDB_SUM_STEP(16) SDB_SUM_STEP(8) SDB_SUM_STEP(4) SDB_SUM_STEP(2) SDB_SUM_STEP(1)
+#undef SDB_SUM_STEP
+    return v;
+}
+
+template <int LOGD, int MODE, bool COS>
 __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
     constexpr int D = 1 << LOGD;
     constexpr int VPL = D / 64;
@@ -388,114 +363,180 @@ __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
     const int wib = threadIdx.x / kWave;
     const long long gw = (long long)blockIdx.x * 4 + wib;
     const long long nw = (long long)gridDim.x * 4;
+    const int P = a.P;
 
     extern __shared__ __align__(16) unsigned char smem[];
+    // LDS: [top-K scratch 4 x 512 B (kFull)] [2P keys] [embedding 4 x D x 4 B (!COS)]
+    const size_t off_red = MODE == kFull ? (size_t)4 * 512 : 0;
+    const size_t off_emb = off_red + (size_t)2 * P * 4;
+    uint64_t* tks = (uint64_t*)smem + (size_t)wib * 64;
+    uint32_t* red = (uint32_t*)(smem + off_red);
+    float* emb = (float*)(smem + off_emb) + (size_t)wib * D;
 
     uint32_t sg[3][NW];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            uint32_t m = 0;
-            for (int b = 0; b < 32 && w * 32 + b < VPL; ++b)
-                m |= (uint32_t)a.bits[r * D + lane + 64 * (w * 32 + b)] << b;
-            sg[r][w] = m;
-        }
+        for (int w = 0; w < NW; ++w) sg[r][w] = a.signmask[(r * NW + w) * 64 + lane];
 
-    float v[VPL];
-    if (MODE == 0) {
-        uint64_t* lds = (uint64_t*)smem + (size_t)wib * D;        // top-K candidates
-        uint32_t* red = (uint32_t*)(smem + (size_t)4 * D * 8);    // [2][P] chunk max/min keys
-        for (int i = threadIdx.x; i < a.P; i += blockDim.x) { red[i] = 0u; red[a.P + i] = 0xffffffffu; }
-        __syncthreads();
-        uint32_t cmax[VPL], cmin[VPL];
+    if (MODE == kSample) {
+        for (int i = threadIdx.x; i < P; i += blockDim.x) { red[i] = 0u; red[P + i] = 0xffffffffu; }
+    } else if (MODE == kFull) {
+        for (int i = threadIdx.x; i < P; i += blockDim.x) { red[i] = a.thr_hi[i]; red[P + i] = a.thr_lo[i]; }
+    }
+    __syncthreads();
+    const uint32_t any_hi = MODE == kFull ? a.thr_any[0] : 0u;
+    const uint32_t any_lo = MODE == kFull ? a.thr_any[1] : 0u;
+
+    uint32_t cmax[MODE == kSample ? VPL : 1], cmin[MODE == kSample ? VPL : 1];
+    if constexpr (MODE == kSample) {
 #pragma unroll
         for (int j = 0; j < VPL; ++j) { cmax[j] = 0u; cmin[j] = 0xffffffffu; }
-        for (long long p = gw; p < a.n; p += nw) {
-            hd3_project<VPL>(a, p, lane, v, sg);
-            uint32_t kc[VPL], kf[VPL];
+    }
+    const long long stride = MODE == kSample ? a.sample_stride : 1;
+    const long long npts = (a.n + stride - 1) / stride;
+    // raw rows of the next point are loaded one iteration ahead (cosine)
+    float xpre[(COS && MODE != kRows) ? VPL : 1];
+    if constexpr (COS && MODE != kRows) {
+        if (gw < npts) {
+            const float* x = a.X + gw * stride * a.d;
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) { const int e = lane + 64 * j; xpre[j] = e < a.d ? x[e] : 0.0f; }
+        }
+    }
+    for (long long q = gw; q < npts; q += nw) {
+        const long long p = q * stride;
+        float v[VPL];
+        if constexpr (COS) {
+            if (MODE == kRows) {
+                const float* x = a.Xn + p * a.ldx;
+#pragma unroll
+                for (int j = 0; j < VPL; ++j) { const int e = lane + 64 * j; v[j] = e < a.d ? x[e] : 0.0f; }
+            } else {
+                // transformData for Cosine (src/Utilities.h:295-299): x / ||x||,
+                // ||x||^2 summed in FP64 per lane (e = lane + 64 j) then by the
+                // xor butterfly -- the order of k_transform.
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < VPL; ++j) {
+                    v[j] = xpre[j];
+                    const double t = v[j];
+                    acc = acc + t * t;
+                }
+                if (q + nw < npts) {
+                    const float* xn = a.X + (q + nw) * stride * a.d;
+#pragma unroll
+                    for (int j = 0; j < VPL; ++j) { const int e = lane + 64 * j; xpre[j] = e < a.d ? xn[e] : 0.0f; }
+                }
+                const float nrm = (float)sqrt(wave_sum_f64(acc, lane));
+                if (!(nrm > 0.0f)) {
+                    if (lane == 0) atomicOr(a.err, 2);
+                }
+#pragma unroll
+                for (int j = 0; j < VPL; ++j) v[j] = v[j] / nrm;
+                if (MODE == kFull) {
+                    float* o = a.Xn_out + p * a.ldx;
+                    uint16_t* ob = a.Xb_out ? a.Xb_out + p * a.ldx : nullptr;
+#pragma unroll
+                    for (int j = 0; j < VPL; ++j) {
+                        const int e = lane + 64 * j;
+                        if (e < a.ldx) {
+                            const float w = e < a.d ? v[j] : 0.0f;
+                            o[e] = w;
+                            if (ob) ob[e] = (uint16_t)f32_to_bf16_rne(w);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < VPL; ++j) if (lane + 64 * j >= a.d) v[j] = 0.0f;
+            }
+        } else {
+            embed_lds<D>(a, p, lane, emb);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) v[j] = emb[lane + 64 * j];
+            __builtin_amdgcn_wave_barrier();
+        }
+        hd3_rotate<LOGD>(v, lane, sg);
+
+        if constexpr (MODE == kSample) {
 #pragma unroll
             for (int j = 0; j < VPL; ++j) {
-                kc[j] = ord_key(v[j]);
-                kf[j] = ord_key(-v[j]);
-                int e = lane + 64 * j;
-                if (e < a.P) {
-                    cmax[j] = kc[j] > cmax[j] ? kc[j] : cmax[j];
-                    cmin[j] = kc[j] < cmin[j] ? kc[j] : cmin[j];
+                if (lane + 64 * j < P) {
+                    const uint32_t k = ord_key(v[j]);
+                    cmax[j] = k > cmax[j] ? k : cmax[j];
+                    cmin[j] = k < cmin[j] ? k : cmin[j];
                 }
             }
+        } else if constexpr (MODE == kFull) {
             int* tk = a.topK + p * 2 * a.K;
-            bool t1 = wave_topk<VPL>(kc, a.P, a.K, lane, lds, tk);
-            bool t2 = wave_topk<VPL>(kf, a.P, a.K, lane, lds, tk + a.K);
+            const bool t1 = wave_topk<VPL>(v, false, P, a.K, lane, tks, tk);
+            const bool t2 = wave_topk<VPL>(v, true, P, a.K, lane, tks, tk + a.K);
             if (t1 || t2) {
                 int slot = 0;
                 if (lane == 0) slot = atomicAdd(a.tie_count, 1);
                 slot = __shfl(slot, 0, kWave);
                 if (slot < a.tie_cap) {
                     if (lane == 0) a.tie_idx[slot] = (int)p;
-                    float* tv = a.tie_vals + (size_t)slot * a.P;
+                    float* tv = a.tie_vals + (size_t)slot * P;
 #pragma unroll
                     for (int j = 0; j < VPL; ++j) {
-                        int e = lane + 64 * j;
-                        if (e < a.P) tv[e] = v[j];
+                        const int e = lane + 64 * j;
+                        if (e < P) tv[e] = v[j];
                     }
                 }
             }
-        }
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) {
-            int e = lane + 64 * j;
-            if (e < a.P) { atomicMax(&red[e], cmax[j]); atomicMin(&red[a.P + e], cmin[j]); }
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < a.P; i += blockDim.x) {
-            a.chunk_max[(size_t)blockIdx.x * a.P + i] = red[i];
-            a.chunk_min[(size_t)blockIdx.x * a.P + i] = red[a.P + i];
-        }
-    } else if (MODE == 1) {
-        uint32_t thi[VPL], tlo[VPL];
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) {
-            int e = lane + 64 * j;
-            thi[j] = e < a.P ? a.thr_hi[e] : 0xffffffffu;
-            tlo[j] = e < a.P ? a.thr_lo[e] : 0u;
-        }
-        for (long long p = gw; p < a.n; p += nw) {
-            hd3_project<VPL>(a, p, lane, v, sg);
+            // values beyond every row's threshold are rare: one wave-wide test
+            bool anyc = false;
 #pragma unroll
             for (int j = 0; j < VPL; ++j) {
-                int e = lane + 64 * j;
-                if (e >= a.P) continue;
-                uint32_t k = ord_key(v[j]);
-                if (k >= thi[j]) {
-                    int pos = atomicAdd(&a.cand_cnt[2 * e], 1);
+                const uint32_t k = ord_key(v[j]);
+                anyc |= (lane + 64 * j < P) && (k >= any_hi || k <= any_lo);
+            }
+            if (__any(anyc))
+#pragma unroll 1
+            for (int j = 0; j < VPL; ++j) {
+                const int e = lane + 64 * j;
+                if (e >= P) continue;
+                const uint32_t k = ord_key(v[j]);
+                if (k >= any_hi && k >= red[e]) {
+                    const int pos = atomicAdd(&a.cand_cnt[2 * e], 1);
                     if (pos < a.cand_cap) {
-                        size_t o = ((size_t)(2 * e) * a.cand_cap + pos);
+                        const size_t o = (size_t)(2 * e) * a.cand_cap + pos;
                         a.cand_key[o] = k;
                         a.cand_idx[o] = (int)p;
                     }
                 }
-                if (k <= tlo[j]) {
-                    int pos = atomicAdd(&a.cand_cnt[2 * e + 1], 1);
+                if (k <= any_lo && k <= red[P + e]) {
+                    const int pos = atomicAdd(&a.cand_cnt[2 * e + 1], 1);
                     if (pos < a.cand_cap) {
-                        size_t o = ((size_t)(2 * e + 1) * a.cand_cap + pos);
+                        const size_t o = (size_t)(2 * e + 1) * a.cand_cap + pos;
                         a.cand_key[o] = k;
                         a.cand_idx[o] = (int)p;
                     }
                 }
             }
-        }
-    } else {
-        for (long long p = gw; p < a.n; p += nw) {
-            hd3_project<VPL>(a, p, lane, v, sg);
+        } else {
 #pragma unroll
             for (int j = 0; j < VPL; ++j) {
-                int e = lane + 64 * j;
-                if (e < a.P) {
-                    int s = a.row_slot[e];
+                const int e = lane + 64 * j;
+                if (e < P) {
+                    const int s = a.row_slot[e];
                     if (s >= 0) a.row_buf[(size_t)s * a.n + p] = v[j];
                 }
             }
+        }
+    }
+    if constexpr (MODE == kSample) {
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            const int e = lane + 64 * j;
+            if (e < P) { atomicMax(&red[e], cmax[j]); atomicMin(&red[P + e], cmin[j]); }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+            a.chunk_max[(size_t)blockIdx.x * P + i] = red[i];
+            a.chunk_min[(size_t)blockIdx.x * P + i] = red[P + i];
         }
     }
 }
@@ -536,6 +577,18 @@ __global__ __launch_bounds__(256) void k_topm_threshold(const uint32_t* __restri
     }
     __syncthreads();
     if (threadIdx.x == 0) { thi[r] = res_hi; tlo[r] = res_lo; }
+}
+
+// thr_any[0] = min over rows of thr_hi, thr_any[1] = max over rows of thr_lo:
+// a value below / above both cannot be a candidate of any row.
+__global__ __launch_bounds__(256) void k_thr_any(const uint32_t* thi, const uint32_t* tlo, int P,
+                                                 uint32_t* thr_any) {
+    __shared__ uint32_t mn, mx;
+    if (threadIdx.x == 0) { mn = 0xffffffffu; mx = 0u; }
+    __syncthreads();
+    for (int i = threadIdx.x; i < P; i += blockDim.x) { atomicMin(&mn, thi[i]); atomicMax(&mx, tlo[i]); }
+    __syncthreads();
+    if (threadIdx.x == 0) { thr_any[0] = mn; thr_any[1] = mx; }
 }
 
 // Exact top-M per row and side from the candidate lists: bitonic sort of the
@@ -772,37 +825,43 @@ int launch_transform(const float* X, long long n, int d, int ldx, int metric, fl
     return (int)hipGetLastError();
 }
 
-template <int LOGD, int MODE>
+template <int LOGD, int MODE, bool COS>
 static int launch_index_t(const IndexArgs& a, int grid, hipStream_t st) {
     constexpr int D = 1 << LOGD;
-    size_t sm = 0;
-    if (MODE == 0) sm = (size_t)4 * D * 8 + (size_t)2 * a.P * 4;
-    if (MODE == 0 && sm > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE>,
+    size_t sm = (MODE == kFull ? (size_t)4 * 512 : 0) + (size_t)2 * a.P * 4 + (COS ? 0 : (size_t)4 * D * 4);
+    if (sm > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE, COS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_index<LOGD, MODE>), dim3(grid), dim3(256), sm, st, a);
+    hipLaunchKernelGGL((k_index<LOGD, MODE, COS>), dim3(grid), dim3(256), sm, st, a);
     return (int)hipGetLastError();
 }
 
+template <int LOGD>
+static int launch_index_d(const IndexArgs& a, int mode, int grid, hipStream_t st) {
+    const bool cos = a.metric == kCosine;
+    if (mode == kSample) return cos ? launch_index_t<LOGD, kSample, true>(a, grid, st) : launch_index_t<LOGD, kSample, false>(a, grid, st);
+    if (mode == kFull) return cos ? launch_index_t<LOGD, kFull, true>(a, grid, st) : launch_index_t<LOGD, kFull, false>(a, grid, st);
+    return cos ? launch_index_t<LOGD, kRows, true>(a, grid, st) : launch_index_t<LOGD, kRows, false>(a, grid, st);
+}
+
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st) {
-#define SDB_CASE(LD)                                                  \
-    case LD:                                                          \
-        if (mode == 0) return launch_index_t<LD, 0>(a, grid, st);     \
-        if (mode == 1) return launch_index_t<LD, 1>(a, grid, st);     \
-        return launch_index_t<LD, 2>(a, grid, st);
     switch (logD) {
-        SDB_CASE(6)
-        SDB_CASE(7)
-        SDB_CASE(8)
-        SDB_CASE(9)
-        SDB_CASE(10)
-        SDB_CASE(11)
-        SDB_CASE(12)
-        default: return -1;
+        case 6: return launch_index_d<6>(a, mode, grid, st);
+        case 7: return launch_index_d<7>(a, mode, grid, st);
+        case 8: return launch_index_d<8>(a, mode, grid, st);
+        case 9: return launch_index_d<9>(a, mode, grid, st);
+        case 10: return launch_index_d<10>(a, mode, grid, st);
+        case 11: return launch_index_d<11>(a, mode, grid, st);
+        case 12: return launch_index_d<12>(a, mode, grid, st);
+        default: return -3;
     }
-#undef SDB_CASE
+}
+
+int launch_thr_any(const uint32_t* thi, const uint32_t* tlo, int P, uint32_t* thr_any, hipStream_t st) {
+    hipLaunchKernelGGL(k_thr_any, dim3(1), dim3(256), 0, st, thi, tlo, P, thr_any);
+    return (int)hipGetLastError();
 }
 
 int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,

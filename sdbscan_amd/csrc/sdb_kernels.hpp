@@ -8,7 +8,14 @@ constexpr int kMaxW = 8;          // top-M lists up to 256 points (8 x 32-bit wo
 constexpr int kLdsBytes = 163840;  // LDS per workgroup on gfx950
 
 struct IndexArgs {
-    const float* Xn;  // n x ldx transformed rows
+    const float* X;   // n x d input rows (cosine passes read and transform them)
+    const float* Xn;  // n x ldx transformed rows (other metrics; row replay)
+    float* Xn_out;    // cosine full pass writes the transformed rows here
+    uint16_t* Xb_out; // and their BF16 copy (optional)
+    int* err;         // bit 2: zero point
+    const uint32_t* signmask;  // [3][NW][64] per-lane sign bits
+    long long sample_stride;
+    const uint32_t* thr_any;   // [0] min of thr_hi, [1] max of thr_lo
     long long n;
     int ldx, d, metric, ker;
     float kerL;        // ker_intervalSampling
@@ -59,6 +66,7 @@ struct DistArgs {
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
                      uint16_t* Xb, int* err, hipStream_t st);
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st);
+int launch_thr_any(const uint32_t* thi, const uint32_t* tlo, int P, uint32_t* thr_any, hipStream_t st);
 int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,
                           uint32_t* thi, uint32_t* tlo, hipStream_t st);
 int launch_topm_select(const uint32_t* key, const int* idx, const int* cnt, int cap, int P, int M,
