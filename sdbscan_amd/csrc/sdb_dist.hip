@@ -27,7 +27,7 @@ __device__ __forceinline__ unsigned long long queue_entry(uint32_t ent, int l, i
 // 16 bytes of query row (l & 15) and of candidate column (l & 15) at k-group
 // (l >> 4) per step; A and B use the same k assignment, so the sum over k is
 // the dot product whatever order the unit uses internally.
-template <bool BF, int QT, int NT, int WAVES>
+template <bool BF, int QT, int NT, int WAVES, int PF>
 __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     constexpr int SZ = BF ? 2 : 4;
     constexpr int SE = 64 / SZ;  // elements per 64-byte k-step
@@ -86,35 +86,44 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
             const char* arow[QT];
 #pragma unroll
             for (int q = 0; q < QT; ++q) arow[q] = Xc + ((size_t)pa[q] * a.ldx + k0) * SZ + g * 16;
-            uint4 af[QT];
+            // query fragments are kept PF k-steps ahead in a register ring so
+            // that enough HBM reads are in flight per CU
+            uint4 af[PF][QT];
 #pragma unroll
-            for (int q = 0; q < QT; ++q) af[q] = *(const uint4*)(arow[q]);
-            for (int st = 0; st < nsteps; ++st) {
-                uint4 an[QT];
-                const int stn = st + 1 < nsteps ? st + 1 : st;
+            for (int i = 0; i < PF; ++i)
 #pragma unroll
-                for (int q = 0; q < QT; ++q) an[q] = *(const uint4*)(arow[q] + (size_t)stn * 64);
+                for (int q = 0; q < QT; ++q)
+                    if (i < nsteps) af[i][q] = *(const uint4*)(arow[q] + (size_t)i * 64);
+            for (int st0 = 0; st0 < nsteps; st0 += PF) {
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    int row = t * 16 + r16;
-                    row = row < M ? row : M - 1;
-                    uint4 bfr = *(const uint4*)(smem + row * RSB + (size_t)st * 64 + g * 16);
+                for (int i = 0; i < PF; ++i) {
+                    const int st = st0 + i;
+                    if (st >= nsteps) break;
 #pragma unroll
-                    for (int q = 0; q < QT; ++q) {
-                        if constexpr (BF) {
-                            bf16x8_t av = __builtin_bit_cast(bf16x8_t, af[q]);
-                            bf16x8_t bv = __builtin_bit_cast(bf16x8_t, bfr);
-                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[q][t], 0, 0, 0);
-                        } else {
-                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[q].x), __uint_as_float(bfr.x), acc[q][t], 0, 0, 0);
-                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[q].y), __uint_as_float(bfr.y), acc[q][t], 0, 0, 0);
-                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[q].z), __uint_as_float(bfr.z), acc[q][t], 0, 0, 0);
-                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[q].w), __uint_as_float(bfr.w), acc[q][t], 0, 0, 0);
+                    for (int t = 0; t < NT; ++t) {
+                        int row = t * 16 + r16;
+                        row = row < M ? row : M - 1;
+                        uint4 bfr = *(const uint4*)(smem + row * RSB + (size_t)st * 64 + g * 16);
+#pragma unroll
+                        for (int q = 0; q < QT; ++q) {
+                            if constexpr (BF) {
+                                bf16x8_t av = __builtin_bit_cast(bf16x8_t, af[i][q]);
+                                bf16x8_t bv = __builtin_bit_cast(bf16x8_t, bfr);
+                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[q][t], 0, 0, 0);
+                            } else {
+                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[i][q].x), __uint_as_float(bfr.x), acc[q][t], 0, 0, 0);
+                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[i][q].y), __uint_as_float(bfr.y), acc[q][t], 0, 0, 0);
+                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[i][q].z), __uint_as_float(bfr.z), acc[q][t], 0, 0, 0);
+                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[i][q].w), __uint_as_float(bfr.w), acc[q][t], 0, 0, 0);
+                            }
                         }
                     }
-                }
+                    const int ns = st + PF;
+                    if (ns < nsteps) {
 #pragma unroll
-                for (int q = 0; q < QT; ++q) af[q] = an[q];
+                        for (int q = 0; q < QT; ++q) af[i][q] = *(const uint4*)(arow[q] + (size_t)ns * 64);
+                    }
+                }
             }
         }
 
@@ -176,13 +185,13 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
 
 template <bool BF, int NT>
 static int launch_mfma_nt(const DistArgs& a, int ntiles, hipStream_t st) {
-    constexpr int QT = 2, WAVES = 8;
+    constexpr int QT = 2, WAVES = 8, PF = 4;
     const size_t sm = (size_t)a.M * ((size_t)a.kc * (BF ? 2 : 4) + 16) + (size_t)a.M * 4;
     if (sm > (size_t)kLdsBytes) return -2;
-    hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES, PF>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL((k_dist_mfma<BF, QT, NT, WAVES>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
+    hipLaunchKernelGGL((k_dist_mfma<BF, QT, NT, WAVES, PF>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
     return (int)hipGetLastError();
 }
 

@@ -725,48 +725,55 @@ __global__ void k_multi(const int* __restrict__ lists, const int* __restrict__ h
 // in an earlier slot of a (the reference's per-point dedup bitset,
 // src/sDbscan.cpp:260-282).  Only multi positions can repeat; each is checked
 // against the lists of the earlier slots through its hub memberships.
-__global__ void k_first_mask(const int* __restrict__ topK, long long n, int P, int K, int M, int W,
-                             const uint32_t* __restrict__ multi, const int* __restrict__ lists,
-                             const int* __restrict__ hub_id, const int* __restrict__ hub_ptr,
-                             const int* __restrict__ hub_mem, uint32_t* __restrict__ F) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_first_mask(const int* __restrict__ topK, long long n, int P, int K,
+                                                    int M, int W, const uint32_t* __restrict__ multi,
+                                                    const int* __restrict__ lists,
+                                                    const int* __restrict__ hub_id,
+                                                    const int* __restrict__ hub_ptr,
+                                                    const int* __restrict__ hub_mem,
+                                                    uint32_t* __restrict__ F) {
+    // one thread per point; its 2K list ids are staged in LDS
+    extern __shared__ int lid_s[];
     const int S = 2 * K;
-    if (t >= n * S) return;
-    long long a = t / S;
-    int s = (int)(t % S);
-    int l = slot_list(topK, a, s, K, P);
-    uint32_t m[kMaxW];
-    for (int w = 0; w < W; ++w) {
-        int nb = M - w * 32;
-        m[w] = nb >= 32 ? 0xffffffffu : ((1u << nb) - 1u);
-    }
-    if (s > 0) {
+    const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    int* lids = lid_s + threadIdx.x * S;
+    for (int s = 0; s < S; ++s) lids[s] = slot_list(topK, a, s, K, P);
+    const int hs = hub_id[a];
+    for (int s = 0; s < S; ++s) {
+        const int l = lids[s];
+        uint32_t m[kMaxW];
         for (int w = 0; w < W; ++w) {
-            uint32_t mw = multi[(size_t)l * W + w];
-            while (mw) {
-                const int b = __ffs(mw) - 1;
-                mw &= mw - 1;
-                const int i = w * 32 + b;
-                const int h = hub_id[lists[(size_t)l * M + i]];
-                bool dup = false;
-                for (int q = hub_ptr[h]; q < hub_ptr[h + 1] && !dup; ++q) {
-                    const int L = hub_mem[q] / M;
-                    if (L == l) continue;
-                    for (int s2 = 0; s2 < s; ++s2)
-                        if (slot_list(topK, a, s2, K, P) == L) { dup = true; break; }
+            const int nb = M - w * 32;
+            m[w] = nb >= 32 ? 0xffffffffu : ((1u << nb) - 1u);
+        }
+        if (s > 0) {
+            for (int w = 0; w < W; ++w) {
+                uint32_t mw = multi[(size_t)l * W + w];
+                while (mw) {
+                    const int b = __ffs(mw) - 1;
+                    mw &= mw - 1;
+                    const int h = hub_id[lists[(size_t)l * M + w * 32 + b]];
+                    bool dup = false;
+                    for (int q = hub_ptr[h]; q < hub_ptr[h + 1] && !dup; ++q) {
+                        const int L = hub_mem[q] / M;
+                        if (L == l) continue;
+                        for (int s2 = 0; s2 < s; ++s2)
+                            if (lids[s2] == L) { dup = true; break; }
+                    }
+                    if (dup) m[w] &= ~(1u << b);
                 }
-                if (dup) m[w] &= ~(1u << b);
             }
         }
-    }
-    int h = hub_id[a];
-    if (h >= 0) {
-        for (int q = hub_ptr[h]; q < hub_ptr[h + 1]; ++q) {
-            int li = hub_mem[q];
-            if (li / M == l) { int i = li % M; m[i >> 5] &= ~(1u << (i & 31)); }
+        if (hs >= 0) {
+            for (int q = hub_ptr[hs]; q < hub_ptr[hs + 1]; ++q) {
+                const int li = hub_mem[q];
+                if (li / M == l) { const int i = li % M; m[i >> 5] &= ~(1u << (i & 31)); }
+            }
         }
+        uint32_t* Fo = F + ((size_t)a * S + s) * W;
+        for (int w = 0; w < W; ++w) Fo[w] = m[w];
     }
-    for (int w = 0; w < W; ++w) F[t * W + w] = m[w];
 }
 
 // Grouping of the (point, slot) entries by list without contended global
@@ -918,8 +925,12 @@ int launch_multi(const int* lists, const int* hub_id, const int* hub_ptr, int L2
 int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const uint32_t* multi,
                       const int* lists, const int* hub_id, const int* hub_ptr, const int* hub_mem,
                       uint32_t* F, hipStream_t st) {
-    long long tot = n * 2 * K;
-    hipLaunchKernelGGL(k_first_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, topK, n, P, K, M, W, multi, lists, hub_id, hub_ptr, hub_mem, F);
+    const size_t sm = (size_t)256 * 2 * K * 4;
+    if (sm > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_first_mask, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(k_first_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), sm, st, topK, n, P, K, M, W, multi, lists, hub_id, hub_ptr, hub_mem, F);
     return (int)hipGetLastError();
 }
 
