@@ -81,7 +81,18 @@ def main():
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a 4 GiB device-to-device copy (profiler counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        src = torch.ones(1 << 30, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU exchange is designed, not built)", file=sys.stderr)

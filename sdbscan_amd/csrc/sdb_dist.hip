@@ -185,7 +185,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
 
 template <bool BF, int NT>
 static int launch_mfma_nt(const DistArgs& a, int ntiles, hipStream_t st) {
-    constexpr int QT = 2, WAVES = 8, PF = 4;
+    constexpr int QT = 2, WAVES = 8, PF = BF ? 4 : 1;
     const size_t sm = (size_t)a.M * ((size_t)a.kc * (BF ? 2 : 4) + 16) + (size_t)a.M * 4;
     if (sm > (size_t)kLdsBytes) return -2;
     hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES, PF>,
