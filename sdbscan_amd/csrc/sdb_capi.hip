@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <queue>
@@ -60,6 +61,8 @@ struct sdb_ctx {
         row_slot, row_buf;
     bool indexed = false;
     int H = 0, L2 = 0, W = 0, S = 0;
+    long long qblk = 0;
+    int nblk = 1;
     std::vector<int> h_grp_ptr;
     // find_core
     DevBuf E, colsum, queue, qcount, tiles, degree, core, stats64;
@@ -476,17 +479,27 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ENSURE(c->F, (size_t)n * S * W * 4);
     LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, W, P_<uint32_t>(c->multi), P_<int>(c->lists),
                            P_<int>(c->hub_id), P_<int>(c->hub_ptr), P_<int>(c->hub_mem), P_<uint32_t>(c->F), st));
-    // group the (point, slot) entries by list
-    const int Gg = (int)std::max<long long>(1, std::min<long long>(1024, (n * S + 4095) / 4096));
-    ENSURE(c->wgcnt, (size_t)L2 * Gg * 4);
-    ENSURE(c->wgoff, (size_t)L2 * Gg * 4);
-    ENSURE(c->grp_ptr, (size_t)(L2 + 1) * 4);
+    // group the (point, slot) entries by (query block, list); a query block's
+    // rows (BF16) fit in the 256 MB last-level cache, so the 2K visits of a
+    // query row after the first are cache hits
+    long long qblk_mb = 1ll << 20;
+    if (const char* e = std::getenv("SDB_QBLK_MB")) qblk_mb = std::max(1ll, std::atoll(e));
+    const long long qblk = std::max<long long>(1, (qblk_mb << 20) / ((long long)ldx * 2));
+    const int nblk = (int)((n + qblk - 1) / qblk);
+    const int Gx = (int)std::max<long long>(1, std::min<long long>(64, 2048 / nblk));
+    const int NG = nblk * L2;
+    c->qblk = qblk;
+    c->nblk = nblk;
+    ENSURE(c->wgcnt, (size_t)NG * Gx * 4);
+    ENSURE(c->wgoff, (size_t)NG * Gx * 4);
+    ENSURE(c->grp_ptr, (size_t)(NG + 1) * 4);
     ENSURE(c->grp, (size_t)n * S * 4);
-    LCHK(launch_group_count(P_<int>(c->topK), n, P, K, L2, Gg, P_<int>(c->wgcnt), st));
-    LCHK(scan_exclusive_i32(P_<int>(c->wgcnt), P_<int>(c->wgoff), (long long)L2 * Gg, nullptr, c->scan_tmp.p, c->scan_tmp.bytes, st));
-    LCHK(launch_group_scatter(P_<int>(c->topK), n, P, K, L2, Gg, P_<int>(c->wgoff), P_<uint32_t>(c->grp), P_<int>(c->grp_ptr), st));
-    c->h_grp_ptr.resize(L2 + 1);
-    HIPCHK(hipMemcpyAsync(c->h_grp_ptr.data(), c->grp_ptr.p, (size_t)(L2 + 1) * 4, hipMemcpyDeviceToHost, st));
+    ENSURE(c->scan_tmp, scan_tmp_bytes((long long)NG * Gx));
+    LCHK(launch_group_count(P_<int>(c->topK), n, P, K, L2, qblk, nblk, Gx, P_<int>(c->wgcnt), st));
+    LCHK(scan_exclusive_i32(P_<int>(c->wgcnt), P_<int>(c->wgoff), (long long)NG * Gx, nullptr, c->scan_tmp.p, c->scan_tmp.bytes, st));
+    LCHK(launch_group_scatter(P_<int>(c->topK), n, P, K, L2, qblk, nblk, Gx, P_<int>(c->wgoff), P_<uint32_t>(c->grp), P_<int>(c->grp_ptr), st));
+    c->h_grp_ptr.resize(NG + 1);
+    HIPCHK(hipMemcpyAsync(c->h_grp_ptr.data(), c->grp_ptr.p, (size_t)(NG + 1) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(c->ev[2], st));
     HIPCHK(hipStreamSynchronize(st));
     c->stats.t_transform_ms = elapsed(c->ev[0], c->ev[1]);
@@ -562,7 +575,8 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
         if (steps <= 0) return fail(c, SDB_EUNSUPPORTED, "topM too large for the LDS tile");
         da.kc = std::min(da.dk, steps * SE);
         da.nchunk = (da.dk + da.kc - 1) / da.kc;
-        QB = da.nchunk == 1 ? 1024 : 256;
+        QB = da.nchunk == 1 ? 4096 : 256;
+        if (const char* e = std::getenv("SDB_TILE_ROWS")) QB = std::max(256, std::atoi(e));
     } else {
         int kc = (kLdsBytes - 8 * M) / (4 * M + 128);
         kc = std::min(kc, c->d);
@@ -573,11 +587,12 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
         QB = 256;
     }
     std::vector<int> tl, tb, te;
-    for (int l = 0; l < L2; ++l)
-        for (int b = c->h_grp_ptr[l]; b < c->h_grp_ptr[l + 1]; b += QB) {
-            tl.push_back(l);
+    const int NG = c->nblk * L2;
+    for (int g = 0; g < NG; ++g)
+        for (int b = c->h_grp_ptr[g]; b < c->h_grp_ptr[g + 1]; b += QB) {
+            tl.push_back(g % L2);
             tb.push_back(b);
-            te.push_back(std::min(b + QB, c->h_grp_ptr[l + 1]));
+            te.push_back(std::min(b + QB, c->h_grp_ptr[g + 1]));
         }
     const int ntiles = (int)tl.size();
     ENSURE(c->tiles, (size_t)3 * std::max(ntiles, 1) * 4);

@@ -10,6 +10,8 @@
 // precision makes the same decision.  Hits are stored only for the first
 // occurrence of a candidate among the point's slots (F masks), which is the
 // reference's per-point dedup (src/sDbscan.cpp:260-282).
+#include <cstdlib>
+
 #include "sdb_kernels.hpp"
 
 namespace sdb {
@@ -27,7 +29,7 @@ __device__ __forceinline__ unsigned long long queue_entry(uint32_t ent, int l, i
 // 16 bytes of query row (l & 15) and of candidate column (l & 15) at k-group
 // (l >> 4) per step; A and B use the same k assignment, so the sum over k is
 // the dot product whatever order the unit uses internally.
-template <bool BF, int QT, int NT, int WAVES, int PF>
+template <bool BF, int QT, int NT, int WAVES, int PF, bool WHOLE>
 __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     constexpr int SZ = BF ? 2 : 4;
     constexpr int SE = 64 / SZ;  // elements per 64-byte k-step
@@ -87,44 +89,59 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
 #pragma unroll
             for (int q = 0; q < QT; ++q) arow[q] = Xc + ((size_t)pa[q] * a.ldx + k0) * SZ + g * 16;
             // query fragments are kept PF k-steps ahead in a register ring so
-            // that enough HBM reads are in flight per CU
+            // that enough HBM reads are in flight per CU.  Ring loads are
+            // unconditional (step index clamped to the last step) so the
+            // compiler can wait on each load individually.
             uint4 af[PF][QT];
 #pragma unroll
             for (int i = 0; i < PF; ++i)
 #pragma unroll
                 for (int q = 0; q < QT; ++q)
-                    if (i < nsteps) af[i][q] = *(const uint4*)(arow[q] + (size_t)i * 64);
-            for (int st0 = 0; st0 < nsteps; st0 += PF) {
+                    af[i][q] = *(const uint4*)(arow[q] + (size_t)min(i, nsteps - 1) * 64);
+            auto mfma_step = [&](int st, const uint4 (&ar)[QT]) {
 #pragma unroll
-                for (int i = 0; i < PF; ++i) {
-                    const int st = st0 + i;
-                    if (st >= nsteps) break;
+                for (int t = 0; t < NT; ++t) {
+                    int row = t * 16 + r16;
+                    row = row < M ? row : M - 1;
+                    const uint4 bfr = *(const uint4*)(smem + row * RSB + (size_t)st * 64 + g * 16);
 #pragma unroll
-                    for (int t = 0; t < NT; ++t) {
-                        int row = t * 16 + r16;
-                        row = row < M ? row : M - 1;
-                        uint4 bfr = *(const uint4*)(smem + row * RSB + (size_t)st * 64 + g * 16);
-#pragma unroll
-                        for (int q = 0; q < QT; ++q) {
-                            if constexpr (BF) {
-                                bf16x8_t av = __builtin_bit_cast(bf16x8_t, af[i][q]);
-                                bf16x8_t bv = __builtin_bit_cast(bf16x8_t, bfr);
-                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[q][t], 0, 0, 0);
-                            } else {
-                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[i][q].x), __uint_as_float(bfr.x), acc[q][t], 0, 0, 0);
-                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[i][q].y), __uint_as_float(bfr.y), acc[q][t], 0, 0, 0);
-                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[i][q].z), __uint_as_float(bfr.z), acc[q][t], 0, 0, 0);
-                                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(af[i][q].w), __uint_as_float(bfr.w), acc[q][t], 0, 0, 0);
-                            }
+                    for (int q = 0; q < QT; ++q) {
+                        if constexpr (BF) {
+                            const bf16x8_t av = __builtin_bit_cast(bf16x8_t, ar[q]);
+                            const bf16x8_t bv = __builtin_bit_cast(bf16x8_t, bfr);
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[q][t], 0, 0, 0);
+                        } else {
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(ar[q].x), __uint_as_float(bfr.x), acc[q][t], 0, 0, 0);
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(ar[q].y), __uint_as_float(bfr.y), acc[q][t], 0, 0, 0);
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(ar[q].z), __uint_as_float(bfr.z), acc[q][t], 0, 0, 0);
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(ar[q].w), __uint_as_float(bfr.w), acc[q][t], 0, 0, 0);
                         }
                     }
-                    const int ns = st + PF;
-                    if (ns < nsteps) {
+                }
+            };
+            if constexpr (WHOLE) {
+                // whole query rows are read in one burst (nsteps <= PF): every
+                // step's fragment was issued above
 #pragma unroll
-                        for (int q = 0; q < QT; ++q) af[i][q] = *(const uint4*)(arow[q] + (size_t)ns * 64);
-                    }
+                for (int i = 0; i < PF; ++i)
+                    if (i < nsteps) mfma_step(i, af[i]);
+                continue;
+            }
+            const int ngroups = nsteps / PF;
+            const int rem = nsteps - ngroups * PF;
+            for (int gi = 0; gi < ngroups; ++gi) {
+#pragma unroll
+                for (int i = 0; i < PF; ++i) {
+                    const int st = gi * PF + i;
+                    mfma_step(st, af[i]);
+                    const int ns = min(st + PF, nsteps - 1);
+#pragma unroll
+                    for (int q = 0; q < QT; ++q) af[i][q] = *(const uint4*)(arow[q] + (size_t)ns * 64);
                 }
             }
+#pragma unroll
+            for (int i = 0; i < PF - 1; ++i)
+                if (i < rem) mfma_step(ngroups * PF + i, af[i]);
         }
 
         // epilogue: lane (r16, g) holds rows 4g..4g+3 of column t*16 + r16
@@ -183,16 +200,30 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     }
 }
 
-template <bool BF, int NT>
-static int launch_mfma_nt(const DistArgs& a, int ntiles, hipStream_t st) {
-    constexpr int QT = 2, WAVES = 8, PF = BF ? 4 : 1;
+template <bool BF, int NT, int QT, int PF, bool WHOLE>
+static int launch_mfma_v(const DistArgs& a, int ntiles, hipStream_t st) {
+    constexpr int WAVES = 8;
     const size_t sm = (size_t)a.M * ((size_t)a.kc * (BF ? 2 : 4) + 16) + (size_t)a.M * 4;
     if (sm > (size_t)kLdsBytes) return -2;
-    hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES, PF>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES, PF, WHOLE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL((k_dist_mfma<BF, QT, NT, WAVES, PF>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
+    hipLaunchKernelGGL((k_dist_mfma<BF, QT, NT, WAVES, PF, WHOLE>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
     return (int)hipGetLastError();
+}
+
+template <bool BF, int NT>
+static int launch_mfma_nt(const DistArgs& a, int ntiles, hipStream_t st) {
+    if constexpr (BF) {
+        int v = 1;
+        if (const char* e = std::getenv("SDB_DIST_VARIANT")) v = std::atoi(e);
+        const int nsteps = a.kc / 32;
+        if (v == 1 && nsteps <= 25) return launch_mfma_v<BF, NT, 1, 25, true>(a, ntiles, st);
+        if (v == 3 && nsteps <= 26) return launch_mfma_v<BF, NT, 2, 13, false>(a, ntiles, st);
+        return launch_mfma_v<BF, NT, 2, 5, false>(a, ntiles, st);
+    } else {
+        return launch_mfma_v<BF, NT, 2, 1, false>(a, ntiles, st);
+    }
 }
 
 template <bool BF>

@@ -776,16 +776,28 @@ __global__ __launch_bounds__(256) void k_first_mask(const int* __restrict__ topK
     }
 }
 
-// Grouping of the (point, slot) entries by list without contended global
-// atomics: each workgroup counts its contiguous range of entries per list in
-// LDS (k_group_count), the counts are scanned list-major, and the workgroup
-// scatters its entries from LDS cursors (k_group_scatter).
+// Grouping of the (point, slot) entries by (query block, list) without
+// contended global atomics.  Query block y holds the points
+// [y * qblk, (y + 1) * qblk); group y * L2 + l lists its entries that visit
+// list l.  Workgroup (x, y) counts a contiguous part of block y's entries per
+// list in LDS (k_group_count); the counts are scanned in (y, l, x) order and
+// the workgroup scatters its entries from LDS cursors (k_group_scatter).
+__device__ __forceinline__ void group_range(long long n, int K, long long qblk, int Gx, long long& t0,
+                                            long long& t1) {
+    const long long S = 2 * K;
+    const long long b0 = (long long)blockIdx.y * qblk * S;
+    const long long b1 = min((long long)(blockIdx.y + 1) * qblk, n) * S;
+    const long long per = (b1 - b0 + Gx - 1) / Gx;
+    t0 = b0 + (long long)blockIdx.x * per;
+    t1 = min(t0 + per, b1);
+}
+
 __global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ topK, long long n, int P,
-                                                     int K, int L2, int G, int* __restrict__ wgcnt) {
+                                                     int K, int L2, long long qblk, int Gx,
+                                                     int* __restrict__ wgcnt) {
     extern __shared__ int hist[];
-    const long long tot = n * 2 * K;
-    const long long per = (tot + G - 1) / G;
-    const long long t0 = (long long)blockIdx.x * per, t1 = t0 + per < tot ? t0 + per : tot;
+    long long t0, t1;
+    group_range(n, K, qblk, Gx, t0, t1);
     for (int i = threadIdx.x; i < L2; i += blockDim.x) hist[i] = 0;
     __syncthreads();
     for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
@@ -794,17 +806,19 @@ __global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ top
         atomicAdd(&hist[slot_list(topK, a, s, K, P)], 1);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < L2; i += blockDim.x) wgcnt[(size_t)i * G + blockIdx.x] = hist[i];
+    for (int i = threadIdx.x; i < L2; i += blockDim.x)
+        wgcnt[((size_t)blockIdx.y * L2 + i) * Gx + blockIdx.x] = hist[i];
 }
 
 __global__ __launch_bounds__(256) void k_group_scatter(const int* __restrict__ topK, long long n, int P,
-                                                       int K, int L2, int G, const int* __restrict__ wgoff,
+                                                       int K, int L2, long long qblk, int Gx,
+                                                       const int* __restrict__ wgoff,
                                                        uint32_t* __restrict__ grp) {
     extern __shared__ int cur[];
-    const long long tot = n * 2 * K;
-    const long long per = (tot + G - 1) / G;
-    const long long t0 = (long long)blockIdx.x * per, t1 = t0 + per < tot ? t0 + per : tot;
-    for (int i = threadIdx.x; i < L2; i += blockDim.x) cur[i] = wgoff[(size_t)i * G + blockIdx.x];
+    long long t0, t1;
+    group_range(n, K, qblk, Gx, t0, t1);
+    for (int i = threadIdx.x; i < L2; i += blockDim.x)
+        cur[i] = wgoff[((size_t)blockIdx.y * L2 + i) * Gx + blockIdx.x];
     __syncthreads();
     for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
         const long long a = t / (2 * K);
@@ -814,11 +828,11 @@ __global__ __launch_bounds__(256) void k_group_scatter(const int* __restrict__ t
     }
 }
 
-__global__ void k_group_ptr(const int* __restrict__ wgoff, int L2, int G, long long tot,
+__global__ void k_group_ptr(const int* __restrict__ wgoff, int NG, int Gx, long long tot,
                             int* __restrict__ grp_ptr) {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < L2) grp_ptr[l] = wgoff[(size_t)l * G];
-    if (l == L2) grp_ptr[L2] = (int)tot;
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < NG) grp_ptr[g] = wgoff[(size_t)g * Gx];
+    if (g == NG) grp_ptr[NG] = (int)tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -934,15 +948,17 @@ int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, 
     return (int)hipGetLastError();
 }
 
-int launch_group_count(const int* topK, long long n, int P, int K, int L2, int G, int* wgcnt, hipStream_t st) {
-    hipLaunchKernelGGL(k_group_count, dim3(G), dim3(256), (size_t)L2 * 4, st, topK, n, P, K, L2, G, wgcnt);
+int launch_group_count(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
+                       int Gx, int* wgcnt, hipStream_t st) {
+    hipLaunchKernelGGL(k_group_count, dim3(Gx, nblk), dim3(256), (size_t)L2 * 4, st, topK, n, P, K, L2, qblk, Gx, wgcnt);
     return (int)hipGetLastError();
 }
 
-int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, int G, const int* wgoff,
-                         uint32_t* grp, int* grp_ptr, hipStream_t st) {
-    hipLaunchKernelGGL(k_group_scatter, dim3(G), dim3(256), (size_t)L2 * 4, st, topK, n, P, K, L2, G, wgoff, grp);
-    hipLaunchKernelGGL(k_group_ptr, dim3((L2 + 1 + 255) / 256), dim3(256), 0, st, wgoff, L2, G, n * 2 * K, grp_ptr);
+int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
+                         int Gx, const int* wgoff, uint32_t* grp, int* grp_ptr, hipStream_t st) {
+    hipLaunchKernelGGL(k_group_scatter, dim3(Gx, nblk), dim3(256), (size_t)L2 * 4, st, topK, n, P, K, L2, qblk, Gx, wgoff, grp);
+    const int NG = nblk * L2;
+    hipLaunchKernelGGL(k_group_ptr, dim3((NG + 1 + 255) / 256), dim3(256), 0, st, wgoff, NG, Gx, n * 2 * K, grp_ptr);
     return (int)hipGetLastError();
 }
 

@@ -82,9 +82,10 @@ int launch_multi(const int* lists, const int* hub_id, const int* hub_ptr, int L2
 int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const uint32_t* multi,
                       const int* lists, const int* hub_id, const int* hub_ptr, const int* hub_mem,
                       uint32_t* F, hipStream_t st);
-int launch_group_count(const int* topK, long long n, int P, int K, int L2, int G, int* wgcnt, hipStream_t st);
-int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, int G, const int* wgoff,
-                         uint32_t* grp, int* grp_ptr, hipStream_t st);
+int launch_group_count(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
+                       int Gx, int* wgcnt, hipStream_t st);
+int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
+                         int Gx, const int* wgoff, uint32_t* grp, int* grp_ptr, hipStream_t st);
 
 // distances
 int launch_dist_mfma(const DistArgs& a, int precision, int ntiles, hipStream_t st);
