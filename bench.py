@@ -36,6 +36,19 @@ def dist_bytes(n, d, s):
     return (n * 2 * K + 2 * P * M) * d * s + n * 2 * K * M / 8
 
 
+def measured_traffic(n):
+    """HBM bytes per launch of the BF16 distance kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None when the
+    summary is missing or was measured at another size."""
+    path = os.path.join(ROOT, "profiles", "k_dist_mfma_bf16_8Mx784_hbm_counters.txt")
+    if n != N or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def run_path(ctx, bits, steps, warmup, tag):
     times, dist_ms, phase = [], [], []
     lab = None
@@ -150,7 +163,7 @@ def main():
                    "n_points": n, "n_features": D_FEAT},
         "roofline": {"bound": "hbm", "kernel": "k_dist_mfma<bf16>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": B, "avg_ms": dist_avg},
+                     "traffic": measured_traffic(n), "algorithmic_bytes": B, "avg_ms": dist_avg},
         "cpu_baseline": cpu,
         "phases_ms": {k: last[k] for k in ("t_transform_ms", "t_index_ms", "t_core_ms",
                                            "t_dist_ms", "t_cluster_ms")},

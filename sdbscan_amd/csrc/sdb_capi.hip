@@ -486,7 +486,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     if (const char* e = std::getenv("SDB_QBLK_MB")) qblk_mb = std::max(1ll, std::atoll(e));
     const long long qblk = std::max<long long>(1, (qblk_mb << 20) / ((long long)ldx * 2));
     const int nblk = (int)((n + qblk - 1) / qblk);
-    const int Gx = (int)std::max<long long>(1, std::min<long long>(64, 2048 / nblk));
+    const int Gx = (int)std::max<long long>(1, std::min<long long>(2048, 2048 / nblk));
     const int NG = nblk * L2;
     c->qblk = qblk;
     c->nblk = nblk;
