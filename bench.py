@@ -40,7 +40,7 @@ def measured_traffic(n):
     """HBM bytes per launch of the BF16 distance kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None when the
     summary is missing or was measured at another size."""
-    path = os.path.join(ROOT, "profiles", "k_dist_mfma_bf16_8Mx784_hbm_counters.txt")
+    path = os.path.join(ROOT, "profiles", "k_dist_pipe_bf16_8Mx784_hbm_counters.txt")
     if n != N or not os.path.exists(path):
         return None
     for line in open(path):
@@ -161,7 +161,7 @@ def main():
         "config": {"workload": f"sDbscan cosine n={n} d={D_FEAT} n_proj={P} topK={K} topM={M} "
                                f"eps={EPS} minPts={MINPTS}, BF16 distance tiles + FP32/FP64 recheck",
                    "n_points": n, "n_features": D_FEAT},
-        "roofline": {"bound": "hbm", "kernel": "k_dist_mfma<bf16>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "k_dist_pipe<bf16>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(n), "algorithmic_bytes": B, "avg_ms": dist_avg},
         "cpu_baseline": cpu,

@@ -9,7 +9,8 @@
  *                         (src/sDbscan.cpp:19-209); sign bits and RFF matrix come from
  *                         the host RNG calls of src/Utilities.h:47-122
  *   sdb_find_core      <- rp_findCorePoints (src/sDbscan.cpp:217-408)
- *   sdb_form_clusters  <- formCluster (src/sDbscan.cpp:509-637), labelNoise excluded
+ *   sdb_form_clusters  <- formCluster (src/sDbscan.cpp:509-637)
+ *   sdb_label_noise    <- labelNoise, sDbscan-1NN (src/sDbscan.cpp:685-807)
  *   sdb_core_dist      <- rp_findCoreDist (src/sDbscan.cpp:1314-1474)
  *   sdb_optics_reach   <- the reachability values of formOptics_scikit
  *                         (src/sDbscan.cpp:1588-1703) for a given ordering
@@ -99,6 +100,15 @@ int sdb_find_core(sdb_ctx* ctx, float eps, int minPts, uint8_t* core_out, int* d
 /* Cluster labels after sdb_find_core; n_clusters as the reference reports
  * it (components + 1, or 2 when there is none). */
 int sdb_form_clusters(sdb_ctx* ctx, int* labels_out, int* n_clusters_out);
+
+/* sDbscan-1NN noise labelling after sdb_form_clusters (labelNoise with
+ * clusterNoise = 1, src/sDbscan.cpp:685-807): the sampled core points are
+ * projected again, and every point labelled -1 takes the label of the
+ * sampled core maximising sum_k proj[close_k] - proj[far_k] (the first such
+ * core in the given order).  sampled_core lists the sampled core points in
+ * increasing index order, drawn by the caller as the reference draws them
+ * (src/sDbscan.cpp:695-709).  labels_out (n ints) may be NULL. */
+int sdb_label_noise(sdb_ctx* ctx, const int* sampled_core, int n_sampled, int* labels_out);
 
 /* Index matrices in the reference layouts (either pointer may be NULL). */
 int sdb_get_index(sdb_ctx* ctx, int* topK_out, int* topM_out);

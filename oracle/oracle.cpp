@@ -498,6 +498,41 @@ int orc_form_cluster(int n, const uint8_t* core, const int* ptr, const int* idx,
     return n_clusters + 2;  // src/sDbscan.cpp:633
 }
 
+// ---------------------------------------------------------- label noise
+// sDbscan-1NN (src/sDbscan.cpp:775-806): cproj is P x nS (row = random
+// vector, column i = sampled core i).  For each point labelled -1:
+// vecDotEst = 0; for k: += col(close_k); -= col(far_k); the first sampled
+// core with the strictly largest estimate (from NEG_INF) gives the label.
+void orc_label_noise(int P, int nS, const float* cproj, int n, int K, const int* topK,
+                     const int* sampled, int* labels) {
+    (void)P;
+    std::vector<int> noise;
+    for (int a = 0; a < n; ++a)
+        if (labels[a] == -1) noise.push_back(a);
+    std::vector<int> out(noise.size(), -1);
+#pragma omp parallel for
+    for (long long t = 0; t < (long long)noise.size(); ++t) {
+        const int a = noise[t];
+        std::vector<float> est(nS, 0.0f);
+        for (int k = 0; k < K; ++k) {
+            const float* rc = cproj + (size_t)topK[(size_t)a * 2 * K + k] * nS;
+            const float* rf = cproj + (size_t)topK[(size_t)a * 2 * K + K + k] * nS;
+            for (int i = 0; i < nS; ++i) est[i] += rc[i];
+            for (int i = 0; i < nS; ++i) est[i] -= rf[i];
+        }
+        int best = -1;
+        float best_so_far = -2147483648.0f;  // NEG_INF
+        for (int i = 0; i < nS; ++i)
+            if (est[i] > best_so_far) {
+                best_so_far = est[i];
+                best = sampled[i];
+            }
+        out[t] = best;
+    }
+    for (size_t t = 0; t < noise.size(); ++t)
+        if (out[t] >= 0) labels[noise[t]] = labels[out[t]];
+}
+
 // ------------------------------------------------------------- sOPTICS
 // Core distance (src/sDbscan.cpp:1423-1447): the (minPts-1)-th smallest
 // neighbour distance of a core point, 0 otherwise.

@@ -30,6 +30,7 @@ def lib():
             "orc_cauchy": ([ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, _P], None),
             "orc_transform": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
             "orc_fht": ([_P, ctypes.c_int], None),
+            "orc_label_noise": ([ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P], None),
             "orc_fwht_textbook": ([_P, ctypes.c_int], None),
             "orc_embed": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P, _P], None),
             "orc_project_embedded": ([_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P], None),
@@ -177,6 +178,20 @@ def form_cluster(core, ptr, idx):
                                 idx.ctypes.data if len(idx) else np.zeros(1, np.int32).ctypes.data,
                                 lab.ctypes.data)
     return lab, nc
+
+
+def label_noise(cproj, topK, labels, sampled):
+    """sDbscan-1NN labels (src/sDbscan.cpp:775-806); cproj: P x nS projections
+    of the sampled core points, in the order of `sampled`."""
+    cproj = _c(cproj, np.float32)
+    topK = _c(topK, np.int32)
+    sampled = _c(sampled, np.int32)
+    lab = np.array(labels, np.int32, copy=True)
+    P, nS = cproj.shape
+    n, K2 = topK.shape
+    lib().orc_label_noise(P, nS, cproj.ctypes.data, n, K2 // 2, topK.ctypes.data, sampled.ctypes.data,
+                          lab.ctypes.data)
+    return lab
 
 
 def core_dist(minPts, ptr, dd):

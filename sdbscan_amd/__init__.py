@@ -57,6 +57,7 @@ _SIGS = {
     "sdb_build_index": ([_P, _P, _P], ctypes.c_int),
     "sdb_find_core": ([_P, ctypes.c_float, ctypes.c_int, _P, _P], ctypes.c_int),
     "sdb_form_clusters": ([_P, _P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "sdb_label_noise": ([_P, _P, ctypes.c_int, _P], ctypes.c_int),
     "sdb_get_index": ([_P, _P, _P], ctypes.c_int),
     "sdb_get_transformed": ([_P, _P], ctypes.c_int),
     "sdb_get_neighbors": ([_P, _P, _P, _P], ctypes.c_int),
@@ -180,6 +181,16 @@ class Context:
         self._chk(self.lib.sdb_form_clusters(self.h, lab.ctypes.data if want else None,
                                              ctypes.byref(nc)))
         return lab, nc.value
+
+    def label_noise(self, sampled_core):
+        """sDbscan-1NN labels of the noise points after form_clusters, for the
+        given sampled core points (increasing index order)."""
+        import numpy as np
+        ids = np.ascontiguousarray(sampled_core, dtype=np.int32)
+        lab = np.zeros(self.params.n_points, np.int32)
+        self._chk(self.lib.sdb_label_noise(self.h, ids.ctypes.data if len(ids) else None, len(ids),
+                                           lab.ctypes.data))
+        return lab
 
     def get_index(self):
         import numpy as np

@@ -84,6 +84,27 @@ PYBIND11_MODULE(sDbscan, m) {
                 s.fit_sOptics(p, eps, minPts);
             },
             py::arg("X"), py::arg("eps"), py::arg("minPts"))
+        .def(
+            "load_fit_sDbscan",
+            [](PySDbscan& s, const std::string& dataset, float eps, int minPts) {
+                py::gil_scoped_release rel;
+                s.load_fit_sDbscan(dataset, eps, minPts);
+            },
+            py::arg("dataset"), py::arg("eps"), py::arg("minPts"))
+        .def(
+            "load_test_sDbscan",
+            [](PySDbscan& s, const std::string& dataset, float eps, float rangeEps, int n_tests, int minPts) {
+                py::gil_scoped_release rel;
+                s.load_test_sDbscan(dataset, eps, rangeEps, n_tests, minPts);
+            },
+            py::arg("dataset"), py::arg("eps"), py::arg("rangeEps"), py::arg("n_tests"), py::arg("minPts"))
+        .def(
+            "load_fit_sOptics",
+            [](PySDbscan& s, const std::string& dataset, float eps, int minPts) {
+                py::gil_scoped_release rel;
+                s.load_fit_sOptics(dataset, eps, minPts);
+            },
+            py::arg("dataset"), py::arg("eps"), py::arg("minPts"))
         // extensions of this build
         .def("set_precision", &PySDbscan::set_precision, py::arg("precision"))
         .def("set_device", &PySDbscan::set_device, py::arg("device"))

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <functional>
 #include <iostream>
@@ -170,8 +172,6 @@ void sDbscan::prepare(const float* X) {
     if (fhtDim <= 0) set_params(n_proj, topK, topM, distance, ker_n_features, ker_sigma,
                                 ker_intervalSampling, clusterNoise, samplingProb, verbose, output,
                                 n_threads, seed);
-    if (clusterNoise != 0)
-        throw std::invalid_argument("clusterNoise (labelNoise, src/sDbscan.cpp:647-822) is not part of this build");
     ensure_ctx();
     sdb_params p{};
     p.n_points = n_points;
@@ -214,14 +214,86 @@ std::string sDbscan::out_name(float eps, int minPts, bool optics) const {
            "_NumProj_" + int2str(n_proj) + "_TopM_" + int2str(topM) + "_TopK_" + int2str(topK);
 }
 
+// Noise labelling of formCluster (src/sDbscan.cpp:625-627 -> labelNoise,
+// :647-807).  clusterNoise = 2 (heuristic, :661-682) walks the kept
+// neighbourhood of each noise point, but a noise point is never core and the
+// neighbourhoods of non-core points are not kept (:374-400), so no label
+// changes.  Any other nonzero value is sDbscan-1NN: the core points are
+// sampled here exactly as the reference samples them (same expressions, the
+// process's rand() in increasing index order, :688-709) and the device does
+// the projections and the 1NN scoring.
+void sDbscan::label_noise(const std::vector<uint8_t>& core) {
+    if (clusterNoise == 0) return;
+    if (verbose) {
+        long long nn = 0;
+        for (int v : labels_) nn += v == -1;
+        std::cout << "Number of noisy points: " << nn << std::endl;
+    }
+    if (clusterNoise == 2) return;
+    int iNumCore = 0;
+    for (uint8_t c : core) iNumCore += c ? 1 : 0;
+    float fProb = 1.0;
+    if (iNumCore >= samplingProb * n_points) fProb = samplingProb * n_points / iNumCore;
+    std::vector<int> sampled;
+    for (int Xi = 0; Xi < n_points; ++Xi) {
+        if (!core[Xi]) continue;
+        if (fProb == 1.0)
+            sampled.push_back(Xi);
+        else if (rand() / (RAND_MAX + 1.) <= fProb)
+            sampled.push_back(Xi);
+    }
+    if (verbose) std::cout << "Number of sampled core points: " << sampled.size() << std::endl;
+    check(sdb_label_noise(ctx, sampled.data(), (int)sampled.size(), labels_.data()), "label_noise");
+}
+
+// loadtxtData (src/Utilities.cpp:60-95): n lines of d whitespace-separated
+// floats read with fscanf("%f") into n rows of d floats (the reference's d x n
+// column-major matrix); values the file lacks stay 0, as in the reference.
+// The transform of the distance runs on the device with the rest of the fit.
+std::vector<float> sDbscan::load_txt(const std::string& dataset) const {
+    FILE* f = std::fopen(dataset.c_str(), "r");
+    if (!f) throw std::runtime_error("Error: Data file does not exist !");
+    std::vector<float> X((size_t)n_points * n_features, 0.0f);
+    for (size_t i = 0; i < X.size(); ++i)
+        if (std::fscanf(f, "%f", &X[i]) != 1) break;
+    std::fclose(f);
+    std::cout << "Finish reading data" << std::endl;
+    std::cout << "Now checking the condition of data given the distance." << std::endl;
+    return X;
+}
+
+// src/sDbscan.cpp:903-976
+void sDbscan::load_fit_sDbscan(const std::string& dataset, float eps, int minPts) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<float> X = load_txt(dataset);
+    if (verbose)
+        std::cout << "Loading data time = " << std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t0).count()
+                  << "[ms]" << std::endl;
+    fit_sDbscan(X.data(), eps, minPts);
+}
+
+// src/sDbscan.cpp:1053-1112
+void sDbscan::load_test_sDbscan(const std::string& dataset, float eps, float rangeEps, int n_tests, int minPts) {
+    std::vector<float> X = load_txt(dataset);
+    test_sDbscan(X.data(), eps, rangeEps, n_tests, minPts);
+}
+
+// src/sDbscan.cpp:1777-1838
+void sDbscan::load_fit_sOptics(const std::string& dataset, float eps, int minPts) {
+    std::vector<float> X = load_txt(dataset);
+    fit_sOptics(X.data(), eps, minPts);
+}
+
 // src/sDbscan.cpp:830-890
 void sDbscan::fit_sDbscan(const float* X, float eps, int minPts) {
     if (verbose) print_params(eps, minPts);
     prepare(X);
     check(sdb_build_index(ctx, bitHD3.data(), matrix_R.empty() ? nullptr : matrix_R.data()), "build_index");
-    check(sdb_find_core(ctx, eps, minPts, nullptr, nullptr), "find_core");
+    std::vector<uint8_t> core(clusterNoise ? n_points : 0);
+    check(sdb_find_core(ctx, eps, minPts, clusterNoise ? core.data() : nullptr, nullptr), "find_core");
     labels_.assign(n_points, -1);
     check(sdb_form_clusters(ctx, labels_.data(), &n_clusters_), "form_clusters");
+    label_noise(core);
     if (verbose) {
         sdb_stats s{};
         sdb_get_stats(ctx, &s);
@@ -250,9 +322,11 @@ void sDbscan::test_sDbscan(const float* X, float eps, float rangeEps, int n_test
     for (int i = 0; i < n_tests; ++i) {
         const float e = eps + 1.0f * i * rangeEps;
         std::cout << "Eps: " << e << std::endl;
-        check(sdb_find_core(ctx, e, minPts, nullptr, nullptr), "find_core");
+        std::vector<uint8_t> core(clusterNoise ? n_points : 0);
+        check(sdb_find_core(ctx, e, minPts, clusterNoise ? core.data() : nullptr, nullptr), "find_core");
         labels_.assign(n_points, -1);
         check(sdb_form_clusters(ctx, labels_.data(), &n_clusters_), "form_clusters");
+        label_noise(core);
         sdb_get_stats(ctx, &s);
         std::cout << "Find core points time = " << (long long)s.t_core_ms << "[ms]" << std::endl;
         std::cout << "Form clusters time (including clustering noise) = " << (long long)s.t_cluster_ms << "[ms]" << std::endl;

@@ -59,6 +59,8 @@ protected:
     void make_random();
     void print_params(float eps, int minPts) const;
     std::string out_name(float eps, int minPts, bool optics) const;
+    void label_noise(const std::vector<uint8_t>& core);
+    std::vector<float> load_txt(const std::string& dataset) const;
 
 public:
     // Dbscan's output
@@ -95,6 +97,11 @@ public:
     void fit_sDbscan(const float* X, float eps, int minPts);
     void test_sDbscan(const float* X, float eps, float rangeEps, int n_tests, int minPts);
     void fit_sOptics(const float* X, float eps, int minPts);
+
+    // Text datasets: n_points lines of n_features floats (src/Utilities.cpp:60-95)
+    void load_fit_sDbscan(const std::string& dataset, float eps, int minPts);
+    void load_test_sDbscan(const std::string& dataset, float eps, float rangeEps, int n_tests, int minPts);
+    void load_fit_sOptics(const std::string& dataset, float eps, int minPts);
 
     // Extensions of this build (no counterpart in the reference)
     void set_precision(const std::string& p);  // "FP32" (default) or "BF16" distance tiles

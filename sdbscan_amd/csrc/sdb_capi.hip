@@ -3,6 +3,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -30,6 +31,11 @@ int launch_reach(long long n, const long long* ptr, const int* idx, const float*
                  hipStream_t st);
 int launch_kth(long long n, const long long* ptr, const float* sorted, const uint8_t* core,
                int k, float* out, hipStream_t st);
+int launch_noise_list(const int* labels, long long n, int* noise, int* count, hipStream_t st);
+int launch_noise_score(const float* cproj, long long nS, const int* topK, int K, const int* noise, int n_noise,
+                       unsigned long long* best, hipStream_t st);
+int launch_noise_assign(const int* noise, int n_noise, const unsigned long long* best, const int* sampled,
+                        int* labels, hipStream_t st);
 }  // namespace sdb
 
 using namespace sdb;
@@ -57,7 +63,7 @@ struct sdb_ctx {
     // index
     DevBuf bits, R, thr_any, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
         cand_idx, topM, flags, lists, is_hub, hub_flags, hub_excl, hub_id, hub_cnt, hub_ptr,
-        hub_cur, hub_mem, inv, multi, F, wgcnt, wgoff, grp_ptr, grp, scan_tmp, total, err_flag,
+        hub_cur, hub_mem, inv, pos_hub, pcnt, poff, pairs, F, wgcnt, wgoff, grp_ptr, grp, scan_tmp, total, err_flag,
         row_slot, row_buf;
     bool indexed = false;
     int H = 0, L2 = 0, W = 0, S = 0;
@@ -75,6 +81,9 @@ struct sdb_ctx {
     // neighbourhoods / optics
     DevBuf nb_deg, nb_ptr, nb_cur, nb_idx, nb_dist, nb_sorted, core_dist, pos, reach;
     bool have_nb = false;
+    bool clustered = false;
+    // noise labelling
+    DevBuf ns_ids, cproj, noise, nbest;
     sdb_stats stats{};
 };
 
@@ -189,12 +198,13 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->tie_count, &c->cmax, &c->cmin, &c->thi, &c->tlo, &c->cand_cnt, &c->cand_key,
                      &c->cand_idx, &c->topM, &c->flags, &c->lists, &c->is_hub, &c->hub_flags,
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
-                     &c->inv, &c->multi, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
+                     &c->inv, &c->pos_hub, &c->pcnt, &c->poff, &c->pairs, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
                      &c->scan_tmp, &c->total, &c->err_flag, &c->row_slot, &c->row_buf, &c->E,
                      &c->colsum, &c->queue, &c->qcount, &c->tiles, &c->degree, &c->core,
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
                      &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
-                     &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach};
+                     &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
+                     &c->cproj, &c->noise, &c->nbest};
     for (DevBuf* b : all)
         if (b->p) hipFree(b->p);
     for (int i = 0; i < 8; ++i) hipEventDestroy(c->ev[i]);
@@ -474,17 +484,31 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ENSURE(c->inv, tbl);
     HIPCHK(hipMemsetAsync(c->inv.p, 0, tbl, st));
     LCHK(launch_invmask(P_<int>(c->lists), P_<int>(c->topK), P, K, M, L2, W, P_<uint32_t>(c->inv), st));
-    ENSURE(c->multi, (size_t)L2 * W * 4);
-    LCHK(launch_multi(P_<int>(c->lists), P_<int>(c->hub_id), P_<int>(c->hub_ptr), L2, M, W, P_<uint32_t>(c->multi), st));
+    // (position, other list) pairs of every list, for the first-occurrence masks
+    ENSURE(c->pos_hub, (size_t)LM * 4);
+    ENSURE(c->pcnt, (size_t)(LM + 1) * 4);
+    ENSURE(c->poff, (size_t)(LM + 1) * 4);
+    LCHK(launch_pair_count(P_<int>(c->lists), P_<int>(c->hub_id), P_<int>(c->hub_ptr), L2, M, P_<int>(c->pos_hub),
+                           P_<int>(c->pcnt), P_<int>(c->poff), P_<int>(c->total), c->scan_tmp.p, c->scan_tmp.bytes, st));
+    int npairs = 0;
+    HIPCHK(hipMemcpyAsync(&npairs, c->total.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ENSURE(c->pairs, (size_t)std::max(npairs, 1) * 4);
+    LCHK(launch_pair_fill(P_<int>(c->pos_hub), P_<int>(c->hub_ptr), P_<int>(c->hub_mem), L2, M, P_<int>(c->poff),
+                          P_<uint32_t>(c->pairs), st));
     ENSURE(c->F, (size_t)n * S * W * 4);
-    LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, W, P_<uint32_t>(c->multi), P_<int>(c->lists),
+    LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, W, P_<int>(c->poff), P_<uint32_t>(c->pairs),
                            P_<int>(c->hub_id), P_<int>(c->hub_ptr), P_<int>(c->hub_mem), P_<uint32_t>(c->F), st));
-    // group the (point, slot) entries by (query block, list); a query block's
-    // rows (BF16) fit in the 256 MB last-level cache, so the 2K visits of a
-    // query row after the first are cache hits
-    long long qblk_mb = 1ll << 20;
+    // group the (point, slot) entries by (query block, list): the distance
+    // tiles then sweep the points block by block, and a 192 MB block of query
+    // rows stays in the 256 MB last-level cache, so most of the 2K visits of a
+    // row after the first hit there (larger blocks lose hits, smaller ones
+    // restage the lists' candidate rows more often)
+    // (FP32 tiles are MFMA-bound and only lose from smaller tiles: one block)
+    long long qblk_mb = q.precision == SDB_BF16 ? 192 : (1ll << 30);
     if (const char* e = std::getenv("SDB_QBLK_MB")) qblk_mb = std::max(1ll, std::atoll(e));
-    const long long qblk = std::max<long long>(1, (qblk_mb << 20) / ((long long)ldx * 2));
+    const long long row_bytes = (long long)ldx * (q.precision == SDB_BF16 ? 2 : 4);
+    const long long qblk = std::max<long long>(1, (qblk_mb << 20) / row_bytes);
     const int nblk = (int)((n + qblk - 1) / qblk);
     const int Gx = (int)std::max<long long>(1, std::min<long long>(2048, 2048 / nblk));
     const int NG = nblk * L2;
@@ -530,6 +554,45 @@ int sdb_get_transformed(sdb_ctx* c, float* X_out) {
 // Error band of the fast distance, derived from the standard bound
 // |fl(sum) - sum| <= gamma_k * sum |terms| (any summation order, one rounding of
 // at most 2^-23 relative per operation), with a safety factor of 2.
+// Total order on float bit patterns (host side of the device ord_key).
+static uint32_t h_key(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+static float h_unkey(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Cuts on the raw cosine dot s equivalent to the band tests on the distance
+// fl(1 - s): fl(1 - s) is non-increasing in s, so "fl(1 - s) < lo" holds
+// exactly for s > s_hit and "not fl(1 - s) > hi" exactly for s >= s_near.
+// Found by bisection over the ordered finite floats.
+static void cosine_cuts(double lo, double hi, float* s_hit, float* s_near) {
+    auto is_hit = [&](float s) { return (double)(1.0f - s) < lo; };
+    auto is_near = [&](float s) { return !((double)(1.0f - s) > hi); };
+    auto bisect = [](auto pred, float* last_false, float* first_true) {
+        if (pred(-FLT_MAX)) { *last_false = -INFINITY; *first_true = -INFINITY; return; }
+        if (!pred(FLT_MAX)) { *last_false = FLT_MAX; *first_true = INFINITY; return; }
+        uint32_t a = h_key(-FLT_MAX), b = h_key(FLT_MAX);  // pred(a) false, pred(b) true
+        while (b - a > 1) {
+            const uint32_t m = a + (b - a) / 2;
+            if (pred(h_unkey(m))) b = m;
+            else a = m;
+        }
+        *last_false = h_unkey(a);
+        *first_true = h_unkey(b);
+    };
+    float lf, ft;
+    bisect(is_hit, &lf, &ft);
+    *s_hit = lf == FLT_MAX ? INFINITY : lf;
+    bisect(is_near, &lf, &ft);
+    *s_near = ft;
+}
+
 static void compute_bands(const sdb_params& q, float eps, double* band, double* band32) {
     const double u = std::ldexp(1.0, -23);
     const double d = q.n_features;
@@ -571,12 +634,13 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
     if (mfma) {
         const int SE = prec == SDB_BF16 ? 32 : 16;
         da.dk = (c->d + SE - 1) / SE * SE;
-        const int steps = (int)(((kLdsBytes - 4 * M) / M - 16) / 64);
+        const int steps = (int)(((kLdsBytes - 8 * M) / M - 16) / 64);
         if (steps <= 0) return fail(c, SDB_EUNSUPPORTED, "topM too large for the LDS tile");
         da.kc = std::min(da.dk, steps * SE);
         da.nchunk = (da.dk + da.kc - 1) / da.kc;
         QB = da.nchunk == 1 ? 4096 : 256;
         if (const char* e = std::getenv("SDB_TILE_ROWS")) QB = std::max(256, std::atoi(e));
+        QB = std::min(QB, 16 * 8 * 255);  // per-lane byte counters of the single-chunk tiles
     } else {
         int kc = (kLdsBytes - 8 * M) / (4 * M + 128);
         kc = std::min(kc, c->d);
@@ -620,6 +684,8 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
     da.colsum = P_<int>(c->colsum);
     da.eps = eps;
     da.band = band;
+    cosine_cuts((double)eps - band, (double)eps + band, &da.s_hit, &da.s_near);
+    da.magicS = ((1ull << 40) + (unsigned long long)S - 1) / (unsigned long long)S;
 
     HIPCHK(hipEventRecord(c->ev[3], st));
     int nq = 0;
@@ -675,6 +741,7 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
     c->minPts = minPts;
     c->cored = true;
     c->have_nb = false;
+    c->clustered = false;
     return SDB_OK;
 }
 
@@ -725,7 +792,60 @@ int sdb_form_clusters(sdb_ctx* c, int* labels_out, int* n_clusters_out) {
     HIPCHK(hipStreamSynchronize(st));
     c->stats.t_cluster_ms = elapsed(c->ev[0], c->ev[1]);
     c->stats.n_components = comps;
+    c->clustered = true;
     if (n_clusters_out) *n_clusters_out = comps == 0 ? 2 : comps + 1;  // src/sDbscan.cpp:633
+    return SDB_OK;
+}
+
+int sdb_label_noise(sdb_ctx* c, const int* sampled_core, int n_sampled, int* labels_out) {
+    if (!c) return SDB_EINVAL;
+    if (!c->clustered) return fail(c, SDB_ESTATE, "form_clusters first");
+    if (n_sampled < 0 || (n_sampled > 0 && !sampled_core)) return fail(c, SDB_EINVAL, "bad sampled core list");
+    HIPCHK(hipSetDevice(c->device));
+    const sdb_params& q = c->prm;
+    const long long n = c->n;
+    hipStream_t st = c->st;
+    for (int i = 0; i < n_sampled; ++i)
+        if (sampled_core[i] < 0 || sampled_core[i] >= n) return fail(c, SDB_EINVAL, "sampled point %d out of range", sampled_core[i]);
+    if (n_sampled > 0) {
+        const int P = q.n_proj;
+        ENSURE(c->ns_ids, (size_t)n_sampled * 4);
+        ENSURE(c->cproj, (size_t)P * n_sampled * 4);
+        HIPCHK(hipMemcpyAsync(c->ns_ids.p, sampled_core, (size_t)n_sampled * 4, hipMemcpyHostToDevice, st));
+        // projections of the sampled cores (src/sDbscan.cpp:720-770)
+        IndexArgs ia{};
+        ia.X = c->X;
+        ia.Xn = P_<float>(c->Xn);
+        ia.err = P_<int>(c->err_flag);
+        ia.signmask = P_<uint32_t>(c->bits);
+        ia.n = n; ia.ldx = c->ldx; ia.d = c->d; ia.metric = q.metric; ia.ker = q.ker_n_features;
+        ia.kerL = q.ker_intervalSampling;
+        ia.R = (q.metric == SDB_L1 || q.metric == SDB_L2) ? P_<float>(c->R) : nullptr;
+        ia.P = P; ia.K = q.topK;
+        ia.pt_ids = P_<int>(c->ns_ids);
+        ia.n_ids = n_sampled;
+        ia.pt_buf = P_<float>(c->cproj);
+        const int G = (int)std::max<long long>(1, std::min<long long>(2048, ((long long)n_sampled + 3) / 4));
+        LCHK(launch_index(ia, ilog2(q.fhtDim), 3, G, st));
+        // noise points and their best sampled core (src/sDbscan.cpp:775-806)
+        ENSURE(c->noise, (size_t)n * 4);
+        ENSURE(c->qcount, 4);
+        HIPCHK(hipMemsetAsync(c->qcount.p, 0, 4, st));
+        LCHK(launch_noise_list(P_<int>(c->labels), n, P_<int>(c->noise), P_<int>(c->qcount), st));
+        int nn = 0;
+        HIPCHK(hipMemcpyAsync(&nn, c->qcount.p, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (nn > 0) {
+            ENSURE(c->nbest, (size_t)nn * 8);
+            HIPCHK(hipMemsetAsync(c->nbest.p, 0, (size_t)nn * 8, st));
+            LCHK(launch_noise_score(P_<float>(c->cproj), n_sampled, P_<int>(c->topK), q.topK, P_<int>(c->noise), nn,
+                                    (unsigned long long*)c->nbest.p, st));
+            LCHK(launch_noise_assign(P_<int>(c->noise), nn, (const unsigned long long*)c->nbest.p, P_<int>(c->ns_ids),
+                                     P_<int>(c->labels), st));
+        }
+    }
+    if (labels_out) HIPCHK(hipMemcpyAsync(labels_out, c->labels.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return SDB_OK;
 }
 

@@ -24,14 +24,74 @@ __device__ __forceinline__ int slot_list_c(const int* topK, long long a, int s, 
     return (s & 1) ? P + topK[a * 2 * K + K + k] : topK[a * 2 * K + k];
 }
 
-__device__ int point_degree_impl(const ClusterArgs& c, long long a);
-__device__ __forceinline__ int point_degree(const ClusterArgs& c, long long a) { return point_degree_impl(c, a); }
+// The W mask words of entry t (one 16-byte load when W = 4).
+__device__ __forceinline__ void entry_words(const uint32_t* E, size_t t, int W, uint32_t (&m)[kMaxW]) {
+    if (W == 4) {
+        const uint4 v = ((const uint4*)E)[t];
+        m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
+#pragma unroll
+        for (int w = 4; w < kMaxW; ++w) m[w] = 0u;
+    } else {
+#pragma unroll
+        for (int w = 0; w < kMaxW; ++w) m[w] = w < W ? E[t * W + w] : 0u;
+    }
+}
 
-__global__ void k_degree(ClusterArgs c) {
-    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = a < c.n;
+// One workgroup per 256 consecutive points.  |O(a)|: the E words of the
+// block's points are contiguous, so the workgroup reads them as coalesced
+// 16-byte vectors and adds each vector's popcount to its point's LDS counter.
+// |R(a)| and the mutual pairs only exist for points that sit in a list.
+__global__ __launch_bounds__(256) void k_degree(ClusterArgs c) {
+    __shared__ int own[256];
+    const long long a0 = (long long)blockIdx.x * 256;
+    const int npt = (int)min((long long)256, c.n - a0);
+    const int SW = c.S * c.W;
+    own[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t* Eb = c.E + (size_t)a0 * SW;
+    if ((SW & 3) == 0) {
+        const int per = SW >> 2;
+        const int nv = npt * per;
+        for (int v = threadIdx.x; v < nv; v += 256) {
+            const uint4 e = ((const uint4*)Eb)[v];
+            const int pc = __popc(e.x) + __popc(e.y) + __popc(e.z) + __popc(e.w);
+            if (pc) atomicAdd(&own[v / per], pc);
+        }
+    } else {
+        const int nw = npt * SW;
+        for (int v = threadIdx.x; v < nw; v += 256) {
+            const int pc = __popc(Eb[v]);
+            if (pc) atomicAdd(&own[v / SW], pc);
+        }
+    }
+    __syncthreads();
+    const long long a = a0 + threadIdx.x;
+    const bool live = threadIdx.x < npt;
     int deg = 0;
-    if (live) deg = point_degree(c, a);
+    if (live) {
+        deg = own[threadIdx.x];
+        const int h = c.hub_id[a];
+        if (h >= 0) {
+            const uint32_t* Ea = c.E + (size_t)a * SW;
+            int rev = 0, mutual = 0;
+            const int hb = c.hub_ptr[h], he = c.hub_ptr[h + 1];
+            for (int q = hb; q < he; ++q) rev += c.colsum[c.hub_mem[q]];
+            for (int s = 0; s < c.S; ++s) {
+                const int l = slot_list_c(c.topK, a, s, c.K, c.P);
+                for (int w = 0; w < c.W; ++w) {
+                    const uint32_t e = Ea[s * c.W + w];
+                    if (!e) continue;
+                    uint32_t m = 0;
+                    for (int q = hb; q < he; ++q) {
+                        const int L = c.hub_mem[q] / c.M;
+                        m |= c.inv[((size_t)l * c.L2 + L) * c.W + w];
+                    }
+                    mutual += __popc(e & m);
+                }
+            }
+            deg += rev - mutual;
+        }
+    }
     const bool is_core = live && deg >= c.minPts - 1;
     if (live) {
         c.degree[a] = deg;
@@ -39,32 +99,6 @@ __global__ void k_degree(ClusterArgs c) {
     }
     const unsigned long long b = __ballot(is_core);
     if (lane_id() == 0 && b) atomicAdd(c.n_core, (unsigned long long)__popcll(b));
-}
-
-__device__ int point_degree_impl(const ClusterArgs& c, long long a) {
-    int own = 0;
-    const uint32_t* Ea = c.E + (size_t)a * c.S * c.W;
-    for (int q = 0; q < c.S * c.W; ++q) own += __popc(Ea[q]);
-    int rev = 0, mutual = 0;
-    const int h = c.hub_id[a];
-    if (h >= 0) {
-        const int hb = c.hub_ptr[h], he = c.hub_ptr[h + 1];
-        for (int q = hb; q < he; ++q) rev += c.colsum[c.hub_mem[q]];
-        for (int s = 0; s < c.S; ++s) {
-            const int l = slot_list_c(c.topK, a, s, c.K, c.P);
-            for (int w = 0; w < c.W; ++w) {
-                uint32_t e = Ea[s * c.W + w];
-                if (!e) continue;
-                uint32_t m = 0;
-                for (int q = hb; q < he; ++q) {
-                    const int L = c.hub_mem[q] / c.M;
-                    m |= c.inv[((size_t)l * c.L2 + L) * c.W + w];
-                }
-                mutual += __popc(e & m);
-            }
-        }
-    }
-    return own + rev - mutual;
 }
 
 // corehub[l][w]: bit i set when the point at position i of list l is core.
@@ -126,17 +160,22 @@ __global__ void k_uf_adj(ClusterArgs c) {
     const int s = (int)(t % c.S);
     const int l = slot_list_c(c.topK, a, s, c.K, c.P);
     uint32_t m[kMaxW];
+    entry_words(c.E, (size_t)t, c.W, m);
     int first = -1;
-    for (int w = 0; w < c.W; ++w) {
-        m[w] = c.E[(size_t)t * c.W + w] & c.corehub[(size_t)l * c.W + w];
+#pragma unroll
+    for (int w = 0; w < kMaxW; ++w) {
+        if (w < c.W) m[w] &= c.corehub[(size_t)l * c.W + w];
         if (first < 0 && m[w]) first = w * 32 + __ffs(m[w]) - 1;
     }
     if (first < 0) return;
     // only the hits after the first one add edges
-    m[first >> 5] &= ~(1u << (first & 31));
+#pragma unroll
+    for (int w = 0; w < kMaxW; ++w)
+        if (w == (first >> 5)) m[w] &= ~(1u << (first & 31));
     uint32_t* ad = c.adj + ((size_t)l * c.M + first) * c.W;
-    for (int w = 0; w < c.W; ++w)
-        if (m[w] && (load_relaxed((const int*)&ad[w]) & m[w]) != m[w]) atomicOr(&ad[w], m[w]);
+#pragma unroll
+    for (int w = 0; w < kMaxW; ++w)
+        if (w < c.W && m[w] && (load_relaxed((const int*)&ad[w]) & m[w]) != m[w]) atomicOr(&ad[w], m[w]);
 }
 
 __global__ void k_uf_points(ClusterArgs c) {
@@ -146,9 +185,13 @@ __global__ void k_uf_points(ClusterArgs c) {
     for (int s = 0; s < c.S; ++s) {
         const int l = slot_list_c(c.topK, a, s, c.K, c.P);
         const size_t t = (size_t)a * c.S + s;
+        uint32_t ew[kMaxW];
+        entry_words(c.E, t, c.W, ew);
         int first = -1;
-        for (int w = 0; w < c.W && first < 0; ++w) {
-            const uint32_t m = c.E[t * c.W + w] & c.corehub[(size_t)l * c.W + w];
+#pragma unroll
+        for (int w = 0; w < kMaxW; ++w) {
+            if (first >= 0 || w >= c.W || !ew[w]) continue;
+            const uint32_t m = ew[w] & c.corehub[(size_t)l * c.W + w];
             if (m) first = w * 32 + __ffs(m) - 1;
         }
         if (first < 0) continue;
@@ -197,8 +240,10 @@ __global__ void k_border(ClusterArgs c) {
     const int l = slot_list_c(c.topK, a, s, c.K, c.P);
     const bool ca = c.core[a] != 0;
     int best = 0x7fffffff;
+    uint32_t ew[kMaxW];
+    entry_words(c.E, (size_t)t, c.W, ew);
     for (int w = 0; w < c.W; ++w) {
-        const uint32_t e = c.E[(size_t)t * c.W + w];
+        const uint32_t e = ew[w];
         if (!e) continue;
         const uint32_t ch = c.corehub[(size_t)l * c.W + w];
         uint32_t m = ca ? (e & ~ch) : (e & ch);

@@ -65,4 +65,32 @@ __device__ __forceinline__ uint32_t f32_to_bf16_rne(float f) {
     return r >> 16;
 }
 
+// Cross-lane exchange with lane ^ S without the LDS crossbar: DPP quad
+// permutes for S = 1, 2; two DPP row permutes (half-mirror / mirror, then a
+// quad or half-mirror permute) for S = 4, 8; the gfx950 permlane swaps for
+// S = 16, 32.
+template <int S>
+__device__ __forceinline__ uint32_t xchg(uint32_t v, int lane) {
+    const int iv = (int)v;
+    if constexpr (S == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, iv, 0xB1, 0xF, 0xF, false);
+    if constexpr (S == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, iv, 0x4E, 0xF, 0xF, false);
+    if constexpr (S == 4) {
+        const int h = __builtin_amdgcn_update_dpp(0, iv, 0x141, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, h, 0x1B, 0xF, 0xF, false);
+    }
+    if constexpr (S == 8) {
+        const int m = __builtin_amdgcn_update_dpp(0, iv, 0x140, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, m, 0x141, 0xF, 0xF, false);
+    }
+    if constexpr (S == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16) ? p[0] : p[1];
+    }
+    if constexpr (S == 32) {
+        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (lane & 32) ? p[0] : p[1];
+    }
+    return v;
+}
+
 }  // namespace sdb

@@ -29,7 +29,7 @@ __device__ __forceinline__ unsigned long long queue_entry(uint32_t ent, int l, i
 // 16 bytes of query row (l & 15) and of candidate column (l & 15) at k-group
 // (l >> 4) per step; A and B use the same k assignment, so the sum over k is
 // the dot product whatever order the unit uses internally.
-template <bool BF, int QT, int NT, int WAVES, int PF, bool WHOLE>
+template <bool BF, int QT, int NT, int WAVES, int PF, int BATCH>
 __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     constexpr int SZ = BF ? 2 : 4;
     constexpr int SE = 64 / SZ;  // elements per 64-byte k-step
@@ -47,7 +47,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     const char* Xc = (const char*)a.X;
     const double lo = (double)a.eps - a.band, hi = (double)a.eps + a.band;
 
-    for (int i = tid; i < M; i += WAVES * 64) col_lds[i] = 0;
+    int* pts_lds = col_lds + M;
+    if (tid < M) {  // M <= 256 < WAVES * 64
+        col_lds[tid] = 0;
+        pts_lds[tid] = Lm[tid];
+    }
     int colacc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) colacc[t] = 0;
@@ -75,12 +79,23 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
             const int k0 = c * a.kc;
             const int kn = min(a.kc, a.dk - k0);
             if (a.nchunk > 1 || set == 0) {
+                // candidate rows of this k-chunk -> LDS, BATCH loads in flight
                 __syncthreads();
-                const int vpr = kn * SZ / 16;
-                for (int v = tid; v < M * vpr; v += WAVES * 64) {
-                    int row = v / vpr, cv = v - row * vpr;
-                    const uint4* src = (const uint4*)(Xc + ((size_t)Lm[row] * a.ldx + k0) * SZ) + cv;
-                    *(uint4*)(smem + row * RSB + (size_t)cv * 16) = *src;
+                const int vpr = kn * SZ / 16, total = M * vpr;
+                for (int v0 = tid; v0 < total; v0 += WAVES * 64 * BATCH) {
+                    uint4 xs[BATCH];
+#pragma unroll
+                    for (int u = 0; u < BATCH; ++u) {
+                        const int v = min(v0 + u * WAVES * 64, total - 1);  // clamped: loads stay unconditional
+                        const int row = v / vpr, cv = v - row * vpr;
+                        xs[u] = ((const uint4*)(Xc + ((size_t)pts_lds[row] * a.ldx + k0) * SZ))[cv];
+                    }
+#pragma unroll
+                    for (int u = 0; u < BATCH; ++u) {
+                        const int v = v0 + u * WAVES * 64;
+                        const int row = v / vpr, cv = v - row * vpr;
+                        if (v < total) *(uint4*)(smem + row * RSB + (size_t)cv * 16) = xs[u];
+                    }
                 }
                 __syncthreads();
             }
@@ -119,14 +134,6 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
                     }
                 }
             };
-            if constexpr (WHOLE) {
-                // whole query rows are read in one burst (nsteps <= PF): every
-                // step's fragment was issued above
-#pragma unroll
-                for (int i = 0; i < PF; ++i)
-                    if (i < nsteps) mfma_step(i, af[i]);
-                continue;
-            }
             const int ngroups = nsteps / PF;
             const int rem = nsteps - ngroups * PF;
             for (int gi = 0; gi < ngroups; ++gi) {
@@ -200,29 +207,276 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     }
 }
 
-template <bool BF, int NT, int QT, int PF, bool WHOLE>
-static int launch_mfma_v(const DistArgs& a, int ntiles, hipStream_t st) {
+// ---------------------------------------------------------------------------
+// Pipelined single-chunk BF16 tiles (cosine, every dim of a row in one LDS
+// tile).  The list's candidate rows are the MFMA A operand (LDS), the query
+// rows the B operand, so lane (r16, g) accumulates the dots of query r16 of
+// its 16-row block with candidates t*16 + 4g + i (i < 4) and needs no
+// cross-lane traffic for its query's masks.  Query fragments stream through a
+// ring of PF k-steps that runs across blocks: the fragment of step st of the
+// next block is loaded as soon as step st of the current block has been
+// consumed, so the next block's rows are in flight during the MFMAs and the
+// epilogue of the current one.
+//   hits: s > s_hit; band: s_near <= s <= s_hit (queued for the recheck);
+//   E words: per-lane nibbles OR-reduced across the 4 lanes of a query;
+//   colsum: per-lane byte counters (<= 255 blocks per lane and tile).
+__device__ __forceinline__ uint32_t spread4(uint32_t n) {
+    // bit i of n -> byte i
+    return (n * 0x00204081u) & 0x01010101u;
+}
+
+template <int NT, int WAVES, int NS, bool EXACT, int PADB>
+__global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
+    // NS k-steps per row; EXACT: the rows have exactly NS steps, otherwise
+    // fewer, with the LDS rows zero-padded to NS steps and the query loads of
+    // the missing steps clamped to the last step (their products are zero)
+    constexpr int SZ = 2;
+    constexpr int EW = (NT + 1) / 2;
+    constexpr int RSB = NS * 64 + PADB;  // row stride in LDS, padded against bank conflicts
+    static_assert(NT <= 8, "candidate bits per lane exceed 32");
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int M = a.M;
+    int* col_lds = (int*)(smem + (size_t)M * RSB);
+    const int tid = threadIdx.x;
+    const int lane = lane_id(), w = tid >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int l = a.tile_list[blockIdx.x];
+    const int beg = a.tile_beg[blockIdx.x], end = a.tile_end[blockIdx.x];
+    const int* Lm = a.lists + (size_t)l * M;
+    const char* Xc = (const char*)a.X;
+    const int nsteps = EXACT ? NS : a.dk / 32;
+    const size_t rowb = (size_t)a.ldx * SZ;
+
+    const int nblk = (end - beg + 15) >> 4;
+    auto entry_of = [&](int blk, uint32_t& e, bool& v) {
+        const int rr = beg + blk * 16 + r16;
+        v = rr < end;
+        e = a.grp[v ? rr : beg];
+    };
+    auto row_of = [&](uint32_t e) {
+        const size_t p = (size_t)(((unsigned long long)e * a.magicS) >> 40);
+        return Xc + p * rowb + g * 16;
+    };
+    auto step_off = [&](int st) { return (size_t)(EXACT ? st : min(st, nsteps - 1)) * 64; };
+    // the list's point ids are requested first, then the first block's query
+    // rows, so that both latencies overlap the staging of the candidate rows
+    int* pts_lds = col_lds + M;
+    int pid = 0;
+    if (tid < M) pid = Lm[tid];
+    // (waves without a block load a valid row they never use, which keeps the
+    // loads unconditional and the compiler's wait counts exact)
+    const bool active = w < nblk;
+    int b = w;
+    uint32_t ent, entn;
+    bool valid, validn;
+    entry_of(min(b, nblk - 1), ent, valid);
+    entry_of(min(b + WAVES, nblk - 1), entn, validn);
+    uint4 af[NS];
+    {
+        const char* rp = row_of(ent);
+#pragma unroll
+        for (int i = 0; i < NS; ++i) af[i] = *(const uint4*)(rp + step_off(i));
+    }
+    if (tid < M) {  // M <= 256 < WAVES * 64
+        col_lds[tid] = 0;
+        pts_lds[tid] = pid;
+    }
+
+    // candidate rows -> LDS with BATCH 16-byte loads in flight per thread (the
+    // list's point ids went through LDS first, so no row load waits on the
+    // load of its id)
+    __syncthreads();
+    {
+        constexpr int BATCH = 10;
+        const int vpr = nsteps * 4, total = M * NS * 4;
+        for (int v0 = tid; v0 < total; v0 += WAVES * 64 * BATCH) {
+            uint4 xs[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                // clamped indices keep the loads unconditional; the zero fill
+                // of the padding steps is a select
+                const int v = min(v0 + u * WAVES * 64, total - 1);
+                const int row = v / (NS * 4), cv = v - row * (NS * 4);
+                const uint4 x = ((const uint4*)(Xc + (size_t)pts_lds[row] * rowb))[min(cv, vpr - 1)];
+                xs[u] = cv < vpr ? x : uint4{0u, 0u, 0u, 0u};
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int v = v0 + u * WAVES * 64;
+                const int row = v / (NS * 4), cv = v - row * (NS * 4);
+                if (v < total) *(uint4*)(smem + row * RSB + cv * 16) = xs[u];
+            }
+        }
+    }
+    __syncthreads();
+
+    uint32_t colacc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) colacc[t] = 0u;
+    int crow[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) crow[t] = min(t * 16 + r16, M - 1) * RSB + g * 16;
+
+    if (active) {
+        for (;;) {
+            const bool has_next = b + WAVES < nblk;
+            uint32_t Fw[EW];
+#pragma unroll
+            for (int x = 0; x < EW; ++x) Fw[x] = (valid && x < a.W) ? a.F[(size_t)ent * a.W + x] : 0u;
+            uint32_t ent2;
+            bool valid2;
+            entry_of(min(b + 2 * WAVES, nblk - 1), ent2, valid2);
+            const char* np = row_of(entn);
+
+            // the candidate fragments are the same for every block; an empty
+            // asm on their offsets keeps the compiler from hoisting all NS x NT
+            // LDS reads out of the loop (they would not fit in registers)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) asm volatile("" : "+v"(crow[t]));
+            f32x4_t acc[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+            // candidate fragments are read one step ahead; a scheduling
+            // barrier after every step keeps each ring reload next to the step
+            // that consumed its slot (left alone, the scheduler sinks all NS
+            // reloads below the MFMAs, which serialises HBM latency and math)
+            uint4 cf[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cf[t] = *(const uint4*)(smem + crow[t]);
+#pragma unroll
+            for (int st = 0; st < NS; ++st) {
+                uint4 cn[NT];
+                if (st + 1 < NS) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) cn[t] = *(const uint4*)(smem + crow[t] + (st + 1) * 64);
+                }
+                const bf16x8_t qv = __builtin_bit_cast(bf16x8_t, af[st]);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, cf[t]), qv,
+                                                                     acc[t], 0, 0, 0);
+                af[st] = *(const uint4*)(np + step_off(st));
+                if (st + 1 < NS) __builtin_amdgcn_sched_group_barrier(0x100, NT, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (st + 1 < NS) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) cf[t] = cn[t];
+                }
+            }
+
+            // epilogue: lane (r16, g) holds s(query r16, candidate t*16 + 4g + i) in acc[t][i]
+            uint32_t hw[EW];
+#pragma unroll
+            for (int x = 0; x < EW; ++x) hw[x] = 0u;
+            uint32_t unc = 0u;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                uint32_t hr = 0u, nr = 0u;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    hr |= (acc[t][i] > a.s_hit ? 1u : 0u) << i;
+                    nr |= (acc[t][i] >= a.s_near ? 1u : 0u) << i;
+                }
+                const int base = (t & 1) * 16 + 4 * g;
+                const uint32_t f = (Fw[t >> 1] >> base) & 0xFu;
+                const uint32_t hn = hr & f;
+                unc |= ((nr ^ hr) & f) << (4 * t);
+                hw[t >> 1] |= hn << base;
+                colacc[t] += spread4(hn);
+            }
+#pragma unroll
+            for (int x = 0; x < EW; ++x) {
+                hw[x] |= xchg<16>(hw[x], lane);
+                hw[x] |= xchg<32>(hw[x], lane);
+            }
+            uint32_t mine = hw[0];
+#pragma unroll
+            for (int x = 1; x < EW; ++x) mine = g == x ? hw[x] : mine;
+            if (valid && g < a.W) a.E[(size_t)ent * a.W + g] = mine;
+            if (__any(unc != 0u)) {
+                while (unc) {
+                    const int bp = __ffs(unc) - 1;
+                    unc &= unc - 1;
+                    const int col = (bp >> 2) * 16 + 4 * g + (bp & 3);
+                    const int pos = atomicAdd(a.qcount, 1);
+                    if (pos < a.qcap) a.queue[pos] = queue_entry(ent, l, col);
+                }
+            }
+            if (!has_next) break;
+            ent = entn; valid = validn;
+            entn = ent2; validn = valid2;
+            b += WAVES;
+        }
+    }
+    // column hit counts: bytes i of colacc[t] count candidate t*16 + 4g + i;
+    // sum the 16 query lanes in 16-bit fields
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        uint32_t lo = colacc[t] & 0x00FF00FFu, hi = (colacc[t] >> 8) & 0x00FF00FFu;
+        lo += xchg<1>(lo, lane); hi += xchg<1>(hi, lane);
+        lo += xchg<2>(lo, lane); hi += xchg<2>(hi, lane);
+        lo += xchg<4>(lo, lane); hi += xchg<4>(hi, lane);
+        lo += xchg<8>(lo, lane); hi += xchg<8>(hi, lane);
+        if (r16 == 0) {
+            const int c0 = t * 16 + 4 * g;
+            const int v[4] = {(int)(lo & 0xFFFFu), (int)(hi & 0xFFFFu), (int)(lo >> 16), (int)(hi >> 16)};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (c0 + i < M && v[i]) atomicAdd(&col_lds[c0 + i], v[i]);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < M; i += WAVES * 64) {
+        const int v = col_lds[i];
+        if (v) atomicAdd(&a.colsum[(size_t)l * M + i], v);
+    }
+}
+
+template <int NT, int NS, bool EXACT, int PADB>
+static int launch_pipe_p(const DistArgs& a, int ntiles, hipStream_t st) {
     constexpr int WAVES = 8;
-    const size_t sm = (size_t)a.M * ((size_t)a.kc * (BF ? 2 : 4) + 16) + (size_t)a.M * 4;
+    const size_t sm = (size_t)a.M * (NS * 64 + PADB) + (size_t)a.M * 8;
     if (sm > (size_t)kLdsBytes) return -2;
-    hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES, PF, WHOLE>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_pipe<NT, WAVES, NS, EXACT, PADB>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL((k_dist_mfma<BF, QT, NT, WAVES, PF, WHOLE>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
+    hipLaunchKernelGGL((k_dist_pipe<NT, WAVES, NS, EXACT, PADB>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
+    return (int)hipGetLastError();
+}
+
+template <int NT, int NS, bool EXACT>
+static int launch_pipe(const DistArgs& a, int ntiles, hipStream_t st) {
+    return launch_pipe_p<NT, NS, EXACT, 16>(a, ntiles, st);
+}
+
+// (staging one 16-byte row piece per thread and step measured faster for these
+// tiles than deeper batches: 271 vs 310 ms for the FP32 8M x 784 tiles)
+template <bool BF, int NT, int QT, int PF, int BATCH = 1>
+static int launch_mfma_v(const DistArgs& a, int ntiles, hipStream_t st) {
+    constexpr int WAVES = 8;
+    const size_t sm = (size_t)a.M * ((size_t)a.kc * (BF ? 2 : 4) + 16) + (size_t)a.M * 8;
+    if (sm > (size_t)kLdsBytes) return -2;
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES, PF, BATCH>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL((k_dist_mfma<BF, QT, NT, WAVES, PF, BATCH>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
     return (int)hipGetLastError();
 }
 
 template <bool BF, int NT>
 static int launch_mfma_nt(const DistArgs& a, int ntiles, hipStream_t st) {
     if constexpr (BF) {
-        int v = 1;
-        if (const char* e = std::getenv("SDB_DIST_VARIANT")) v = std::atoi(e);
+        // rows that fit one LDS tile: the pipelined kernel; longer rows:
+        // k-chunked tiles
         const int nsteps = a.kc / 32;
-        if (v == 1 && nsteps <= 25) return launch_mfma_v<BF, NT, 1, 25, true>(a, ntiles, st);
-        if (v == 3 && nsteps <= 26) return launch_mfma_v<BF, NT, 2, 13, false>(a, ntiles, st);
-        return launch_mfma_v<BF, NT, 2, 5, false>(a, ntiles, st);
+        if (a.nchunk == 1 && nsteps == 25) return launch_pipe<NT, 25, true>(a, ntiles, st);
+        if (a.nchunk == 1 && nsteps <= 8) return launch_pipe<NT, 8, false>(a, ntiles, st);
+        if (a.nchunk == 1 && nsteps <= 25) return launch_pipe<NT, 25, false>(a, ntiles, st);
+        return launch_mfma_v<BF, NT, 2, 5>(a, ntiles, st);
     } else {
-        return launch_mfma_v<BF, NT, 2, 1, false>(a, ntiles, st);
+        return launch_mfma_v<BF, NT, 2, 1>(a, ntiles, st);
     }
 }
 

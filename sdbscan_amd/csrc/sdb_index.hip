@@ -4,6 +4,8 @@
 // Reference: rp_parIndex (src/sDbscan.cpp:19-209), transformData
 // (src/Utilities.h:289-335), embeddings (src/sDbscan.cpp:53-68,
 // src/Utilities.h:132-208), FFHT (src/fht_avx.c:1013-1256).
+#include <algorithm>
+
 #include "sdb_kernels.hpp"
 
 namespace sdb {
@@ -53,35 +55,6 @@ __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, 
     }
 }
 
-// ---------------------------------------------------------------------------
-// Cross-lane exchange with lane ^ S without the LDS crossbar: DPP quad
-// permutes for S = 1, 2; two DPP row permutes (half-mirror / mirror, then a
-// quad or half-mirror permute) for S = 4, 8; the gfx950 permlane swaps for
-// S = 16, 32.
-template <int S>
-__device__ __forceinline__ uint32_t xchg(uint32_t v, int lane) {
-    const int iv = (int)v;
-    if constexpr (S == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, iv, 0xB1, 0xF, 0xF, false);
-    if constexpr (S == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, iv, 0x4E, 0xF, 0xF, false);
-    if constexpr (S == 4) {
-        const int h = __builtin_amdgcn_update_dpp(0, iv, 0x141, 0xF, 0xF, false);
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, h, 0x1B, 0xF, 0xF, false);
-    }
-    if constexpr (S == 8) {
-        const int m = __builtin_amdgcn_update_dpp(0, iv, 0x140, 0xF, 0xF, false);
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, m, 0x141, 0xF, 0xF, false);
-    }
-    if constexpr (S == 16) {
-        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-        return (lane & 16) ? p[0] : p[1];
-    }
-    if constexpr (S == 32) {
-        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-        return (lane & 32) ? p[0] : p[1];
-    }
-    return v;
-}
-
 // One Walsh-Hadamard level across lanes (stride S < 64): the lower element
 // becomes a + b, the upper a - b, computed as fma(+-1, mine, partner) with an
 // exact product, i.e. one rounding of partner +- mine.
@@ -92,6 +65,37 @@ __device__ __forceinline__ void fwht_cross(float (&v)[VPL], int lane) {
     for (int j = 0; j < VPL; ++j) {
         const float o = __uint_as_float(xchg<S>(__float_as_uint(v[j]), lane));
         v[j] = fmaf(sg, v[j], o);
+    }
+}
+
+// Strides 16 and 32 on register pairs: a permlane swap of (x, y) leaves each
+// lane holding one butterfly pair (of x in the lower row / half, of y in the
+// upper one); the sums and differences are swapped back into x and y.  Two
+// swaps, one add and one subtract per two registers.
+template <int S, int VPL>
+__device__ __forceinline__ void fwht_cross_pairs(float (&v)[VPL]) {
+#pragma unroll
+    for (int j = 0; j + 1 < VPL; j += 2) {
+        float x, y;
+        if constexpr (S == 16) {
+            const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[j]), __float_as_uint(v[j + 1]), false, false);
+            x = __uint_as_float(p[0]);
+            y = __uint_as_float(p[1]);
+        } else {
+            const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[j]), __float_as_uint(v[j + 1]), false, false);
+            x = __uint_as_float(p[0]);
+            y = __uint_as_float(p[1]);
+        }
+        const float sm = x + y, df = x - y;
+        if constexpr (S == 16) {
+            const auto q = __builtin_amdgcn_permlane16_swap(__float_as_uint(sm), __float_as_uint(df), false, false);
+            v[j] = __uint_as_float(q[0]);
+            v[j + 1] = __uint_as_float(q[1]);
+        } else {
+            const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(sm), __float_as_uint(df), false, false);
+            v[j] = __uint_as_float(q[0]);
+            v[j + 1] = __uint_as_float(q[1]);
+        }
     }
 }
 
@@ -119,8 +123,13 @@ __device__ __forceinline__ void ffht_wave(float (&v)[(1 << LOGD) / 64], int lane
     fwht_cross<2, VPL>(v, lane);
     fwht_cross<4, VPL>(v, lane);
     fwht_cross<8, VPL>(v, lane);
-    fwht_cross<16, VPL>(v, lane);
-    fwht_cross<32, VPL>(v, lane);
+    if constexpr (VPL % 2 == 0) {
+        fwht_cross_pairs<16, VPL>(v);
+        fwht_cross_pairs<32, VPL>(v);
+    } else {
+        fwht_cross<16, VPL>(v, lane);
+        fwht_cross<32, VPL>(v, lane);
+    }
 #pragma unroll
     for (int js = 1; js < VPL; js <<= 1) fwht_inreg<VPL>(v, js);
 }
@@ -237,66 +246,14 @@ __device__ __forceinline__ int lanes_below(unsigned long long b) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
 }
 
-// K largest keys (value desc, index asc) of the first P elements, K <= 64.
-// T0 = K-th largest lane maximum bounds the K-th largest value from below, so
-// every element with value >= T0 (ties with the K-th value included) is a
-// candidate.  Up to 64 candidates are compacted with ballots and sorted with
-// one 64-lane bitonic network; more candidates take the general path.
-// Returns true when the (K+1)-th largest value equals the K-th.
+// Fallback selection of the K largest keys (key desc, index asc) among the
+// first P elements, used when more than 64 elements pass the candidate bound.
+// Winners are written to out[0..K); returns true when the (K+1)-th largest
+// key equals the K-th (a tie across the selection boundary).
 template <int VPL>
-__device__ __forceinline__ bool wave_topk_general(const float (&v)[VPL], bool neg, int P, int K, int lane,
-                                               int* out);
-
-template <int VPL>
-__device__ __forceinline__ bool wave_topk(const float (&v)[VPL], bool neg, int P, int K, int lane,
-                                          uint64_t* lds, int* out) {
-    uint32_t key[VPL];
-#pragma unroll
-    for (int j = 0; j < VPL; ++j) key[j] = ord_key(neg ? -v[j] : v[j]);
-    uint32_t lm = 0;
-#pragma unroll
-    for (int j = 0; j < VPL; ++j)
-        if (lane + 64 * j < P) lm = key[j] > lm ? key[j] : lm;
-    bitonic_sort_u32<64>(lm, lane);
-    const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)lm, K - 1);
-    uint32_t* sh = (uint32_t*)lds;
-    int total = 0;
-#pragma unroll
-    for (int j = 0; j < VPL; ++j) {
-        const int e = lane + 64 * j;
-        const bool c = e < P && key[j] >= t0;
-        const unsigned long long b = __ballot(c);
-        const int pos = total + lanes_below(b);
-        if (c && pos < 64) { sh[pos] = key[j]; sh[64 + pos] = 0xffffffffu - (uint32_t)e; }
-        total += __popcll(b);
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (total > 64) return wave_topk_general<VPL>(v, neg, P, K, lane, out);
-    uint32_t h = lane < total ? sh[lane] : 0u;
-    uint32_t lo = lane < total ? sh[64 + lane] : 0u;
-    __builtin_amdgcn_wave_barrier();
-    bitonic_sort_u64<64>(h, lo, lane);
-    if (lane < K) out[lane] = (int)(0xffffffffu - lo);
-    if (total <= K) return false;
-    const uint32_t vk = (uint32_t)__builtin_amdgcn_readlane((int)h, K - 1);
-    const uint32_t vn = (uint32_t)__builtin_amdgcn_readlane((int)h, K);
-    return vk == vn;
-}
-
-// Wave-level selection of the K largest keys (value desc, index asc) among
-// the first P elements.  T0 = K-th largest lane maximum bounds the K-th
-// largest element from below, so only elements whose value is >= value(T0)
-// are ranked (ties with the K-th value included).  Winners are written to
-// out[0..K); returns true when the (K+1)-th largest value equals the K-th
-// (a tie across the selection boundary).
-template <int VPL>
-__device__ __forceinline__ bool wave_topk_general(const float (&v)[VPL], bool neg, int P, int K, int lane,
-                                               int* out) {
+__device__ __forceinline__ bool wave_topk_general(const uint32_t (&key)[VPL], int P, int K, int lane, int* out) {
     // K + 1 rounds of a wave-wide argmax over the register keys (value desc,
     // index asc); the winner of a round is masked out in its lane.
-    uint32_t key[VPL];
-#pragma unroll
-    for (int j = 0; j < VPL; ++j) key[j] = ord_key(neg ? -v[j] : v[j]);
     uint64_t removed = 0;
     uint64_t prev = 0;
     bool tie = false;
@@ -324,6 +281,99 @@ __device__ __forceinline__ bool wave_topk_general(const float (&v)[VPL], bool ne
     return tie;
 }
 
+// Both ends of the per-point selection (src/sDbscan.cpp:97-143) in one pass:
+// the K largest (close) and the K smallest (far) projection values, each as
+// (value desc / asc, index asc).  Far keys are the bitwise complements of the
+// close keys (order reversed; +0 and -0 stay equal); elements at or beyond P
+// get key 0 on both sides and are never selected.
+//   1. T <= the K-th largest lane maximum (kth_bounds): at least K elements
+//      are >= T, so every winner is a candidate.
+//   2. Candidates (key >= T) are compacted into LDS with ballots.
+//   3. Each candidate lane counts the candidates ranked before it (a readlane
+//      loop over at most 64 entries); rank r < K goes to out[r].
+// More than 64 candidates on a side take the K + 1 round wave argmax.
+// Returns the tie flags: the (K+1)-th key equals the K-th.
+// Lower bounds for the K-th largest lane maximum of the close and the far
+// keys: a ballot bisection over the top NB bits of the key, keeping the
+// largest prefix T with at least K lane maxima >= T.  Lanes whose maximum is
+// 0 (no valid element) never count.
+template <int NB>
+__device__ __forceinline__ void kth_bounds(uint32_t mc, uint32_t mf, int K, uint32_t& tc, uint32_t& tf) {
+    uint32_t pc = 0u, pf = 0u;
+#pragma unroll
+    for (int b = 31; b >= 32 - NB; --b) {
+        const uint32_t xc = pc | (1u << b), xf = pf | (1u << b);
+        if (__popcll(__ballot(mc >= xc)) >= K) pc = xc;
+        if (__popcll(__ballot(mf >= xf)) >= K) pf = xf;
+    }
+    tc = pc;
+    tf = pf;
+}
+
+__device__ __forceinline__ void rank_select(uint32_t* sh, int total, int K, int lane, int* out, bool& tie) {
+    // sh[2 i] = key, sh[2 i + 1] = ~index for i < total (total <= 64)
+    uint32_t h = 0u, lo = 0u;
+    if (lane < total) {
+        const uint2 e = ((const uint2*)sh)[lane];
+        h = e.x;
+        lo = e.y;
+    }
+    const uint64_t mine = ((uint64_t)h << 32) | lo;
+    int rank = 0;
+    for (int i = 0; i < total; ++i) {
+        const uint64_t o = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)h, i) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)lo, i);
+        rank += o > mine ? 1 : 0;
+    }
+    if (lane < total && rank < K) out[rank] = (int)~lo;
+    tie = false;
+    if (total > K) {
+        const unsigned long long bk = __ballot(lane < total && rank == K - 1);
+        const unsigned long long bn = __ballot(lane < total && rank == K);
+        const uint32_t vk = (uint32_t)__builtin_amdgcn_readlane((int)h, __builtin_ctzll(bk));
+        const uint32_t vn = (uint32_t)__builtin_amdgcn_readlane((int)h, __builtin_ctzll(bn));
+        tie = vk == vn;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int VPL>
+__device__ __forceinline__ void wave_topk_both(const uint32_t (&kc)[VPL], int P, int K, int lane, uint32_t* lds,
+                                               int* out_close, int* out_far, bool& tie_c, bool& tie_f) {
+    // kc: close keys, 0 for elements at or beyond P; far key = ~kc (0 stays 0)
+    uint32_t kf[VPL];
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) kf[j] = kc[j] ? ~kc[j] : 0u;
+    uint32_t mc = 0u, mf = 0u;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) {
+        mc = kc[j] > mc ? kc[j] : mc;
+        mf = kf[j] > mf ? kf[j] : mf;
+    }
+    uint32_t tc, tf;
+    kth_bounds<16>(mc, mf, K, tc, tf);
+    uint32_t* shc = lds;
+    uint32_t* shf = lds + 128;
+    int nc = 0, nf = 0;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) {
+        const uint32_t e = (uint32_t)(lane + 64 * j);
+        const bool cc = kc[j] >= tc && kc[j] != 0u;
+        const bool cf = kf[j] >= tf && kf[j] != 0u;
+        const unsigned long long bc = __ballot(cc), bf = __ballot(cf);
+        const int pc = nc + lanes_below(bc), pf = nf + lanes_below(bf);
+        if (cc && pc < 64) ((uint2*)shc)[pc] = uint2{kc[j], ~e};
+        if (cf && pf < 64) ((uint2*)shf)[pf] = uint2{kf[j], ~e};
+        nc += __popcll(bc);
+        nf += __popcll(bf);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (nc <= 64) rank_select(shc, nc, K, lane, out_close, tie_c);
+    else tie_c = wave_topk_general<VPL>(kc, P, K, lane, out_close);
+    if (nf <= 64) rank_select(shf, nf, K, lane, out_far, tie_f);
+    else tie_f = wave_topk_general<VPL>(kf, P, K, lane, out_far);
+}
+
 // ---------------------------------------------------------------------------
 // Index passes over the points, one wavefront per point:
 //   kSample: every sample_stride-th point; HD3 projections and per-chunk row
@@ -334,9 +384,11 @@ __device__ __forceinline__ bool wave_topk_general(const float (&v)[VPL], bool ne
 //            per-row thresholds appended to the top-M candidate lists.
 //   kRows:   recompute and store whole projection rows listed in row_slot
 //            (host replay of tied top-M rows).
+//   kPoints: projections of the points listed in pt_ids (P x n_ids, column
+//            per listed point) for the noise labelling.
 // Grid: G workgroups x 4 waves; wave w of group g takes the points
 // q = (4 g + w) + t * 4 G, so chunk g is an interleaved subset.
-enum { kSample = 0, kFull = 1, kRows = 2 };
+enum { kSample = 0, kFull = 1, kRows = 2, kPoints = 3 };
 
 __device__ __forceinline__ double wave_sum_f64(double v, int lane) {
     uint32_t lo = (uint32_t)__double_as_longlong(v);
@@ -355,7 +407,7 @@ __device__ __forceinline__ double wave_sum_f64(double v, int lane) {
 }
 
 template <int LOGD, int MODE, bool COS>
-__global__ __launch_bounds__(256) void k_index(IndexArgs a) {
+__global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
     constexpr int D = 1 << LOGD;
     constexpr int VPL = D / 64;
     constexpr int NW = (VPL + 31) / 32;
@@ -366,10 +418,10 @@ __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
     const int P = a.P;
 
     extern __shared__ __align__(16) unsigned char smem[];
-    // LDS: [top-K scratch 4 x 512 B (kFull)] [2P keys] [embedding 4 x D x 4 B (!COS)]
-    const size_t off_red = MODE == kFull ? (size_t)4 * 512 : 0;
+    // LDS: [top-K scratch 4 x 1 KB (kFull)] [2P keys] [embedding 4 x D x 4 B (!COS)]
+    const size_t off_red = MODE == kFull ? (size_t)4 * 1024 : 0;
     const size_t off_emb = off_red + (size_t)2 * P * 4;
-    uint64_t* tks = (uint64_t*)smem + (size_t)wib * 64;
+    uint64_t* tks = (uint64_t*)smem + (size_t)wib * 128;
     uint32_t* red = (uint32_t*)(smem + off_red);
     float* emb = (float*)(smem + off_emb) + (size_t)wib * D;
 
@@ -394,10 +446,11 @@ __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
         for (int j = 0; j < VPL; ++j) { cmax[j] = 0u; cmin[j] = 0xffffffffu; }
     }
     const long long stride = MODE == kSample ? a.sample_stride : 1;
-    const long long npts = (a.n + stride - 1) / stride;
+    const long long npts = MODE == kPoints ? a.n_ids : (a.n + stride - 1) / stride;
+    constexpr bool RAW = COS && (MODE == kSample || MODE == kFull);  // reads untransformed rows
     // raw rows of the next point are loaded one iteration ahead (cosine)
-    float xpre[(COS && MODE != kRows) ? VPL : 1];
-    if constexpr (COS && MODE != kRows) {
+    float xpre[RAW ? VPL : 1];
+    if constexpr (RAW) {
         if (gw < npts) {
             const float* x = a.X + gw * stride * a.d;
 #pragma unroll
@@ -405,10 +458,10 @@ __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
         }
     }
     for (long long q = gw; q < npts; q += nw) {
-        const long long p = q * stride;
+        const long long p = MODE == kPoints ? (long long)a.pt_ids[q] : q * stride;
         float v[VPL];
         if constexpr (COS) {
-            if (MODE == kRows) {
+            if constexpr (!RAW) {
                 const float* x = a.Xn + p * a.ldx;
 #pragma unroll
                 for (int j = 0; j < VPL; ++j) { const int e = lane + 64 * j; v[j] = e < a.d ? x[e] : 0.0f; }
@@ -432,23 +485,26 @@ __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
                 if (!(nrm > 0.0f)) {
                     if (lane == 0) atomicOr(a.err, 2);
                 }
+                // x / nrm, correctly rounded, without a division per element:
+                // the FP64 product with the FP64 reciprocal is within 2^-52
+                // (relative) of the exact quotient, while an FP32 quotient of two
+                // FP32 numbers is never closer than 2^-49 (relative) to an FP32
+                // rounding boundary, so rounding the product to FP32 gives the
+                // rounded quotient.  Elements past d are 0 and stay 0.
+                const double rinv = 1.0 / (double)nrm;
 #pragma unroll
-                for (int j = 0; j < VPL; ++j) v[j] = v[j] / nrm;
+                for (int j = 0; j < VPL; ++j) v[j] = (float)((double)v[j] * rinv);
                 if (MODE == kFull) {
-                    float* o = a.Xn_out + p * a.ldx;
-                    uint16_t* ob = a.Xb_out ? a.Xb_out + p * a.ldx : nullptr;
+                    float* o = a.Xn_out + p * a.ldx + lane;
+                    uint16_t* ob = a.Xb_out ? a.Xb_out + p * a.ldx + lane : nullptr;
 #pragma unroll
                     for (int j = 0; j < VPL; ++j) {
-                        const int e = lane + 64 * j;
-                        if (e < a.ldx) {
-                            const float w = e < a.d ? v[j] : 0.0f;
-                            o[e] = w;
-                            if (ob) ob[e] = (uint16_t)f32_to_bf16_rne(w);
+                        if (lane + 64 * j < a.ldx) {
+                            o[64 * j] = v[j];
+                            if (ob) ob[64 * j] = (uint16_t)f32_to_bf16_rne(v[j]);
                         }
                     }
                 }
-#pragma unroll
-                for (int j = 0; j < VPL; ++j) if (lane + 64 * j >= a.d) v[j] = 0.0f;
             }
         } else {
             embed_lds<D>(a, p, lane, emb);
@@ -469,9 +525,15 @@ __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
                 }
             }
         } else if constexpr (MODE == kFull) {
+            uint32_t kc[VPL];
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) {
+                kc[j] = ord_key(v[j]);
+                if (64 * j + 63 >= P && lane + 64 * j >= P) kc[j] = 0u;
+            }
             int* tk = a.topK + p * 2 * a.K;
-            const bool t1 = wave_topk<VPL>(v, false, P, a.K, lane, tks, tk);
-            const bool t2 = wave_topk<VPL>(v, true, P, a.K, lane, tks, tk + a.K);
+            bool t1, t2;
+            wave_topk_both<VPL>(kc, P, a.K, lane, (uint32_t*)tks, tk, tk + a.K, t1, t2);
             if (t1 || t2) {
                 int slot = 0;
                 if (lane == 0) slot = atomicAdd(a.tie_count, 1);
@@ -482,23 +544,20 @@ __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
 #pragma unroll
                     for (int j = 0; j < VPL; ++j) {
                         const int e = lane + 64 * j;
-                        if (e < P) tv[e] = v[j];
+                        if (e < P) tv[e] = key_to_float(kc[j]);
                     }
                 }
             }
             // values beyond every row's threshold are rare: one wave-wide test
             bool anyc = false;
 #pragma unroll
-            for (int j = 0; j < VPL; ++j) {
-                const uint32_t k = ord_key(v[j]);
-                anyc |= (lane + 64 * j < P) && (k >= any_hi || k <= any_lo);
-            }
+            for (int j = 0; j < VPL; ++j) anyc |= kc[j] != 0u && (kc[j] >= any_hi || kc[j] <= any_lo);
             if (__any(anyc))
 #pragma unroll 1
             for (int j = 0; j < VPL; ++j) {
                 const int e = lane + 64 * j;
-                if (e >= P) continue;
-                const uint32_t k = ord_key(v[j]);
+                const uint32_t k = kc[j];
+                if (k == 0u) continue;
                 if (k >= any_hi && k >= red[e]) {
                     const int pos = atomicAdd(&a.cand_cnt[2 * e], 1);
                     if (pos < a.cand_cap) {
@@ -516,7 +575,7 @@ __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
                     }
                 }
             }
-        } else {
+        } else if constexpr (MODE == kRows) {
 #pragma unroll
             for (int j = 0; j < VPL; ++j) {
                 const int e = lane + 64 * j;
@@ -524,6 +583,12 @@ __global__ __launch_bounds__(256) void k_index(IndexArgs a) {
                     const int s = a.row_slot[e];
                     if (s >= 0) a.row_buf[(size_t)s * a.n + p] = v[j];
                 }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) {
+                const int e = lane + 64 * j;
+                if (e < P) a.pt_buf[(size_t)e * a.n_ids + q] = v[j];
             }
         }
     }
@@ -702,77 +767,103 @@ __global__ void k_invmask(const int* __restrict__ lists, const int* __restrict__
     atomicOr(&inv[((size_t)l * L2 + L) * W + (i >> 5)], 1u << (i & 31));
 }
 
-// multi[l][w] bit i: the point at position i of list l occurs in at least two
-// lists, so it can repeat among a point's slots.
-__global__ void k_multi(const int* __restrict__ lists, const int* __restrict__ hub_id,
-                        const int* __restrict__ hub_ptr, int L2, int M, int W,
-                        uint32_t* __restrict__ multi) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L2 * W) return;
-    int l = t / W, w = t % W;
-    uint32_t m = 0;
-    for (int b = 0; b < 32; ++b) {
-        int i = w * 32 + b;
-        if (i >= M) break;
-        int h = hub_id[lists[(size_t)l * M + i]];
-        if (hub_ptr[h + 1] - hub_ptr[h] >= 2) m |= 1u << b;
+// Per list position (l, i): pos_hub = hub id of the point there and pcnt =
+// the number of its other list memberships (scanned into pair offsets).
+__global__ void k_pos_hub(const int* __restrict__ lists, const int* __restrict__ hub_id,
+                          const int* __restrict__ hub_ptr, int LM, int* __restrict__ pos_hub,
+                          int* __restrict__ pcnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > LM) return;
+    if (t == LM) { pcnt[LM] = 0; return; }
+    const int h = hub_id[lists[t]];
+    pos_hub[t] = h;
+    pcnt[t] = hub_ptr[h + 1] - hub_ptr[h] - 1;
+}
+
+// pairs[poff[l M + i] ...]: (L << 8) | i for every other list L holding the
+// point at position i of list l; the pairs of list l are contiguous.
+__global__ void k_pair_fill(const int* __restrict__ pos_hub, const int* __restrict__ hub_ptr,
+                            const int* __restrict__ hub_mem, int LM, int M, const int* __restrict__ poff,
+                            uint32_t* __restrict__ pairs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= LM) return;
+    const int l = t / M, i = t - l * M;
+    const int h = pos_hub[t];
+    int o = poff[t];
+    for (int q = hub_ptr[h]; q < hub_ptr[h + 1]; ++q) {
+        const int L = hub_mem[q] / M;
+        if (L != l) pairs[o++] = ((uint32_t)L << 8) | (uint32_t)i;
     }
-    multi[t] = m;
 }
 
 // First-occurrence mask per (point, slot): bit i of F[a][s] is set when the
 // point at position i of the slot's list is not a itself and does not occur
 // in an earlier slot of a (the reference's per-point dedup bitset,
-// src/sDbscan.cpp:260-282).  Only multi positions can repeat; each is checked
-// against the lists of the earlier slots through its hub memberships.
-__global__ __launch_bounds__(256) void k_first_mask(const int* __restrict__ topK, long long n, int P, int K,
-                                                    int M, int W, const uint32_t* __restrict__ multi,
-                                                    const int* __restrict__ lists,
+// src/sDbscan.cpp:260-282).  Only points held by several lists can repeat;
+// the (position, other list) pairs of the slot's list are checked against the
+// lists of the earlier slots.  A workgroup takes ppb consecutive points with one thread per (point, slot)
+// entry: the slot lists go to LDS for the duplicate checks, and the W mask
+// words of consecutive entries are written by consecutive threads (one
+// 16-byte store each when W = 4).
+__global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK, long long n, int P, int K,
+                                                    int M, int W, int ppb, const int* __restrict__ poff,
+                                                    const uint32_t* __restrict__ pairs,
                                                     const int* __restrict__ hub_id,
                                                     const int* __restrict__ hub_ptr,
                                                     const int* __restrict__ hub_mem,
                                                     uint32_t* __restrict__ F) {
-    // one thread per point; its 2K list ids are staged in LDS
-    extern __shared__ int lid_s[];
+    extern __shared__ int lids[];
     const int S = 2 * K;
-    const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
-    int* lids = lid_s + threadIdx.x * S;
-    for (int s = 0; s < S; ++s) lids[s] = slot_list(topK, a, s, K, P);
+    const int tid = threadIdx.x;
+    const int pl = tid / S, s = tid - pl * S;
+    const long long a = (long long)blockIdx.x * ppb + pl;
+    const bool live = pl < ppb && a < n;
+    if (live) lids[tid] = slot_list(topK, a, s, K, P);
+    __syncthreads();
+    if (!live) return;
+    const int l = lids[tid];
+    const int* mine = lids + pl * S;
+    uint32_t m[kMaxW];
+#pragma unroll
+    for (int w = 0; w < kMaxW; ++w) {
+        const int nb = M - w * 32;
+        m[w] = w >= W ? 0u : (nb >= 32 ? 0xffffffffu : ((1u << nb) - 1u));
+    }
+    if (s > 0) {
+        // positions of l whose point is also in another list L: a duplicate
+        // when L is one of the earlier slots' lists
+        const int q1 = poff[(size_t)(l + 1) * M];
+        for (int q = poff[(size_t)l * M]; q < q1; ++q) {
+            const uint32_t pr = pairs[q];
+            const int L = (int)(pr >> 8), i = (int)(pr & 255u);
+            bool dup = false;
+            for (int s2 = 0; s2 < s; ++s2) dup |= mine[s2] == L;
+            if (dup) {
+#pragma unroll
+                for (int x = 0; x < kMaxW; ++x)
+                    if (x == (i >> 5)) m[x] &= ~(1u << (i & 31));
+            }
+        }
+    }
     const int hs = hub_id[a];
-    for (int s = 0; s < S; ++s) {
-        const int l = lids[s];
-        uint32_t m[kMaxW];
-        for (int w = 0; w < W; ++w) {
-            const int nb = M - w * 32;
-            m[w] = nb >= 32 ? 0xffffffffu : ((1u << nb) - 1u);
-        }
-        if (s > 0) {
-            for (int w = 0; w < W; ++w) {
-                uint32_t mw = multi[(size_t)l * W + w];
-                while (mw) {
-                    const int b = __ffs(mw) - 1;
-                    mw &= mw - 1;
-                    const int h = hub_id[lists[(size_t)l * M + w * 32 + b]];
-                    bool dup = false;
-                    for (int q = hub_ptr[h]; q < hub_ptr[h + 1] && !dup; ++q) {
-                        const int L = hub_mem[q] / M;
-                        if (L == l) continue;
-                        for (int s2 = 0; s2 < s; ++s2)
-                            if (lids[s2] == L) { dup = true; break; }
-                    }
-                    if (dup) m[w] &= ~(1u << b);
-                }
+    if (hs >= 0) {
+        for (int q = hub_ptr[hs]; q < hub_ptr[hs + 1]; ++q) {
+            const int li = hub_mem[q];
+            if (li / M == l) {
+                const int i = li % M;
+#pragma unroll
+                for (int w = 0; w < kMaxW; ++w)
+                    if (w == (i >> 5)) m[w] &= ~(1u << (i & 31));
             }
         }
-        if (hs >= 0) {
-            for (int q = hub_ptr[hs]; q < hub_ptr[hs + 1]; ++q) {
-                const int li = hub_mem[q];
-                if (li / M == l) { const int i = li % M; m[i >> 5] &= ~(1u << (i & 31)); }
-            }
-        }
-        uint32_t* Fo = F + ((size_t)a * S + s) * W;
-        for (int w = 0; w < W; ++w) Fo[w] = m[w];
+    }
+    uint32_t* Fo = F + ((size_t)a * S + s) * W;
+    if (W == 4) {
+        *(uint4*)Fo = uint4{m[0], m[1], m[2], m[3]};
+    } else {
+#pragma unroll
+        for (int w = 0; w < kMaxW; ++w)
+            if (w < W) Fo[w] = m[w];
     }
 }
 
@@ -849,7 +940,7 @@ int launch_transform(const float* X, long long n, int d, int ldx, int metric, fl
 template <int LOGD, int MODE, bool COS>
 static int launch_index_t(const IndexArgs& a, int grid, hipStream_t st) {
     constexpr int D = 1 << LOGD;
-    size_t sm = (MODE == kFull ? (size_t)4 * 512 : 0) + (size_t)2 * a.P * 4 + (COS ? 0 : (size_t)4 * D * 4);
+    size_t sm = (MODE == kFull ? (size_t)4 * 1024 : 0) + (size_t)2 * a.P * 4 + (COS ? 0 : (size_t)4 * D * 4);
     if (sm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE, COS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -864,7 +955,8 @@ static int launch_index_d(const IndexArgs& a, int mode, int grid, hipStream_t st
     const bool cos = a.metric == kCosine;
     if (mode == kSample) return cos ? launch_index_t<LOGD, kSample, true>(a, grid, st) : launch_index_t<LOGD, kSample, false>(a, grid, st);
     if (mode == kFull) return cos ? launch_index_t<LOGD, kFull, true>(a, grid, st) : launch_index_t<LOGD, kFull, false>(a, grid, st);
-    return cos ? launch_index_t<LOGD, kRows, true>(a, grid, st) : launch_index_t<LOGD, kRows, false>(a, grid, st);
+    if (mode == kRows) return cos ? launch_index_t<LOGD, kRows, true>(a, grid, st) : launch_index_t<LOGD, kRows, false>(a, grid, st);
+    return cos ? launch_index_t<LOGD, kPoints, true>(a, grid, st) : launch_index_t<LOGD, kPoints, false>(a, grid, st);
 }
 
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st) {
@@ -930,21 +1022,30 @@ int launch_invmask(const int* lists, const int* topK, int P, int K, int M, int L
     return (int)hipGetLastError();
 }
 
-int launch_multi(const int* lists, const int* hub_id, const int* hub_ptr, int L2, int M, int W,
-                 uint32_t* multi, hipStream_t st) {
-    hipLaunchKernelGGL(k_multi, dim3((L2 * W + 255) / 256), dim3(256), 0, st, lists, hub_id, hub_ptr, L2, M, W, multi);
+int launch_pair_count(const int* lists, const int* hub_id, const int* hub_ptr, int L2, int M, int* pos_hub,
+                      int* pcnt, int* poff, int* total, void* tmp, size_t tmp_bytes, hipStream_t st) {
+    const int LM = L2 * M;
+    hipLaunchKernelGGL(k_pos_hub, dim3((LM + 1 + 255) / 256), dim3(256), 0, st, lists, hub_id, hub_ptr, LM, pos_hub,
+                       pcnt);
+    return scan_exclusive_i32(pcnt, poff, (long long)LM + 1, total, tmp, tmp_bytes, st);
+}
+
+int launch_pair_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem, int L2, int M, const int* poff,
+                     uint32_t* pairs, hipStream_t st) {
+    const int LM = L2 * M;
+    hipLaunchKernelGGL(k_pair_fill, dim3((LM + 255) / 256), dim3(256), 0, st, pos_hub, hub_ptr, hub_mem, LM, M, poff,
+                       pairs);
     return (int)hipGetLastError();
 }
 
-int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const uint32_t* multi,
-                      const int* lists, const int* hub_id, const int* hub_ptr, const int* hub_mem,
+int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const int* poff,
+                      const uint32_t* pairs, const int* hub_id, const int* hub_ptr, const int* hub_mem,
                       uint32_t* F, hipStream_t st) {
-    const size_t sm = (size_t)256 * 2 * K * 4;
-    if (sm > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_first_mask, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        if (e != hipSuccess) return (int)e;
-    }
-    hipLaunchKernelGGL(k_first_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), sm, st, topK, n, P, K, M, W, multi, lists, hub_id, hub_ptr, hub_mem, F);
+    const int S = 2 * K;
+    const int ppb = std::max(1, 512 / S);
+    const long long blocks = (n + ppb - 1) / ppb;
+    hipLaunchKernelGGL(k_first_mask, dim3((unsigned)blocks), dim3(ppb * S), (size_t)ppb * S * 4, st, topK, n, P, K, M,
+                       W, ppb, poff, pairs, hub_id, hub_ptr, hub_mem, F);
     return (int)hipGetLastError();
 }
 

@@ -37,6 +37,9 @@ struct IndexArgs {
     int* cand_idx;
     const int* row_slot;  // P (mode 2)
     float* row_buf;       // slots x n (mode 2)
+    const int* pt_ids;    // points whose projections are written (mode 3)
+    long long n_ids;
+    float* pt_buf;        // P x n_ids (mode 3)
 };
 
 struct DistArgs {
@@ -60,6 +63,11 @@ struct DistArgs {
     int kc;               // dims per LDS chunk
     int nchunk;
     int dk;               // dims covered by the tiles (multiple of the k-step)
+    // single-chunk cosine tiles decide on the raw dot s: hit iff s > s_hit,
+    // hit or inside the band iff s >= s_near (equivalent to the band test on
+    // fl(1 - s), see cosine_cuts)
+    float s_hit, s_near;
+    unsigned long long magicS;  // ceil(2^40 / S): a = (ent * magicS) >> 40
 };
 
 // index
@@ -77,10 +85,12 @@ int launch_hub_fill(const int* lists, int LM, const int* hub_id, const int* hub_
                     int* hub_mem, hipStream_t st);
 int launch_invmask(const int* lists, const int* topK, int P, int K, int M, int L2, int W,
                    uint32_t* inv, hipStream_t st);
-int launch_multi(const int* lists, const int* hub_id, const int* hub_ptr, int L2, int M, int W,
-                 uint32_t* multi, hipStream_t st);
-int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const uint32_t* multi,
-                      const int* lists, const int* hub_id, const int* hub_ptr, const int* hub_mem,
+int launch_pair_count(const int* lists, const int* hub_id, const int* hub_ptr, int L2, int M, int* pos_hub,
+                      int* pcnt, int* poff, int* total, void* tmp, size_t tmp_bytes, hipStream_t st);
+int launch_pair_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem, int L2, int M, const int* poff,
+                     uint32_t* pairs, hipStream_t st);
+int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const int* poff,
+                      const uint32_t* pairs, const int* hub_id, const int* hub_ptr, const int* hub_mem,
                       uint32_t* F, hipStream_t st);
 int launch_group_count(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
                        int Gx, int* wgcnt, hipStream_t st);

@@ -36,7 +36,8 @@ def test_dropin_module_surface():
     cls = m.sDbscan
     for name in ("set_params", "set_sngParams", "set_kernel_params", "clear", "set_topM",
                  "set_topK", "set_threads", "set_clusterNoise", "fit_sDbscan", "test_sDbscan",
-                 "fit_sOptics", "labels_", "n_clusters_", "ordering_", "reachability_"):
+                 "fit_sOptics", "load_fit_sDbscan", "load_test_sDbscan", "load_fit_sOptics",
+                 "labels_", "n_clusters_", "ordering_", "reachability_"):
         assert hasattr(cls, name), name
     doc = cls.set_params.__doc__
     for kw, val in (("n_proj", "1024"), ("topK", "10"), ("topM", "50"), ("distance", "'Cosine'"),

@@ -103,6 +103,22 @@ def test_bf16_matches_fp32(ctx):
     assert np.array_equal(l32, l16) and n32 == n16
 
 
+@pytest.mark.parametrize("n,d,K,M,eps", [
+    (6000, 784, 10, 100, 0.33),   # 25 k-steps, 7 candidate tiles (the 8M x 784 shape)
+    (6000, 784, 5, 50, 0.33),     # 4 candidate tiles, 2 mask words
+    (5000, 300, 8, 64, 0.30),     # 10 k-steps padded to 25
+    (5000, 128, 8, 50, 0.30),     # 4 k-steps padded to 8
+])
+def test_bf16_tiles_match_oracle(ctx, n, d, K, M, eps):
+    # eps sits inside the bulk of the within-cluster distances, so many pairs
+    # fall in the BF16 error band and go through the FP32/FP64 recheck
+    X, _ = datagen.mixture(n, d, sigma=0.3, seed=81 + d)
+    print(f"bf16 tiles n={n} d={d} K={K} M={M} eps={eps}", flush=True)
+    p, bits, R, tk, tm, Xt, st = full_check(ctx, X, S.COSINE, eps, 10, K=K, M=M, precision=S.BF16)
+    print("queue", st["n_queue"], "fp32", st["n_recheck32"], "fp64", st["n_recheck64"], flush=True)
+    assert st["n_queue"] > 0
+
+
 def full_check(ctx, X, metric, eps, minPts, **kw):
     """GPU index + core + labels against the oracle fed the GPU's transformed
     points and index (staged parity); returns the GPU stats."""
@@ -212,6 +228,61 @@ def test_optics_core_dist_and_reachability(ctx):
     dreach = ctx.optics_reach(order)
     print("optics: unreached", int((reach == -2147483648.0).sum()), flush=True)
     assert np.array_equal(dreach, reach)
+
+
+def test_label_noise_matches_oracle(ctx):
+    X, _ = datagen.mixture(6000, 128, k=6, sigma=0.35, seed=91)
+    print("label noise n=6000 d=128 K=8 M=50", flush=True)
+    p, bits, R = run_index(ctx, X, K=8, M=50, n_proj=256)
+    core, deg = ctx.find_core(0.33, 20)
+    lab, nc = ctx.form_clusters()
+    noise = int((lab == -1).sum())
+    print("noise points", noise, "cores", int(core.sum()), flush=True)
+    assert noise > 50 and core.sum() > 50
+    cores = np.where(core)[0].astype(np.int32)
+    sampled = cores[np.random.default_rng(0).random(len(cores)) < 0.3]
+    glab = ctx.label_noise(sampled)
+    tk, tm = ctx.get_index()
+    Xt = ctx.get_transformed()
+    E = O.embed(Xt[sampled], S.COSINE, p.ker_n_features, p.ker_intervalSampling, R)
+    cproj = O.project(E, bits, p.fhtDim, p.n_proj)
+    olab = O.label_noise(cproj, tk, lab, sampled)
+    assert np.array_equal(glab, olab)
+    assert (glab == -1).sum() == 0
+
+
+def test_dropin_module_cluster_noise(ctx):
+    m = S.module()
+    X, _ = datagen.mixture(4000, 128, k=5, sigma=0.35, seed=93)
+    labs = {}
+    for cn in (0, 1, 2):
+        db = m.sDbscan(4000, 128)
+        db.set_params(n_proj=256, topK=8, topM=50, distance="Cosine", clusterNoise=cn, samplingProb=0.05,
+                      random_seed=42)
+        db.fit_sDbscan(X.T, 0.33, 20)
+        labs[cn] = np.array(db.labels_)
+    print("noise with clusterNoise 0/1/2:", [int((labs[c] == -1).sum()) for c in (0, 1, 2)], flush=True)
+    assert (labs[0] == -1).sum() > 0 and (labs[1] == -1).sum() == 0
+    assert np.array_equal(labs[0], labs[2])           # the heuristic changes nothing (no kept neighbourhoods)
+    keep = labs[0] >= 0
+    assert np.array_equal(labs[1][keep], labs[0][keep])
+
+
+def test_dropin_module_text_loader(ctx, tmp_path):
+    m = S.module()
+    X, _ = datagen.mixture(2000, 64, k=4, sigma=0.3, seed=95)
+    path = tmp_path / "points.txt"
+    np.savetxt(path, X)                     # one point per line, exact float32 round trip
+    runs = []
+    for use_file in (False, True):
+        db = m.sDbscan(2000, 64)
+        db.set_params(n_proj=256, topK=5, topM=30, distance="Cosine", random_seed=7)
+        if use_file:
+            db.load_fit_sDbscan(str(path), 0.3, 10)
+        else:
+            db.fit_sDbscan(X.T, 0.3, 10)
+        runs.append((np.array(db.labels_), db.n_clusters_))
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
 
 
 def test_dropin_module_fit(ctx):

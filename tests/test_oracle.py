@@ -250,3 +250,35 @@ def test_optics_full_candidates_matches_sklearn_core_distances():
             assert reach[j] == np.float32(min(preds))
         else:
             assert reach[j] == np.float32(-2147483648.0)
+
+
+# ------------------------------------------------------- noise labelling
+def test_label_noise_takes_the_best_estimate_and_first_on_ties():
+    rng = np.random.default_rng(4)
+    P, n, K, nS = 64, 400, 5, 60
+    proj = rng.standard_normal((P, n)).astype(np.float32)
+    topK = O.topk(proj, K)
+    labels = np.full(n, -1, np.int32)
+    sampled = np.sort(rng.choice(n, nS, replace=False)).astype(np.int32)
+    labels[sampled] = np.arange(nS)          # label = position in the sample
+    cproj = rng.standard_normal((P, nS)).astype(np.float32)
+    out = O.label_noise(cproj, topK, labels, sampled)
+    noise = np.where(labels == -1)[0]
+    c64 = cproj.astype(np.float64)
+    for a in noise:
+        est = c64[topK[a, :K]].sum(0) - c64[topK[a, K:]].sum(0)
+        i = out[a]
+        # the chosen core's exact estimate is the maximum up to FP32 rounding
+        assert est[i] >= est.max() - 4 * 2 * K * np.abs(c64).max() * 2.0 ** -23
+    # a column built for noise point a0 wins for it; an identical copy later
+    # in the sample never wins (the first of equal estimates is taken)
+    a0 = noise[0]
+    cp = cproj.copy()
+    cp[:, 7] = 0.0
+    cp[topK[a0, :K], 7] = 5.0
+    cp[topK[a0, K:], 7] = -5.0
+    cp[:, 30] = cp[:, 7]
+    out = O.label_noise(cp, topK, labels, sampled)
+    assert out[a0] == 7
+    assert not np.any(out[noise] == 30)
+    assert np.array_equal(out[sampled], labels[sampled])
