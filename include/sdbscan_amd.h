@@ -11,6 +11,7 @@
  *   sdb_find_core      <- rp_findCorePoints (src/sDbscan.cpp:217-408)
  *   sdb_form_clusters  <- formCluster (src/sDbscan.cpp:509-637)
  *   sdb_label_noise    <- labelNoise, sDbscan-1NN (src/sDbscan.cpp:685-807)
+ *   sdb_sng_find_core  <- sng_findCorePoints / sng_findCoreDist (src/sDbscan.cpp:416-504, 1484-1583)
  *   sdb_core_dist      <- rp_findCoreDist (src/sDbscan.cpp:1314-1474)
  *   sdb_optics_reach   <- the reachability values of formOptics_scikit
  *                         (src/sDbscan.cpp:1588-1703) for a given ordering
@@ -109,6 +110,22 @@ int sdb_form_clusters(sdb_ctx* ctx, int* labels_out, int* n_clusters_out);
  * increasing index order, drawn by the caller as the reference draws them
  * (src/sDbscan.cpp:695-709).  labels_out (n ints) may be NULL. */
 int sdb_label_noise(sdb_ctx* ctx, const int* sampled_core, int n_sampled, int* labels_out);
+
+/* sngDbscan / sngOptics neighbourhoods without the index
+ * (sng_findCorePoints, src/sDbscan.cpp:416-504; sng_findCoreDist,
+ * src/sDbscan.cpp:1484-1583).  Point a is compared with samples[j] for
+ * j < win_end[a] (n_samples for every point when win_end is NULL), skipping
+ * the positions where samples[j] == a; a pair is a hit when its distance is
+ * <= eps (decided exactly as in sdb_find_core).
+ *   flags bit 0: a hit also puts a into the neighbourhood of samples[j]
+ *                (the sampled variants; the exact variant sets it to 0);
+ *   flags bit 1: deduplicate the neighbourhoods (sngOptics); otherwise
+ *                |N(a)| counts repeated entries, as sngDbscan does.
+ * Core iff |N(a)| >= minPts - 1.  Afterwards sdb_form_clusters,
+ * sdb_get_neighbors, sdb_core_dist and sdb_optics_reach work on these
+ * neighbourhoods; sdb_label_noise needs the index and is refused. */
+int sdb_sng_find_core(sdb_ctx* ctx, const int* samples, int n_samples, const int* win_end, int flags,
+                      float eps, int minPts, uint8_t* core_out, int* degree_out);
 
 /* Index matrices in the reference layouts (either pointer may be NULL). */
 int sdb_get_index(sdb_ctx* ctx, int* topK_out, int* topM_out);

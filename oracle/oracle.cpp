@@ -498,6 +498,59 @@ int orc_form_cluster(int n, const uint8_t* core, const int* ptr, const int* idx,
     return n_clusters + 2;  // src/sDbscan.cpp:633
 }
 
+// -------------------------------------------------------- sng neighbours
+// sng_findCorePoints (src/sDbscan.cpp:416-504) / sng_findCoreDist
+// (:1484-1583) with the draw sequence given: point p takes the draws
+// samples[j], j < win_end[p] (nS when win_end is NULL), that are not p; a hit
+// (FP64 decision, as orc_find_core mode 1) goes to N(p) and, with flags bit 0,
+// p goes to N(samples[j]).  |N| counts duplicates unless flags bit 1
+// (sngOptics dedups, :1528-1540).  The returned CSR is deduplicated and
+// sorted by index (what formCluster and formOptics consume).
+void* orc_sng_find_core(const float* X, int n, int d, int dist, const int* samples, int nS, const int* win_end,
+                        int flags, float eps, int minPts, uint8_t* core, int* degree) {
+    std::vector<std::vector<std::pair<int, float>>> nbr(n);
+    const bool both = flags & 1, dedup = flags & 2;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int p = 0; p < n; ++p) {
+        const int we = win_end ? win_end[p] : nS;
+        const float* xp = X + (size_t)p * d;
+        for (int j = 0; j < we && j < nS; ++j) {
+            const int q = samples[j];
+            if (q == p) continue;
+            const float* xq = X + (size_t)q * d;
+            if (orc_dist64(xp, xq, d, dist) <= (double)eps) {
+                const float d32 = dist32(xp, xq, d, dist);
+#pragma omp critical
+                {
+                    nbr[p].emplace_back(q, d32);
+                    if (both) nbr[q].emplace_back(p, d32);
+                }
+            }
+        }
+    }
+    CoreResult* res = new CoreResult();
+    res->ptr.assign(n + 1, 0);
+#pragma omp parallel for
+    for (int p = 0; p < n; ++p) {
+        const int raw = (int)nbr[p].size();
+        std::unordered_map<int, float> uniq(nbr[p].begin(), nbr[p].end());
+        std::vector<std::pair<int, float>> v(uniq.begin(), uniq.end());
+        std::sort(v.begin(), v.end());
+        nbr[p].swap(v);
+        degree[p] = dedup ? (int)nbr[p].size() : raw;
+        core[p] = degree[p] >= minPts - 1 ? 1 : 0;
+    }
+    for (int p = 0; p < n; ++p) res->ptr[p + 1] = res->ptr[p] + (int)nbr[p].size();
+    res->idx.resize(res->ptr[n]);
+    res->dd.resize(res->ptr[n]);
+    for (int p = 0; p < n; ++p)
+        for (size_t j = 0; j < nbr[p].size(); ++j) {
+            res->idx[res->ptr[p] + j] = nbr[p][j].first;
+            res->dd[res->ptr[p] + j] = nbr[p][j].second;
+        }
+    return res;
+}
+
 // ---------------------------------------------------------- label noise
 // sDbscan-1NN (src/sDbscan.cpp:775-806): cproj is P x nS (row = random
 // vector, column i = sampled core i).  For each point labelled -1:

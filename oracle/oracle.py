@@ -31,6 +31,8 @@ def lib():
             "orc_transform": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
             "orc_fht": ([_P, ctypes.c_int], None),
             "orc_label_noise": ([ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P], None),
+            "orc_sng_find_core": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, _P, ctypes.c_int,
+                                   ctypes.c_float, ctypes.c_int, _P, _P], _P),
             "orc_fwht_textbook": ([_P, ctypes.c_int], None),
             "orc_embed": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P, _P], None),
             "orc_project_embedded": ([_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P], None),
@@ -166,6 +168,28 @@ def find_core(X, metric, topK, topM, eps, minPts, mode=1):
     lib().orc_csr_copy(h, ptr.ctypes.data, idx.ctypes.data, dd.ctypes.data)
     lib().orc_free(h)
     return core, deg, (ptr, idx[:tot], dd[:tot]), near.value, nd.value
+
+
+def sng_find_core(X, metric, samples, eps, minPts, win_end=None, both_ways=True, dedup=False):
+    """sngDbscan / sngOptics neighbourhoods from a draw sequence: core, degree,
+    deduplicated CSR (ptr, idx, dist32)."""
+    X = _c(X, np.float32)
+    n, d = X.shape
+    smp = _c(samples, np.int32)
+    we = _c(win_end, np.int32) if win_end is not None else None
+    core = np.zeros(n, np.uint8)
+    deg = np.zeros(n, np.int32)
+    flags = (1 if both_ways else 0) | (2 if dedup else 0)
+    h = lib().orc_sng_find_core(X.ctypes.data, n, d, metric, smp.ctypes.data if len(smp) else None, len(smp),
+                                we.ctypes.data if we is not None else None, flags, eps, minPts,
+                                core.ctypes.data, deg.ctypes.data)
+    tot = lib().orc_csr_size(h)
+    ptr = np.zeros(n + 1, np.int32)
+    idx = np.zeros(max(tot, 1), np.int32)
+    dd = np.zeros(max(tot, 1), np.float32)
+    lib().orc_csr_copy(h, ptr.ctypes.data, idx.ctypes.data, dd.ctypes.data)
+    lib().orc_free(h)
+    return core, deg, (ptr, idx[:tot], dd[:tot])
 
 
 def form_cluster(core, ptr, idx):

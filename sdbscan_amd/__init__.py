@@ -58,6 +58,8 @@ _SIGS = {
     "sdb_find_core": ([_P, ctypes.c_float, ctypes.c_int, _P, _P], ctypes.c_int),
     "sdb_form_clusters": ([_P, _P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "sdb_label_noise": ([_P, _P, ctypes.c_int, _P], ctypes.c_int),
+    "sdb_sng_find_core": ([_P, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int, _P, _P],
+                          ctypes.c_int),
     "sdb_get_index": ([_P, _P, _P], ctypes.c_int),
     "sdb_get_transformed": ([_P, _P], ctypes.c_int),
     "sdb_get_neighbors": ([_P, _P, _P, _P], ctypes.c_int),
@@ -181,6 +183,21 @@ class Context:
         self._chk(self.lib.sdb_form_clusters(self.h, lab.ctypes.data if want else None,
                                              ctypes.byref(nc)))
         return lab, nc.value
+
+    def sng_find_core(self, samples, eps, minPts, win_end=None, both_ways=True, dedup=False):
+        """Neighbourhoods from a sample sequence (sngDbscan / sngOptics):
+        core flags and degrees."""
+        import numpy as np
+        n = self.params.n_points
+        smp = np.ascontiguousarray(samples, dtype=np.int32)
+        we = None if win_end is None else np.ascontiguousarray(win_end, dtype=np.int32)
+        core = np.zeros(n, np.uint8)
+        deg = np.zeros(n, np.int32)
+        flags = (1 if both_ways else 0) | (2 if dedup else 0)
+        self._chk(self.lib.sdb_sng_find_core(self.h, smp.ctypes.data if len(smp) else None, len(smp),
+                                             we.ctypes.data if we is not None else None, flags,
+                                             eps, minPts, core.ctypes.data, deg.ctypes.data))
+        return core, deg
 
     def label_noise(self, sampled_core):
         """sDbscan-1NN labels of the noise points after form_clusters, for the

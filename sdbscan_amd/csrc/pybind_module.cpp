@@ -105,6 +105,44 @@ PYBIND11_MODULE(sDbscan, m) {
                 s.load_fit_sOptics(dataset, eps, minPts);
             },
             py::arg("dataset"), py::arg("eps"), py::arg("minPts"))
+        .def(
+            "fit_sngDbscan",
+            [](PySDbscan& s, const FArr& X, float eps, int minPts) {
+                const float* p = check_x(s, X, s.dims.d, s.dims.n);
+                py::gil_scoped_release rel;
+                s.fit_sngDbscan(p, eps, minPts);
+            },
+            py::arg("X"), py::arg("eps"), py::arg("minPts"))
+        .def(
+            "test_sngDbscan",
+            [](PySDbscan& s, const FArr& X, float eps, float rangeEps, int n_tests, int minPts) {
+                const float* p = check_x(s, X, s.dims.d, s.dims.n);
+                py::gil_scoped_release rel;
+                s.test_sngDbscan(p, eps, rangeEps, n_tests, minPts);
+            },
+            py::arg("X"), py::arg("eps"), py::arg("rangeEps"), py::arg("n_tests"), py::arg("minPts"))
+        .def(
+            "load_test_sngDbscan",
+            [](PySDbscan& s, const std::string& dataset, float eps, float rangeEps, int n_tests, int minPts) {
+                py::gil_scoped_release rel;
+                s.load_test_sngDbscan(dataset, eps, rangeEps, n_tests, minPts);
+            },
+            py::arg("dataset"), py::arg("eps"), py::arg("rangeEps"), py::arg("n_tests"), py::arg("minPts"))
+        .def(
+            "fit_sngOptics",
+            [](PySDbscan& s, const FArr& X, float eps, int minPts) {
+                const float* p = check_x(s, X, s.dims.d, s.dims.n);
+                py::gil_scoped_release rel;
+                s.fit_sngOptics(p, eps, minPts);
+            },
+            py::arg("X"), py::arg("eps"), py::arg("minPts"))
+        .def(
+            "load_fit_sngOptics",
+            [](PySDbscan& s, const std::string& dataset, float eps, int minPts) {
+                py::gil_scoped_release rel;
+                s.load_fit_sngOptics(dataset, eps, minPts);
+            },
+            py::arg("dataset"), py::arg("eps"), py::arg("minPts"))
         // extensions of this build
         .def("set_precision", &PySDbscan::set_precision, py::arg("precision"))
         .def("set_device", &PySDbscan::set_device, py::arg("device"))

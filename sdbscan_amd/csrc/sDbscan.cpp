@@ -17,6 +17,7 @@
 #include <random>
 #include <sstream>
 #include <stdexcept>
+#include <unordered_map>
 
 namespace {
 
@@ -346,6 +347,17 @@ void sDbscan::fit_sOptics(const float* X, float eps, int minPts) {
     check(sdb_build_index(ctx, bitHD3.data(), matrix_R.empty() ? nullptr : matrix_R.data()), "build_index");
     std::vector<uint8_t> core(n_points);
     check(sdb_find_core(ctx, eps, minPts, core.data(), nullptr), "find_core");
+    optics_order(core, minPts);
+    if (!output.empty()) {
+        std::ofstream f(out_name(eps, minPts, true));
+        for (int k = 0; k < n_points; ++k) f << ordering_[k] << " " << reachability_[k] << '\n';
+    }
+}
+
+// formOptics_scikit (src/sDbscan.cpp:1588-1703) over the context's
+// neighbourhoods: the ordering is the sequential heap walk on the host, the
+// reachability values are computed on the device for that ordering.
+void sDbscan::optics_order(const std::vector<uint8_t>& core, int minPts) {
     std::vector<long long> ptr(n_points + 1);
     check(sdb_get_neighbors(ctx, ptr.data(), nullptr, nullptr), "neighbors");
     std::vector<int> idx(std::max<long long>(ptr[n_points], 1));
@@ -389,10 +401,137 @@ void sDbscan::fit_sOptics(const float* X, float eps, int minPts) {
     }
     reachability_.assign(n_points, NEG);
     check(sdb_optics_reach(ctx, ordering_.data(), reachability_.data()), "optics_reach");
+}
+
+// ---------------------------------------------------------------------------
+// sngDbscan / sngOptics.  The sample draws are made here with the
+// reference's std:: calls; the device compares every point with its samples.
+//
+// sng_findCorePoints (src/sDbscan.cpp:425-465): every point seeds its own
+// default_random_engine with `seed` (random_seed >= 0) or with the clock, and
+// draws uniform_int_distribution<>(0, n - 1) until it has iNumSamples draws
+// other than itself.  With random_seed >= 0 all points therefore walk the
+// same draw sequence; this build uses one shared sequence in both cases (with
+// a clock seed the reference's per-thread clock reads are not reproducible).
+// Point a's window is the shortest prefix holding iNumSamples draws != a.
+void sDbscan::sng_samples(bool optics, std::vector<int>& samples, std::vector<int>& win_end) const {
+    const int iNumSamples = (int)std::ceil(1.0 * n_points * samplingProb);
+    samples.clear();
+    win_end.clear();
+    if (!optics && samplingProb >= 1.0f) {  // exact branch (src/sDbscan.cpp:466-478)
+        samples.resize(n_points);
+        for (int i = 0; i < n_points; ++i) samples[i] = i;
+        return;
+    }
+    if (optics) {
+        // sng_findCoreDist (src/sDbscan.cpp:1497-1505): mt19937, exactly
+        // iNumSamples draws per point, draws equal to the point are skipped
+        std::random_device rd;
+        std::mt19937 gen(seed >= 0 ? (unsigned)seed : rd());
+        std::uniform_int_distribution<> distrib(0, n_points - 1);
+        samples.resize(iNumSamples);
+        for (int j = 0; j < iNumSamples; ++j) samples[j] = distrib(gen);
+        return;
+    }
+    unsigned sd = (unsigned)std::chrono::system_clock::now().time_since_epoch().count();
+    if (seed >= 0) sd = (unsigned)seed;
+    std::default_random_engine generator(sd);
+    std::uniform_int_distribution<> distrib(0, n_points - 1);
+    long long T = (long long)iNumSamples + 64;
+    for (;;) {
+        while ((long long)samples.size() < T) samples.push_back(distrib(generator));
+        // window ends of the points that occur in the sequence
+        std::vector<std::vector<int>> occ;
+        std::vector<int> who;
+        std::unordered_map<int, int> slot;
+        for (int j = 0; j < (int)samples.size(); ++j) {
+            auto it = slot.find(samples[j]);
+            if (it == slot.end()) {
+                slot.emplace(samples[j], (int)occ.size());
+                occ.emplace_back(1, j);
+                who.push_back(samples[j]);
+            } else {
+                occ[it->second].push_back(j);
+            }
+        }
+        bool ok = true;
+        win_end.assign(n_points, iNumSamples);
+        for (size_t k = 0; k < occ.size(); ++k) {
+            int cnt = 0;
+            while (cnt < (int)occ[k].size() && occ[k][cnt] < iNumSamples + cnt) ++cnt;
+            win_end[who[k]] = iNumSamples + cnt;
+            if (iNumSamples + cnt > T) ok = false;
+        }
+        if (ok) break;
+        T *= 2;
+    }
+}
+
+// src/sDbscan.cpp:1131-1190
+void sDbscan::fit_sngDbscan(const float* X, float eps, int minPts) {
+    if (verbose) print_params(eps, minPts);
+    clusterNoise = 0;  // the reference does not cluster noise for sngDbscan (:1159)
+    prepare(X);
+    std::vector<int> samples, win_end;
+    sng_samples(false, samples, win_end);
+    const bool exact = win_end.empty() && samplingProb >= 1.0f;
+    check(sdb_sng_find_core(ctx, samples.data(), (int)samples.size(), win_end.empty() ? nullptr : win_end.data(),
+                            exact ? 0 : 1, eps, minPts, nullptr, nullptr),
+          "sng_find_core");
+    labels_.assign(n_points, -1);
+    check(sdb_form_clusters(ctx, labels_.data(), &n_clusters_), "form_clusters");
+    if (verbose) {
+        sdb_stats st{};
+        sdb_get_stats(ctx, &st);
+        std::cout << "Number of core points: " << st.n_core << std::endl;
+        std::cout << "Find core points time = " << (long long)st.t_core_ms << "[ms]" << std::endl;
+        std::cout << "Form clusters (including clustering noise) time = " << (long long)st.t_cluster_ms << "[ms]" << std::endl;
+        std::ofstream f(output + "_" + distance + "_Eps_" + int2str(std::lround(1000 * eps)) + "_MinPts_" +
+                        int2str(minPts) + "_Prob_" + int2str(std::lround(1000 * samplingProb)));
+        for (int v : labels_) f << v << '\n';
+    }
+}
+
+// src/sDbscan.cpp:1192-1252: one sampled neighbourhood per eps
+void sDbscan::test_sngDbscan(const float* X, float eps, float rangeEps, int n_tests, int minPts) {
+    std::cout << "base eps: " << eps << std::endl;
+    std::cout << "range eps: " << rangeEps << std::endl;
+    std::cout << "n_tests: " << n_tests << std::endl;
+    const bool v = verbose;
+    for (int i = 0; i < n_tests; ++i) {
+        const float e = eps + 1.0f * i * rangeEps;
+        std::cout << "Eps: " << e << std::endl;
+        verbose = true;
+        fit_sngDbscan(X, e, minPts);
+    }
+    verbose = v;
+}
+
+// src/sDbscan.cpp:1846-1886
+void sDbscan::fit_sngOptics(const float* X, float eps, int minPts) {
+    if (verbose) print_params(eps, minPts);
+    prepare(X);
+    std::vector<int> samples, win_end;
+    sng_samples(true, samples, win_end);
+    std::vector<uint8_t> core(n_points);
+    check(sdb_sng_find_core(ctx, samples.data(), (int)samples.size(), nullptr, 3, eps, minPts, core.data(), nullptr),
+          "sng_find_core");
+    optics_order(core, minPts);
     if (!output.empty()) {
-        std::ofstream f(out_name(eps, minPts, true));
+        std::ofstream f(output + "_" + distance + "_Eps_" + int2str(std::lround(1000 * eps)) + "_MinPts_" +
+                        int2str(minPts) + "_Prob_" + int2str(std::lround(1000 * samplingProb)));
         for (int k = 0; k < n_points; ++k) f << ordering_[k] << " " << reachability_[k] << '\n';
     }
+}
+
+void sDbscan::load_test_sngDbscan(const std::string& dataset, float eps, float rangeEps, int n_tests, int minPts) {
+    std::vector<float> X = load_txt(dataset);
+    test_sngDbscan(X.data(), eps, rangeEps, n_tests, minPts);
+}
+
+void sDbscan::load_fit_sngOptics(const std::string& dataset, float eps, int minPts) {
+    std::vector<float> X = load_txt(dataset);
+    fit_sngOptics(X.data(), eps, minPts);
 }
 
 sdb_stats sDbscan::get_stats() const {

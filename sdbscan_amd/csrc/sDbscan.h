@@ -61,6 +61,8 @@ protected:
     std::string out_name(float eps, int minPts, bool optics) const;
     void label_noise(const std::vector<uint8_t>& core);
     std::vector<float> load_txt(const std::string& dataset) const;
+    void optics_order(const std::vector<uint8_t>& core, int minPts);
+    void sng_samples(bool optics, std::vector<int>& samples, std::vector<int>& win_end) const;
 
 public:
     // Dbscan's output
@@ -102,6 +104,13 @@ public:
     void load_fit_sDbscan(const std::string& dataset, float eps, int minPts);
     void load_test_sDbscan(const std::string& dataset, float eps, float rangeEps, int n_tests, int minPts);
     void load_fit_sOptics(const std::string& dataset, float eps, int minPts);
+
+    // sngDbscan / sngOptics (sampled neighbourhoods, no index)
+    void fit_sngDbscan(const float* X, float eps, int minPts);
+    void test_sngDbscan(const float* X, float eps, float rangeEps, int n_tests, int minPts);
+    void fit_sngOptics(const float* X, float eps, int minPts);
+    void load_test_sngDbscan(const std::string& dataset, float eps, float rangeEps, int n_tests, int minPts);
+    void load_fit_sngOptics(const std::string& dataset, float eps, int minPts);
 
     // Extensions of this build (no counterpart in the reference)
     void set_precision(const std::string& p);  // "FP32" (default) or "BF16" distance tiles

@@ -32,6 +32,25 @@ int launch_reach(long long n, const long long* ptr, const int* idx, const float*
 int launch_kth(long long n, const long long* ptr, const float* sorted, const uint8_t* core,
                int k, float* out, hipStream_t st);
 int launch_noise_list(const int* labels, long long n, int* noise, int* count, hipStream_t st);
+int launch_sng_fmask(long long n, int L, int S2, int Ms, int W, int n_samples, const int* win_end,
+                     const int* occ_ptr, const int* occ_pos, uint32_t* F, hipStream_t st);
+int launch_sng_grp(long long n, int L, int S2, uint32_t* grp, hipStream_t st);
+int launch_sng_rev(int n_samples, const int* samples, const int* colsum, int* rev, hipStream_t st);
+int launch_sng_degree(long long n, int L, int S2, int W, const uint32_t* E, const int* rev, int minPts, int* degree,
+                      uint8_t* core, unsigned long long* n_core, hipStream_t st);
+int launch_sng_uf(long long n, int L, int S2, int Ms, int W, const uint32_t* E, const int* samples,
+                  const uint8_t* core, int* par, hipStream_t st);
+int launch_sng_border(long long n, int L, int S2, int Ms, int W, const uint32_t* E, const int* samples,
+                      const uint8_t* core, const int* par, int* border_min, hipStream_t st);
+int launch_sng_csr_count(long long n, int L, int S2, int Ms, int W, int both_ways, const uint32_t* E,
+                         const int* samples, long long* cnt, hipStream_t st);
+int launch_sng_csr_fill(long long n, int L, int S2, int Ms, int W, int both_ways, const uint32_t* E,
+                        const int* samples, const long long* ptr, unsigned long long* cur, int* idx, hipStream_t st);
+int launch_uniq(long long n, const long long* ptr, const int* idx, long long* ucnt, hipStream_t st);
+int launch_uniq_fill(long long n, const long long* ptr, const int* idx, const long long* uptr, int* uidx,
+                     hipStream_t st);
+int launch_rowsize_core(long long n, const long long* ptr, int minPts, int* degree, uint8_t* core,
+                        unsigned long long* n_core, hipStream_t st);
 int launch_noise_score(const float* cproj, long long nS, const int* topK, int K, const int* noise, int n_noise,
                        unsigned long long* best, hipStream_t st);
 int launch_noise_assign(const int* noise, int n_noise, const unsigned long long* best, const int* sampled,
@@ -67,6 +86,9 @@ struct sdb_ctx {
         row_slot, row_buf;
     bool indexed = false;
     int H = 0, L2 = 0, W = 0, S = 0;
+    int Mt = 0;        // candidates per list in the distance tiles (topM, or the sample chunk)
+    bool sng = false;  // neighbourhoods from a sample sequence (sngDbscan / sngOptics)
+    int sng_flags = 0, sngL = 0;
     long long qblk = 0;
     int nblk = 1;
     std::vector<int> h_grp_ptr;
@@ -84,6 +106,11 @@ struct sdb_ctx {
     bool clustered = false;
     // noise labelling
     DevBuf ns_ids, cproj, noise, nbest;
+    // sample-sequence neighbourhoods
+    DevBuf occ_ptr, occ_pos, win, rev, csr_cnt, csr_ptr, csr_cur, csr_idx;
+    bool transformed = false;
+    int transformed_prec = -1;
+    int sng_Ms = 64;
     sdb_stats stats{};
 };
 
@@ -97,6 +124,16 @@ int fail(sdb_ctx* c, int code, const char* fmt, ...) {
     va_end(ap);
     c->err = buf;
     return code;
+}
+
+// Pointer kind as the HIP runtime sees it, for error messages.
+static std::string ptr_desc(const void* p) {
+    hipPointerAttribute_t at{};
+    const hipError_t e = hipPointerGetAttributes(&at, p);
+    (void)hipGetLastError();
+    char buf[160];
+    snprintf(buf, sizeof(buf), "%p: %s, type %d, device %d", p, hipGetErrorString(e), (int)at.type, at.device);
+    return buf;
 }
 
 #define HIPCHK(call)                                                                        \
@@ -204,7 +241,8 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
                      &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
                      &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
-                     &c->cproj, &c->noise, &c->nbest};
+                     &c->cproj, &c->noise, &c->nbest, &c->occ_ptr, &c->occ_pos, &c->win, &c->rev,
+                     &c->csr_cnt, &c->csr_ptr, &c->csr_cur, &c->csr_idx};
     for (DevBuf* b : all)
         if (b->p) hipFree(b->p);
     for (int i = 0; i < 8; ++i) hipEventDestroy(c->ev[i]);
@@ -251,6 +289,7 @@ static int set_input_dims(sdb_ctx* c, long long n, int d) {
     c->d = d;
     c->ldx = (d + 31) / 32 * 32;
     c->indexed = c->cored = c->have_nb = false;
+    c->transformed = c->sng = c->clustered = false;
     return 0;
 }
 
@@ -308,6 +347,8 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     c->L2 = 2 * P;
     c->W = (M + 31) / 32;
     c->S = 2 * K;
+    c->Mt = M;
+    c->sng = false;
     c->stats = sdb_stats{};
     hipStream_t st = c->st;
 
@@ -529,6 +570,8 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     c->stats.t_transform_ms = elapsed(c->ev[0], c->ev[1]);
     c->stats.t_index_ms = elapsed(c->ev[1], c->ev[2]);
     c->indexed = true;
+    c->transformed = true;
+    c->transformed_prec = q.precision;
     c->cored = false;
     c->have_nb = false;
     return SDB_OK;
@@ -545,7 +588,7 @@ int sdb_get_index(sdb_ctx* c, int* topK_out, int* topM_out) {
 
 int sdb_get_transformed(sdb_ctx* c, float* X_out) {
     if (!c || !X_out) return SDB_EINVAL;
-    if (!c->indexed) return fail(c, SDB_ESTATE, "build_index first");
+    if (!c->transformed) return fail(c, SDB_ESTATE, "build_index or sng_find_core first");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpy2D(X_out, (size_t)c->d * 4, c->Xn.p, (size_t)c->ldx * 4, (size_t)c->d * 4, (size_t)c->n, hipMemcpyDeviceToHost));
     return SDB_OK;
@@ -615,13 +658,13 @@ static void compute_bands(const sdb_params& q, float eps, double* band, double* 
     }
 }
 
-int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* degree_out) {
-    if (!c) return SDB_EINVAL;
-    if (!c->indexed) return fail(c, SDB_ESTATE, "build_index first");
-    HIPCHK(hipSetDevice(c->device));
+// Distance tiles over the ctx's list groups (c->h_grp_ptr over nblk x L2
+// groups of entries, lists of c->Mt points, masks F -> hits E, colsum) and the
+// FP32/FP64 recheck of the pairs inside the error band.
+static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, double* band32_out) {
     const sdb_params& q = c->prm;
     const long long n = c->n;
-    const int M = q.topM, W = c->W, S = c->S, L2 = c->L2;
+    const int M = c->Mt, W = c->W, S = c->S, L2 = c->L2;
     hipStream_t st = c->st;
     const bool mfma = q.metric == SDB_COSINE && M <= 128;
     const int prec = mfma ? q.precision : SDB_FP32;
@@ -711,6 +754,25 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
     HIPCHK(hipMemsetAsync(c->stats64.p, 0, 64, st));
     LCHK(launch_recheck_k(da, P_<int>(c->topK), q.topK, q.n_proj, prec, band32, nq, (unsigned long long*)c->stats64.p, st));
 
+    *nq_out = nq;
+    *band_out = band;
+    *band32_out = band32;
+    return SDB_OK;
+}
+
+int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* degree_out) {
+    if (!c) return SDB_EINVAL;
+    if (!c->indexed || c->sng) return fail(c, SDB_ESTATE, "build_index first");
+    HIPCHK(hipSetDevice(c->device));
+    const sdb_params& q = c->prm;
+    const long long n = c->n;
+    const int M = c->Mt, W = c->W, S = c->S, L2 = c->L2;
+    hipStream_t st = c->st;
+    int nq = 0;
+    double band = 0, band32 = 0;
+    int r = run_distances(c, eps, &nq, &band, &band32);
+    if (r) return r;
+
     // degrees and core flags
     ENSURE(c->degree, (size_t)n * 4);
     ENSURE(c->core, (size_t)n);
@@ -748,7 +810,7 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
 static ClusterArgs cluster_args(sdb_ctx* c) {
     const sdb_params& q = c->prm;
     ClusterArgs ca{};
-    ca.n = c->n; ca.S = c->S; ca.W = c->W; ca.M = q.topM; ca.P = q.n_proj; ca.K = q.topK; ca.L2 = c->L2;
+    ca.n = c->n; ca.S = c->S; ca.W = c->W; ca.M = c->Mt; ca.P = q.n_proj; ca.K = q.topK; ca.L2 = c->L2;
     ca.topK = P_<int>(c->topK); ca.lists = P_<int>(c->lists); ca.E = P_<uint32_t>(c->E);
     ca.colsum = P_<int>(c->colsum); ca.hub_id = P_<int>(c->hub_id); ca.hub_ptr = P_<int>(c->hub_ptr);
     ca.hub_mem = P_<int>(c->hub_mem); ca.inv = P_<uint32_t>(c->inv); ca.minPts = c->minPts;
@@ -764,7 +826,8 @@ int sdb_form_clusters(sdb_ctx* c, int* labels_out, int* n_clusters_out) {
     HIPCHK(hipSetDevice(c->device));
     hipStream_t st = c->st;
     const long long n = c->n;
-    const int M = c->prm.topM, W = c->W, L2 = c->L2;
+    const int M = c->Mt, W = c->W, L2 = c->L2;
+    ENSURE(c->total, 16);
     ENSURE(c->corehub, (size_t)L2 * W * 4);
     ENSURE(c->parent, (size_t)n * 4);
     ENSURE(c->border, (size_t)n * 4);
@@ -774,20 +837,40 @@ int sdb_form_clusters(sdb_ctx* c, int* labels_out, int* n_clusters_out) {
     ENSURE(c->rank, (size_t)n * 4);
     ClusterArgs ca = cluster_args(c);
     HIPCHK(hipEventRecord(c->ev[0], st));
-    HIPCHK(hipMemsetAsync(c->adj.p, 0, (size_t)L2 * M * W * 4, st));
-    LCHK(launch_corehub(ca, st));
-    LCHK(launch_uf_init(ca, st));
-    LCHK(launch_uf_slots(ca, st));
-    LCHK(launch_uf_lists(ca, st));
-    LCHK(launch_uf_points(ca, st));
-    LCHK(launch_uf_compress(ca, st));
-    LCHK(launch_root_flags(ca, P_<int>(c->rootf), st));
-    LCHK(scan_exclusive_i32(P_<int>(c->rootf), P_<int>(c->rank), n, P_<int>(c->total), c->scan_tmp.p, c->scan_tmp.bytes, st));
-    LCHK(launch_border(ca, st));
-    LCHK(launch_labels(ca, P_<int>(c->rank), st));
+    if (c->sng) {
+        // the hits are the edges of the neighbourhood graph
+        const int L = c->sngL, S2 = c->S, Ms = c->Mt;
+        LCHK(launch_uf_init(ca, st));
+        LCHK(launch_sng_uf(n, L, S2, Ms, W, P_<uint32_t>(c->E), P_<int>(c->lists), P_<uint8_t>(c->core),
+                           P_<int>(c->parent), st));
+        LCHK(launch_uf_compress(ca, st));
+        LCHK(launch_root_flags(ca, P_<int>(c->rootf), st));
+        ENSURE(c->scan_tmp, scan_tmp_bytes(n));
+        LCHK(scan_exclusive_i32(P_<int>(c->rootf), P_<int>(c->rank), n, P_<int>(c->total), c->scan_tmp.p, c->scan_tmp.bytes, st));
+        LCHK(launch_sng_border(n, L, S2, Ms, W, P_<uint32_t>(c->E), P_<int>(c->lists), P_<uint8_t>(c->core),
+                               P_<int>(c->parent), P_<int>(c->border), st));
+        LCHK(launch_labels(ca, P_<int>(c->rank), st));
+    } else {
+        HIPCHK(hipMemsetAsync(c->adj.p, 0, (size_t)L2 * M * W * 4, st));
+        LCHK(launch_corehub(ca, st));
+        LCHK(launch_uf_init(ca, st));
+        LCHK(launch_uf_slots(ca, st));
+        LCHK(launch_uf_lists(ca, st));
+        LCHK(launch_uf_points(ca, st));
+        LCHK(launch_uf_compress(ca, st));
+        LCHK(launch_root_flags(ca, P_<int>(c->rootf), st));
+        LCHK(scan_exclusive_i32(P_<int>(c->rootf), P_<int>(c->rank), n, P_<int>(c->total), c->scan_tmp.p, c->scan_tmp.bytes, st));
+        LCHK(launch_border(ca, st));
+        LCHK(launch_labels(ca, P_<int>(c->rank), st));
+    }
     HIPCHK(hipEventRecord(c->ev[1], st));
     int comps = 0;
-    HIPCHK(hipMemcpyAsync(&comps, c->total.p, 4, hipMemcpyDeviceToHost, st));
+    {
+        const hipError_t e = hipMemcpyAsync(&comps, c->total.p, 4, hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess)
+            return fail(c, SDB_EHIP, "component count read failed: %s; device %s (%zu bytes); host %s",
+                        hipGetErrorString(e), ptr_desc(c->total.p).c_str(), c->total.bytes, ptr_desc(&comps).c_str());
+    }
     if (labels_out) HIPCHK(hipMemcpyAsync(labels_out, c->labels.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     c->stats.t_cluster_ms = elapsed(c->ev[0], c->ev[1]);
@@ -800,6 +883,7 @@ int sdb_form_clusters(sdb_ctx* c, int* labels_out, int* n_clusters_out) {
 int sdb_label_noise(sdb_ctx* c, const int* sampled_core, int n_sampled, int* labels_out) {
     if (!c) return SDB_EINVAL;
     if (!c->clustered) return fail(c, SDB_ESTATE, "form_clusters first");
+    if (c->sng) return fail(c, SDB_ESTATE, "noise labelling needs the random-projection index");
     if (n_sampled < 0 || (n_sampled > 0 && !sampled_core)) return fail(c, SDB_EINVAL, "bad sampled core list");
     HIPCHK(hipSetDevice(c->device));
     const sdb_params& q = c->prm;
@@ -849,8 +933,11 @@ int sdb_label_noise(sdb_ctx* c, const int* sampled_core, int n_sampled, int* lab
     return SDB_OK;
 }
 
+static int build_neighbors_sng(sdb_ctx* c);
+
 static int build_neighbors(sdb_ctx* c) {
     if (c->have_nb) return 0;
+    if (c->sng) return build_neighbors_sng(c);
     hipStream_t st = c->st;
     const long long n = c->n;
     ENSURE(c->nb_ptr, (size_t)(n + 1) * 8);
@@ -963,6 +1050,190 @@ int sdb_optics_reach(sdb_ctx* c, const int* ordering, float* reach_out) {
                       P_<float>(c->core_dist), P_<int>(c->pos), P_<uint32_t>(c->reach), st));
     HIPCHK(hipMemcpyAsync(reach_out, c->reach.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return SDB_OK;
+}
+
+// transformData (src/Utilities.h:289-335) on its own, for the paths that do
+// not build the index (the index passes fuse it for Cosine).
+static int ensure_transformed(sdb_ctx* c) {
+    const sdb_params& q = c->prm;
+    if (c->transformed && (q.precision != SDB_BF16 || c->transformed_prec == SDB_BF16)) return 0;
+    hipStream_t st = c->st;
+    const long long n = c->n;
+    ENSURE(c->Xn, (size_t)n * c->ldx * 4);
+    if (q.precision == SDB_BF16) ENSURE(c->Xb, (size_t)n * c->ldx * 2);
+    ENSURE(c->err_flag, 4);
+    HIPCHK(hipMemsetAsync(c->err_flag.p, 0, 4, st));
+    LCHK(launch_transform(c->X, n, c->d, c->ldx, q.metric, P_<float>(c->Xn),
+                          q.precision == SDB_BF16 ? P_<uint16_t>(c->Xb) : nullptr, P_<int>(c->err_flag), st));
+    int errf = 0;
+    HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
+    if (errf & 2) return fail(c, SDB_EZERO, "Error: There is an zero point !");
+    c->transformed = true;
+    c->transformed_prec = q.precision;
+    return 0;
+}
+
+// Deduplicated neighbourhood CSR of the sample-sequence hits (sorted by index)
+// with FP32 distances.
+static int build_neighbors_sng(sdb_ctx* c) {
+    if (c->have_nb) return 0;
+    hipStream_t st = c->st;
+    const long long n = c->n;
+    const int L = c->sngL, S2 = c->S, Ms = c->Mt, W = c->W;
+    const int both = c->sng_flags & 1;
+    ENSURE(c->csr_cnt, (size_t)(n + 1) * 8);
+    ENSURE(c->csr_ptr, (size_t)(n + 1) * 8);
+    HIPCHK(hipMemsetAsync(c->csr_cnt.p, 0, (size_t)(n + 1) * 8, st));
+    LCHK(launch_sng_csr_count(n, L, S2, Ms, W, both, P_<uint32_t>(c->E), P_<int>(c->lists), P_<long long>(c->csr_cnt),
+                              st));
+    std::vector<long long> cnt(n), ptr(n + 1, 0);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->csr_cnt.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (long long a = 0; a < n; ++a) ptr[a + 1] = ptr[a] + cnt[a];
+    const long long tot = ptr[n];
+    if (tot >= (1ll << 31)) return fail(c, SDB_EUNSUPPORTED, "%lld neighbourhood entries", tot);
+    HIPCHK(hipMemcpyAsync(c->csr_ptr.p, ptr.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, st));
+    ENSURE(c->csr_cur, (size_t)n * 8);
+    HIPCHK(hipMemsetAsync(c->csr_cur.p, 0, (size_t)n * 8, st));
+    ENSURE(c->csr_idx, (size_t)std::max<long long>(tot, 1) * 4);
+    ENSURE(c->nb_sorted, (size_t)std::max<long long>(tot, 1) * 8);
+    LCHK(launch_sng_csr_fill(n, L, S2, Ms, W, both, P_<uint32_t>(c->E), P_<int>(c->lists), P_<long long>(c->csr_ptr),
+                             (unsigned long long*)c->csr_cur.p, P_<int>(c->csr_idx), st));
+    int* sorted = (int*)c->nb_sorted.p;
+    if (tot > 0) {
+        size_t tb = 0;
+        hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, P_<int>(c->csr_idx), sorted, (int)tot, (int)n,
+                                                   P_<long long>(c->csr_ptr), P_<long long>(c->csr_ptr) + 1, 0, 32, st);
+        DevBuf tmp;
+        if (ensure(c, tmp, tb)) return SDB_ENOMEM;
+        hipError_t e = hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, tb, P_<int>(c->csr_idx), sorted, (int)tot, (int)n,
+                                                                  P_<long long>(c->csr_ptr), P_<long long>(c->csr_ptr) + 1, 0, 32, st);
+        HIPCHK(hipStreamSynchronize(st));
+        hipFree(tmp.p);
+        if (e != hipSuccess) return fail(c, SDB_EHIP, "segmented sort failed: %s", hipGetErrorString(e));
+    }
+    // distinct ids per row
+    LCHK(launch_uniq(n, P_<long long>(c->csr_ptr), sorted, P_<long long>(c->csr_cnt), st));
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->csr_cnt.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::fill(ptr.begin(), ptr.end(), 0);
+    for (long long a = 0; a < n; ++a) ptr[a + 1] = ptr[a] + cnt[a];
+    const long long utot = ptr[n];
+    ENSURE(c->nb_ptr, (size_t)(n + 1) * 8);
+    HIPCHK(hipMemcpyAsync(c->nb_ptr.p, ptr.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, st));
+    ENSURE(c->nb_idx, (size_t)std::max<long long>(utot, 1) * 4);
+    ENSURE(c->nb_dist, (size_t)std::max<long long>(utot, 1) * 4);
+    LCHK(launch_uniq_fill(n, P_<long long>(c->csr_ptr), sorted, P_<long long>(c->nb_ptr), P_<int>(c->nb_idx), st));
+    if (utot > 0)
+        LCHK(launch_csr_dist(P_<float>(c->Xn), c->ldx, c->d, c->prm.metric, n, P_<long long>(c->nb_ptr),
+                             P_<int>(c->nb_idx), P_<float>(c->nb_dist), st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->have_nb = true;
+    return 0;
+}
+
+int sdb_sng_find_core(sdb_ctx* c, const int* samples, int n_samples, const int* win_end, int flags, float eps,
+                      int minPts, uint8_t* core_out, int* degree_out) {
+    if (!c) return SDB_EINVAL;
+    if (!c->X) return fail(c, SDB_ESTATE, "no input loaded");
+    if (n_samples < 0 || (n_samples > 0 && !samples)) return fail(c, SDB_EINVAL, "bad sample sequence");
+    HIPCHK(hipSetDevice(c->device));
+    const long long n = c->n;
+    hipStream_t st = c->st;
+    for (int j = 0; j < n_samples; ++j)
+        if (samples[j] < 0 || samples[j] >= n) return fail(c, SDB_EINVAL, "sample %d out of range", samples[j]);
+    const bool both = (flags & 1) != 0, dedup = (flags & 2) != 0;
+    int r = ensure_transformed(c);
+    if (r) return r;
+
+    // chunks of the sequence act as lists of Ms points
+    const int Ms = c->sng_Ms, W = (Ms + 31) / 32;
+    const int L = std::max(1, (n_samples + Ms - 1) / Ms);
+    int S2 = 1;
+    while (S2 < L) S2 <<= 1;
+    if ((long long)n * S2 >= (1ll << 31)) return fail(c, SDB_EUNSUPPORTED, "n_points x sample chunks too large");
+    std::vector<int> seq((size_t)L * Ms, n_samples > 0 ? samples[0] : 0);
+    std::copy(samples, samples + n_samples, seq.begin());
+    ENSURE(c->lists, seq.size() * 4);
+    HIPCHK(hipMemcpyAsync(c->lists.p, seq.data(), seq.size() * 4, hipMemcpyHostToDevice, st));
+    // positions of every point in the sequence (the point itself is skipped)
+    std::vector<int> optr(n + 1, 0), opos(std::max(n_samples, 1));
+    for (int j = 0; j < n_samples; ++j) ++optr[samples[j] + 1];
+    for (long long a = 0; a < n; ++a) optr[a + 1] += optr[a];
+    {
+        std::vector<int> cur(optr.begin(), optr.end() - 1);
+        for (int j = 0; j < n_samples; ++j) opos[cur[samples[j]]++] = j;
+    }
+    ENSURE(c->occ_ptr, (size_t)(n + 1) * 4);
+    ENSURE(c->occ_pos, opos.size() * 4);
+    HIPCHK(hipMemcpyAsync(c->occ_ptr.p, optr.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->occ_pos.p, opos.data(), opos.size() * 4, hipMemcpyHostToDevice, st));
+    if (win_end) {
+        ENSURE(c->win, (size_t)n * 4);
+        HIPCHK(hipMemcpyAsync(c->win.p, win_end, (size_t)n * 4, hipMemcpyHostToDevice, st));
+    }
+    ENSURE(c->F, (size_t)n * S2 * W * 4);
+    LCHK(launch_sng_fmask(n, L, S2, Ms, W, n_samples, win_end ? P_<int>(c->win) : nullptr, P_<int>(c->occ_ptr),
+                          P_<int>(c->occ_pos), P_<uint32_t>(c->F), st));
+    ENSURE(c->grp, (size_t)n * L * 4);
+    LCHK(launch_sng_grp(n, L, S2, P_<uint32_t>(c->grp), st));
+    c->h_grp_ptr.resize(L + 1);
+    for (int l = 0; l <= L; ++l) c->h_grp_ptr[l] = (int)((long long)l * n);
+    c->nblk = 1;
+    c->L2 = L;
+    c->S = S2;
+    c->W = W;
+    c->Mt = Ms;
+    c->sngL = L;
+    c->sng = true;
+    c->sng_flags = flags;
+    c->indexed = false;
+    c->have_nb = false;
+    c->clustered = false;
+    c->stats = sdb_stats{};
+
+    int nq = 0;
+    double band = 0, band32 = 0;
+    r = run_distances(c, eps, &nq, &band, &band32);
+    if (r) return r;
+    ENSURE(c->degree, (size_t)n * 4);
+    ENSURE(c->core, (size_t)n);
+    unsigned long long* ncore = (unsigned long long*)c->stats64.p + 3;
+    if (!dedup) {
+        // sngDbscan: |N(a)| with duplicates (src/sDbscan.cpp:487-501)
+        ENSURE(c->rev, (size_t)n * 4);
+        HIPCHK(hipMemsetAsync(c->rev.p, 0, (size_t)n * 4, st));
+        if (both) LCHK(launch_sng_rev(n_samples, P_<int>(c->lists), P_<int>(c->colsum), P_<int>(c->rev), st));
+        LCHK(launch_sng_degree(n, L, S2, W, P_<uint32_t>(c->E), P_<int>(c->rev), minPts, P_<int>(c->degree),
+                               P_<uint8_t>(c->core), ncore, st));
+    } else {
+        // sngOptics: deduplicated neighbourhoods (src/sDbscan.cpp:1528-1540)
+        r = build_neighbors_sng(c);
+        if (r) return r;
+        LCHK(launch_rowsize_core(n, P_<long long>(c->nb_ptr), minPts, P_<int>(c->degree), P_<uint8_t>(c->core), ncore,
+                                 st));
+    }
+    HIPCHK(hipEventRecord(c->ev[6], st));
+    unsigned long long hs[8] = {0};
+    HIPCHK(hipMemcpyAsync(hs, c->stats64.p, 64, hipMemcpyDeviceToHost, st));
+    if (core_out) HIPCHK(hipMemcpyAsync(core_out, c->core.p, (size_t)n, hipMemcpyDeviceToHost, st));
+    if (degree_out) HIPCHK(hipMemcpyAsync(degree_out, c->degree.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->stats.t_core_ms = elapsed(c->ev[3], c->ev[6]);
+    c->stats.n_pairs = (long long)n * n_samples;
+    c->stats.n_queue = nq;
+    c->stats.n_recheck32 = (long long)hs[0];
+    c->stats.n_recheck64 = (long long)hs[1];
+    c->stats.n_recheck_hits = (long long)hs[2];
+    c->stats.n_core = (long long)hs[3];
+    c->stats.band = band;
+    c->stats.band32 = band32;
+    c->eps = eps;
+    c->minPts = minPts;
+    c->cored = true;
     return SDB_OK;
 }
 

@@ -37,6 +37,7 @@ def test_dropin_module_surface():
     for name in ("set_params", "set_sngParams", "set_kernel_params", "clear", "set_topM",
                  "set_topK", "set_threads", "set_clusterNoise", "fit_sDbscan", "test_sDbscan",
                  "fit_sOptics", "load_fit_sDbscan", "load_test_sDbscan", "load_fit_sOptics",
+                 "fit_sngDbscan", "test_sngDbscan", "load_test_sngDbscan", "fit_sngOptics", "load_fit_sngOptics",
                  "labels_", "n_clusters_", "ordering_", "reachability_"):
         assert hasattr(cls, name), name
     doc = cls.set_params.__doc__

@@ -285,6 +285,116 @@ def test_dropin_module_text_loader(ctx, tmp_path):
     assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
 
 
+def std_uniform_ints(seed, N, count):
+    """std::default_random_engine (minstd_rand0) + libstdc++
+    uniform_int_distribution<int>(0, N - 1), published algorithms."""
+    m = 2147483647
+    x = seed % m or 1
+    urange = (m - 1) - 1
+    scaling = urange // N
+    past = N * scaling
+    out = []
+    while len(out) < count:
+        while True:
+            x = (16807 * x) % m
+            r = x - 1
+            if r < past:
+                break
+        out.append(r // scaling)
+    return np.array(out, np.int32)
+
+
+def sng_windows(seq, n, S):
+    """Window end per point: the shortest prefix holding S draws other than
+    the point (sng_findCorePoints' while loop)."""
+    win = np.full(n, S, np.int32)
+    for v in np.unique(seq):
+        pos = np.where(seq == v)[0]
+        c = 0
+        while c < len(pos) and pos[c] < S + c:
+            c += 1
+        win[v] = S + c
+    return win
+
+
+def test_sng_exact_matches_oracle(ctx):
+    n = 3000
+    X, _ = datagen.mixture(n, 64, k=5, sigma=0.3, seed=101)
+    print("sng exact n=3000 d=64", flush=True)
+    ctx.set_params(n, 64, n_proj=64, topK=4, topM=20, metric=S.COSINE)
+    ctx.load(X)
+    core, deg = ctx.sng_find_core(np.arange(n), 0.2, 15, both_ways=False)
+    lab, nc = ctx.form_clusters()
+    Xt = ctx.get_transformed()
+    oc, od, (ptr, idx, dd) = O.sng_find_core(Xt, S.COSINE, np.arange(n), 0.2, 15, both_ways=False)
+    olab, onc = O.form_cluster(oc, ptr, idx)
+    print("cores", int(core.sum()), "clusters", nc, flush=True)
+    assert np.array_equal(deg, od) and np.array_equal(core, oc)
+    assert np.array_equal(lab, olab) and nc == onc
+
+
+@pytest.mark.parametrize("precision", [S.FP32, S.BF16])
+def test_sng_sampled_matches_oracle(ctx, precision):
+    n, S_ = 5000, 500
+    X, _ = datagen.mixture(n, 128, k=5, sigma=0.3, seed=103)
+    seq = std_uniform_ints(11, n, S_ + 64)
+    seq[10] = seq[20] = 7            # repeated draws
+    seq[30] = 30                     # a point drawing itself
+    win = sng_windows(seq, n, S_)
+    print("sng sampled n=5000 d=128 samples=500 precision", precision, flush=True)
+    ctx.set_params(n, 128, n_proj=128, topK=4, topM=20, metric=S.COSINE, precision=precision)
+    ctx.load(X)
+    core, deg = ctx.sng_find_core(seq, 0.3, 12, win_end=win)
+    lab, nc = ctx.form_clusters()
+    Xt = ctx.get_transformed()
+    oc, od, (ptr, idx, dd) = O.sng_find_core(Xt, S.COSINE, seq, 0.3, 12, win_end=win)
+    olab, onc = O.form_cluster(oc, ptr, idx)
+    print("cores", int(core.sum()), "clusters", nc, "queue", ctx.stats()["n_queue"], flush=True)
+    assert np.array_equal(deg, od) and np.array_equal(core, oc)
+    assert np.array_equal(lab, olab) and nc == onc
+
+
+def test_sng_optics_matches_oracle(ctx):
+    n = 4000
+    X, _ = datagen.mixture(n, 96, k=4, sigma=0.3, seed=105)
+    seq = std_uniform_ints(5, n, 400)
+    ctx.set_params(n, 96, n_proj=128, topK=4, topM=20, metric=S.COSINE)
+    ctx.load(X)
+    eps, minPts = 0.3, 10
+    core, deg = ctx.sng_find_core(seq, eps, minPts, dedup=True)
+    ptr, idx, dist = ctx.get_neighbors()
+    Xt = ctx.get_transformed()
+    oc, od, (optr, oidx, odd) = O.sng_find_core(Xt, S.COSINE, seq, eps, minPts, dedup=True)
+    assert np.array_equal(core, oc) and np.array_equal(deg, od)
+    assert np.array_equal(ptr.astype(np.int32), optr) and np.array_equal(idx, oidx)
+    assert np.max(np.abs(dist - odd)) < 1e-5
+    cd = ctx.core_dist(minPts)
+    occ, ocd = O.core_dist(minPts, ptr.astype(np.int32), dist)
+    assert np.array_equal(cd, ocd)
+    order, reach = O.optics(core, cd, ptr.astype(np.int32), idx, dist)
+    assert np.array_equal(ctx.optics_reach(order), reach)
+
+
+def test_dropin_module_sng(ctx):
+    m = S.module()
+    n = 3000
+    X, _ = datagen.mixture(n, 64, k=4, sigma=0.3, seed=107)
+    db = m.sDbscan(n, 64)
+    db.set_params(distance="Cosine", samplingProb=0.1, random_seed=3)
+    db.fit_sngDbscan(X.T, 0.25, 10)
+    # the same draw sequence through the C ABI
+    S_ = int(np.ceil(1.0 * n * float(np.float32(0.1))))   # ceil(1.0 * n * prob) with prob a float
+    seq = std_uniform_ints(3, n, S_ + 64)
+    win = sng_windows(seq, n, S_)
+    ctx.set_params(n, 64, n_proj=64, topK=4, topM=20, metric=S.COSINE)
+    ctx.load(X)
+    ctx.sng_find_core(seq, 0.25, 10, win_end=win)
+    lab, nc = ctx.form_clusters()
+    assert np.array_equal(np.array(db.labels_), lab) and db.n_clusters_ == nc
+    db.fit_sngOptics(X.T, 0.25, 10)
+    assert sorted(db.ordering_) == list(range(n))
+
+
 def test_dropin_module_fit(ctx):
     m = S.module()
     X, _ = datagen.mixture(4000, 784, seed=71)

@@ -282,3 +282,39 @@ def test_label_noise_takes_the_best_estimate_and_first_on_ties():
     assert out[a0] == 7
     assert not np.any(out[noise] == 30)
     assert np.array_equal(out[sampled], labels[sampled])
+
+
+# ------------------------------------------------------------ sngDbscan
+def test_sng_exact_branch_is_exact_dbscan():
+    # sampling probability 1 (src/sDbscan.cpp:466-478): every point against
+    # every other point, own pushes only -> exact DBSCAN
+    from sklearn.cluster import DBSCAN
+    X, _ = datagen.mixture(500, 40, k=5, sigma=0.25, seed=5)
+    Xt, _ = O.transform(X, COSINE)
+    eps, minPts = 0.12, 8
+    core, deg, (ptr, idx, dd) = O.sng_find_core(Xt, COSINE, np.arange(500), eps, minPts, both_ways=False)
+    D = 1.0 - Xt.astype(np.float64) @ Xt.astype(np.float64).T
+    np.fill_diagonal(D, np.inf)
+    assert np.array_equal(deg, (D <= eps).sum(1))
+    sk = DBSCAN(eps=eps, min_samples=minPts, metric="precomputed").fit(np.where(np.isinf(D), 0, D))
+    lab, nc = O.form_cluster(core, ptr, idx)
+    assert np.array_equal(lab, sk.labels_)
+
+
+def test_sng_sampled_counts_duplicates():
+    # a sequence that repeats a point: both its draws count in sngDbscan
+    # (no dedup) and once in sngOptics (dedup); self draws are skipped
+    X, _ = datagen.mixture(60, 16, k=2, sigma=0.05, seed=6)
+    Xt, _ = O.transform(X, COSINE)
+    D = 1.0 - Xt.astype(np.float64) @ Xt.astype(np.float64).T
+    seq = np.array([3, 3, 7, 0], np.int32)
+    eps = float(np.sort(D[0])[30])
+    _, deg, _ = O.sng_find_core(Xt, COSINE, seq, eps, 2)
+    _, degd, _ = O.sng_find_core(Xt, COSINE, seq, eps, 2, dedup=True)
+    hit = (D <= eps)
+    for p in range(60):
+        own = sum(1 for q in seq if q != p and hit[p, q])
+        rev = sum(hit[m, p] for m in range(60) for q in seq if q == p and m != p)
+        assert deg[p] == own + rev
+        nb = {int(q) for q in seq if q != p and hit[p, q]} | {m for m in range(60) if m != p and p in seq and hit[m, p]}
+        assert degd[p] == len(nb)
