@@ -370,7 +370,8 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     }
     HIPCHK(hipEventRecord(c->ev[1], st));
 
-    // ---- random structures: per-lane sign masks of the 3 x D sign bits
+    // ---- random structures: per-lane sign masks of the 3 x D sign bits (lane
+    // p holds elements lane_logical(p) + 64 j in the index kernels)
     const int VPL = D / 64, NWs = (VPL + 31) / 32;
     std::vector<uint32_t> smask((size_t)3 * NWs * 64, 0u);
     for (int r = 0; r < 3; ++r)
@@ -378,7 +379,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
             for (int lane = 0; lane < 64; ++lane) {
                 uint32_t m = 0;
                 for (int b = 0; b < 32 && w * 32 + b < VPL; ++b)
-                    m |= (uint32_t)(hd3_bits[(size_t)r * D + lane + 64 * (w * 32 + b)] & 1u) << b;
+                    m |= (uint32_t)(hd3_bits[(size_t)r * D + lane_logical(lane) + 64 * (w * 32 + b)] & 1u) << b;
                 smask[((size_t)r * NWs + w) * 64 + lane] = m;
             }
     ENSURE(c->bits, smask.size() * 4);

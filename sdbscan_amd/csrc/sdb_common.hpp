@@ -68,19 +68,20 @@ __device__ __forceinline__ uint32_t f32_to_bf16_rne(float f) {
 // Cross-lane exchange with lane ^ S without the LDS crossbar: DPP quad
 // permutes for S = 1, 2; two DPP row permutes (half-mirror / mirror, then a
 // quad or half-mirror permute) for S = 4, 8; the gfx950 permlane swaps for
-// S = 16, 32.
+// S = 16, 32.  Every lane has a valid DPP source, so bound_ctrl leaves no
+// lane to an old value and no old value is materialised.
 template <int S>
 __device__ __forceinline__ uint32_t xchg(uint32_t v, int lane) {
     const int iv = (int)v;
-    if constexpr (S == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, iv, 0xB1, 0xF, 0xF, false);
-    if constexpr (S == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, iv, 0x4E, 0xF, 0xF, false);
+    if constexpr (S == 1) return (uint32_t)__builtin_amdgcn_mov_dpp(iv, 0xB1, 0xF, 0xF, true);
+    if constexpr (S == 2) return (uint32_t)__builtin_amdgcn_mov_dpp(iv, 0x4E, 0xF, 0xF, true);
     if constexpr (S == 4) {
-        const int h = __builtin_amdgcn_update_dpp(0, iv, 0x141, 0xF, 0xF, false);
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, h, 0x1B, 0xF, 0xF, false);
+        const int h = __builtin_amdgcn_mov_dpp(iv, 0x141, 0xF, 0xF, true);
+        return (uint32_t)__builtin_amdgcn_mov_dpp(h, 0x1B, 0xF, 0xF, true);
     }
     if constexpr (S == 8) {
-        const int m = __builtin_amdgcn_update_dpp(0, iv, 0x140, 0xF, 0xF, false);
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, m, 0x141, 0xF, 0xF, false);
+        const int m = __builtin_amdgcn_mov_dpp(iv, 0x140, 0xF, 0xF, true);
+        return (uint32_t)__builtin_amdgcn_mov_dpp(m, 0x141, 0xF, 0xF, true);
     }
     if constexpr (S == 16) {
         const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
@@ -91,6 +92,27 @@ __device__ __forceinline__ uint32_t xchg(uint32_t v, int lane) {
         return (lane & 32) ? p[0] : p[1];
     }
     return v;
+}
+
+// Mirror lane layout of the index kernels.  Physical lane p holds logical
+// element lane_logical(p): bits 4-5 are kept and the low nibble is mapped by
+// the GF(2)-linear bijection with logical bit 0 -> p ^ 1, bit 1 -> p ^ 2,
+// bit 2 -> p ^ 7 (DPP half-mirror), bit 3 -> p ^ 15 (DPP mirror), so the
+// exchange with logical partner lane ^ S is a single DPP move for every S < 16.
+__host__ __device__ constexpr int lane_logical(int p) {
+    return (p & 48) | (((p ^ (p >> 2)) & 1)) | ((((p >> 1) ^ (p >> 2)) & 1) << 1) |
+           ((((p >> 2) ^ (p >> 3)) & 1) << 2) | (((p >> 3) & 1) << 3);
+}
+
+// Exchange with the lane holding logical element lg ^ S (mirror layout).
+template <int S>
+__device__ __forceinline__ uint32_t xchg_l(uint32_t v, int lg) {
+    const int iv = (int)v;
+    if constexpr (S == 1) return (uint32_t)__builtin_amdgcn_mov_dpp(iv, 0xB1, 0xF, 0xF, true);
+    if constexpr (S == 2) return (uint32_t)__builtin_amdgcn_mov_dpp(iv, 0x4E, 0xF, 0xF, true);
+    if constexpr (S == 4) return (uint32_t)__builtin_amdgcn_mov_dpp(iv, 0x141, 0xF, 0xF, true);
+    if constexpr (S == 8) return (uint32_t)__builtin_amdgcn_mov_dpp(iv, 0x140, 0xF, 0xF, true);
+    return xchg<S>(v, lg);  // bits 4 and 5 of lg are those of the physical lane
 }
 
 }  // namespace sdb

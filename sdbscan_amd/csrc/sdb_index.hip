@@ -55,15 +55,16 @@ __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, 
     }
 }
 
-// One Walsh-Hadamard level across lanes (stride S < 64): the lower element
-// becomes a + b, the upper a - b, computed as fma(+-1, mine, partner) with an
-// exact product, i.e. one rounding of partner +- mine.
+// One Walsh-Hadamard level across lanes (logical stride S < 64, mirror lane
+// layout, lg = logical index of the lane): the lower element becomes a + b,
+// the upper a - b, computed as fma(+-1, mine, partner) with an exact product,
+// i.e. one rounding of partner +- mine.
 template <int S, int VPL>
-__device__ __forceinline__ void fwht_cross(float (&v)[VPL], int lane) {
-    const float sg = (lane & S) ? -1.0f : 1.0f;
+__device__ __forceinline__ void fwht_cross(float (&v)[VPL], int lg) {
+    const float sg = (lg & S) ? -1.0f : 1.0f;
 #pragma unroll
     for (int j = 0; j < VPL; ++j) {
-        const float o = __uint_as_float(xchg<S>(__float_as_uint(v[j]), lane));
+        const float o = __uint_as_float(xchg_l<S>(__float_as_uint(v[j]), lg));
         v[j] = fmaf(sg, v[j], o);
     }
 }
@@ -112,23 +113,24 @@ __device__ __forceinline__ void fwht_inreg(float (&v)[VPL], int js) {
 }
 
 // Unnormalised FWHT with increasing strides (the level order of FFHT,
-// src/fht_avx.c:1013-1251).  Element e lives in lane (e & 63), register e >> 6.
+// src/fht_avx.c:1013-1251).  Element e lives in register e >> 6 of the lane
+// whose logical index (mirror layout, lane_logical) is e & 63.
 // Outputs equal FFHT's as IEEE values; FFHT's (0 - v) butterflies at strides
 // 1, 2, 4 can only change the sign of an exact zero, which no comparison of
 // the index distinguishes.
 template <int LOGD>
-__device__ __forceinline__ void ffht_wave(float (&v)[(1 << LOGD) / 64], int lane) {
+__device__ __forceinline__ void ffht_wave(float (&v)[(1 << LOGD) / 64], int lg) {
     constexpr int VPL = (1 << LOGD) / 64;
-    fwht_cross<1, VPL>(v, lane);
-    fwht_cross<2, VPL>(v, lane);
-    fwht_cross<4, VPL>(v, lane);
-    fwht_cross<8, VPL>(v, lane);
+    fwht_cross<1, VPL>(v, lg);
+    fwht_cross<2, VPL>(v, lg);
+    fwht_cross<4, VPL>(v, lg);
+    fwht_cross<8, VPL>(v, lg);
     if constexpr (VPL % 2 == 0) {
         fwht_cross_pairs<16, VPL>(v);
         fwht_cross_pairs<32, VPL>(v);
     } else {
-        fwht_cross<16, VPL>(v, lane);
-        fwht_cross<32, VPL>(v, lane);
+        fwht_cross<16, VPL>(v, lg);
+        fwht_cross<32, VPL>(v, lg);
     }
 #pragma unroll
     for (int js = 1; js < VPL; js <<= 1) fwht_inreg<VPL>(v, js);
@@ -187,9 +189,10 @@ __device__ __noinline__ void embed_lds(const IndexArgs& a, long long p, int lane
 }
 
 // HD3 rotation in registers (src/sDbscan.cpp:77-88): 3 x {sign flip, FWHT};
-// the (2 bit - 1) multiplier negates where the bit is 0.
+// the (2 bit - 1) multiplier negates where the bit is 0.  sg[r][w] bit b is
+// the sign bit of register 32 w + b in round r for this lane's elements.
 template <int LOGD>
-__device__ __forceinline__ void hd3_rotate(float (&v)[(1 << LOGD) / 64], int lane,
+__device__ __forceinline__ void hd3_rotate(float (&v)[(1 << LOGD) / 64], int lg,
                                            const uint32_t (&sg)[3][((1 << LOGD) / 64 + 31) / 32]) {
     constexpr int VPL = (1 << LOGD) / 64;
 #pragma unroll
@@ -199,7 +202,7 @@ __device__ __forceinline__ void hd3_rotate(float (&v)[(1 << LOGD) / 64], int lan
             const uint32_t bit = (sg[r][j >> 5] >> (j & 31)) & 1u;
             v[j] = __uint_as_float(__float_as_uint(v[j]) ^ ((bit ^ 1u) << 31));
         }
-        ffht_wave<LOGD>(v, lane);
+        ffht_wave<LOGD>(v, lg);
     }
 }
 
@@ -251,7 +254,8 @@ __device__ __forceinline__ int lanes_below(unsigned long long b) {
 // Winners are written to out[0..K); returns true when the (K+1)-th largest
 // key equals the K-th (a tie across the selection boundary).
 template <int VPL>
-__device__ __forceinline__ bool wave_topk_general(const uint32_t (&key)[VPL], int P, int K, int lane, int* out) {
+__device__ __forceinline__ bool wave_topk_general(const uint32_t (&key)[VPL], int P, int K, int lane, int lg,
+                                                  int* out) {
     // K + 1 rounds of a wave-wide argmax over the register keys (value desc,
     // index asc); the winner of a round is masked out in its lane.
     uint64_t removed = 0;
@@ -262,7 +266,7 @@ __device__ __forceinline__ bool wave_topk_general(const uint32_t (&key)[VPL], in
         int jb = -1;
 #pragma unroll
         for (int j = 0; j < VPL; ++j) {
-            const int e = lane + 64 * j;
+            const int e = lg + 64 * j;
             if (e < P && !((removed >> j) & 1ull)) {
                 const uint64_t c = ((uint64_t)key[j] << 32) | (uint32_t)(0xffffffffu - (uint32_t)e);
                 if (c > best) { best = c; jb = j; }
@@ -310,21 +314,10 @@ __device__ __forceinline__ void kth_bounds(uint32_t mc, uint32_t mf, int K, uint
     tf = pf;
 }
 
-__device__ __forceinline__ void rank_select(uint32_t* sh, int total, int K, int lane, int* out, bool& tie) {
-    // sh[2 i] = key, sh[2 i + 1] = ~index for i < total (total <= 64)
-    uint32_t h = 0u, lo = 0u;
-    if (lane < total) {
-        const uint2 e = ((const uint2*)sh)[lane];
-        h = e.x;
-        lo = e.y;
-    }
-    const uint64_t mine = ((uint64_t)h << 32) | lo;
-    int rank = 0;
-    for (int i = 0; i < total; ++i) {
-        const uint64_t o = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)h, i) << 32) |
-                           (uint32_t)__builtin_amdgcn_readlane((int)lo, i);
-        rank += o > mine ? 1 : 0;
-    }
+// Winners of one side from its ranks: rank r < K goes to out[r]; the tie flag
+// compares the K-th and (K+1)-th keys.
+__device__ __forceinline__ void rank_emit(uint32_t h, uint32_t lo, int rank, int total, int K, int lane, int* out,
+                                          bool& tie) {
     if (lane < total && rank < K) out[rank] = (int)~lo;
     tie = false;
     if (total > K) {
@@ -334,44 +327,105 @@ __device__ __forceinline__ void rank_select(uint32_t* sh, int total, int K, int 
         const uint32_t vn = (uint32_t)__builtin_amdgcn_readlane((int)h, __builtin_ctzll(bn));
         tie = vk == vn;
     }
+}
+
+// Ranks of the compacted candidates of both sides (at most 64 each; slot i
+// in lane i, as (key, ~index) with the far keys stored as close keys): the
+// rank of a candidate counts the candidates of its side ordered before it
+// (key desc, index asc).  One readlane loop serves both sides; slots past a
+// side's count hold (0, 0), which orders after every candidate.
+__device__ __forceinline__ void rank_select2(const uint2* shc, int nc, const uint2* shf, int nf, int K, int lane,
+                                             int* out_c, int* out_f, bool& tie_c, bool& tie_f) {
+    uint32_t hc = 0u, lc = 0u, hf = 0u, lf = 0u;
+    if (lane < nc) { const uint2 e = shc[lane]; hc = e.x; lc = e.y; }
+    if (lane < nf) { const uint2 e = shf[lane]; hf = ~e.x; lf = e.y; }
+    const uint64_t mc = ((uint64_t)hc << 32) | lc, mf = ((uint64_t)hf << 32) | lf;
+    int rc = 0, rf = 0;
+    const int nmax = nc > nf ? nc : nf;
+    for (int i = 0; i < nmax; ++i) {
+        const uint64_t oc = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hc, i) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)lc, i);
+        const uint64_t of = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hf, i) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)lf, i);
+        rc += oc > mc ? 1 : 0;
+        rf += of > mf ? 1 : 0;
+    }
+    rank_emit(hc, lc, rc, nc, K, lane, out_c, tie_c);
+    rank_emit(hf, lf, rf, nf, K, lane, out_f, tie_f);
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int VPL>
-__device__ __forceinline__ void wave_topk_both(const uint32_t (&kc)[VPL], int P, int K, int lane, uint32_t* lds,
-                                               int* out_close, int* out_far, bool& tie_c, bool& tie_f) {
-    // kc: close keys, 0 for elements at or beyond P; far key = ~kc (0 stays 0)
-    uint32_t kf[VPL];
+// Candidate compaction of both sides into LDS slots [0, 64); lanes without a
+// candidate in a register store to a slot of their own past 64, so the stores
+// need no branch (a count past 64 sends the side to the general selection,
+// which does not read the slots).  FULL: every register holds P elements.
+template <int VPL, bool FULL>
+__device__ __forceinline__ void compact_both(const uint32_t (&kc)[VPL], int P, int lane, int lg, uint32_t tc,
+                                             uint32_t tfn, uint32_t* shc, uint32_t* shf, int& nc_out, int& nf_out) {
+    int nc = 0, nf = 0;
+    const uint32_t dummy = (uint32_t)(64 + lane);
 #pragma unroll
-    for (int j = 0; j < VPL; ++j) kf[j] = kc[j] ? ~kc[j] : 0u;
-    uint32_t mc = 0u, mf = 0u;
+    for (int j = 0; j < VPL; ++j) {
+        const uint32_t ne = ~(uint32_t)(lg + 64 * j);
+        bool cc, cf;
+        if constexpr (FULL) {
+            cc = kc[j] >= tc;
+            cf = kc[j] <= tfn;
+        } else {
+            const bool live = 64 * j + 63 < P || kc[j] != 0u;
+            cc = live && kc[j] >= tc;
+            cf = live && kc[j] <= tfn;
+        }
+        const unsigned long long bc = __ballot(cc), bf = __ballot(cf);
+        const uint32_t pc = __builtin_amdgcn_mbcnt_hi((uint32_t)(bc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bc, (uint32_t)nc));
+        const uint32_t pf = __builtin_amdgcn_mbcnt_hi((uint32_t)(bf >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bf, (uint32_t)nf));
+        const uint32_t sc = cc ? (pc & 63u) : dummy;
+        const uint32_t sf = cf ? (pf & 63u) : dummy;
+        shc[2 * sc] = kc[j];
+        shc[2 * sc + 1] = ne;
+        shf[2 * sf] = kc[j];
+        shf[2 * sf + 1] = ne;
+        nc += (int)__popcll(bc);
+        nf += (int)__popcll(bf);
+    }
+    nc_out = nc;
+    nf_out = nf;
+}
+
+template <int VPL>
+__device__ __forceinline__ void wave_topk_both(const uint32_t (&kc)[VPL], int P, int K, int lane, int lg, uint32_t* lds,
+                                               int* out_close, int* out_far, bool& tie_c, bool& tie_f) {
+    // kc: close keys, 0 for elements at or beyond P; the far key of a valid
+    // element is ~kc, so far comparisons run on kc with complemented bounds.
+    const bool full = P >= VPL * 64;
+    uint32_t mc = 0u, mn = 0xffffffffu;
 #pragma unroll
     for (int j = 0; j < VPL; ++j) {
         mc = kc[j] > mc ? kc[j] : mc;
-        mf = kf[j] > mf ? kf[j] : mf;
+        const bool live = 64 * j + 63 < P || kc[j] != 0u;
+        mn = live && kc[j] < mn ? kc[j] : mn;
     }
     uint32_t tc, tf;
-    kth_bounds<16>(mc, mf, K, tc, tf);
+    kth_bounds<16>(mc, mn == 0xffffffffu ? 0u : ~mn, K, tc, tf);
+    const uint32_t tfn = ~tf;  // far candidate: kc <= ~tf
     uint32_t* shc = lds;
-    uint32_t* shf = lds + 128;
-    int nc = 0, nf = 0;
-#pragma unroll
-    for (int j = 0; j < VPL; ++j) {
-        const uint32_t e = (uint32_t)(lane + 64 * j);
-        const bool cc = kc[j] >= tc && kc[j] != 0u;
-        const bool cf = kf[j] >= tf && kf[j] != 0u;
-        const unsigned long long bc = __ballot(cc), bf = __ballot(cf);
-        const int pc = nc + lanes_below(bc), pf = nf + lanes_below(bf);
-        if (cc && pc < 64) ((uint2*)shc)[pc] = uint2{kc[j], ~e};
-        if (cf && pf < 64) ((uint2*)shf)[pf] = uint2{kf[j], ~e};
-        nc += __popcll(bc);
-        nf += __popcll(bf);
-    }
+    uint32_t* shf = lds + 256;
+    int nc, nf;
+    if (full) compact_both<VPL, true>(kc, P, lane, lg, tc, tfn, shc, shf, nc, nf);
+    else compact_both<VPL, false>(kc, P, lane, lg, tc, tfn, shc, shf, nc, nf);
     __builtin_amdgcn_wave_barrier();
-    if (nc <= 64) rank_select(shc, nc, K, lane, out_close, tie_c);
-    else tie_c = wave_topk_general<VPL>(kc, P, K, lane, out_close);
-    if (nf <= 64) rank_select(shf, nf, K, lane, out_far, tie_f);
-    else tie_f = wave_topk_general<VPL>(kf, P, K, lane, out_far);
+    bool tc_, tf_;
+    rank_select2((const uint2*)shc, nc <= 64 ? nc : 0, (const uint2*)shf, nf <= 64 ? nf : 0, K, lane, out_close,
+                 out_far, tc_, tf_);
+    tie_c = tc_;
+    tie_f = tf_;
+    if (nc > 64) tie_c = wave_topk_general<VPL>(kc, P, K, lane, lg, out_close);
+    if (nf > 64) {
+        uint32_t kf[VPL];
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) kf[j] = kc[j] ? ~kc[j] : 0u;
+        tie_f = wave_topk_general<VPL>(kf, P, K, lane, lg, out_far);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -390,12 +444,14 @@ __device__ __forceinline__ void wave_topk_both(const uint32_t (&kc)[VPL], int P,
 // q = (4 g + w) + t * 4 G, so chunk g is an interleaved subset.
 enum { kSample = 0, kFull = 1, kRows = 2, kPoints = 3 };
 
-__device__ __forceinline__ double wave_sum_f64(double v, int lane) {
+// Sum over the wave in FP64: xor butterfly over the logical lane index,
+// offsets 32, 16, ..., 1 (the order of k_transform's tree).
+__device__ __forceinline__ double wave_sum_f64(double v, int lg) {
     uint32_t lo = (uint32_t)__double_as_longlong(v);
     uint32_t hi = (uint32_t)((unsigned long long)__double_as_longlong(v) >> 32);
 #define SDB_SUM_STEP(S)                                                                   \
     {                                                                                     \
-        const uint32_t plo = xchg<S>(lo, lane), phi = xchg<S>(hi, lane);                  \
+        const uint32_t plo = xchg_l<S>(lo, lg), phi = xchg_l<S>(hi, lg);                  \
         const double o = __longlong_as_double((long long)(((unsigned long long)phi << 32) | plo)); \
         v = v + o;                                                                        \
         lo = (uint32_t)__double_as_longlong(v);                                           \
@@ -406,22 +462,38 @@ __device__ __forceinline__ double wave_sum_f64(double v, int lane) {
     return v;
 }
 
+// Row loads and stores through buffer resources (base = the row, num_records
+// = its valid bytes): element e sits at byte 4 e, and accesses at or past the
+// end read 0 / are dropped by the hardware range check, so the ragged last
+// register of a row needs no branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* row, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), (short)0, bytes, 0x00020000);
+}
+template <int VPL>
+__device__ __forceinline__ void load_row(const float* row, int d, int lg, float (&out)[VPL]) {
+    const __amdgpu_buffer_rsrc_t r = row_rsrc(row, d * 4);
+#pragma unroll
+    for (int j = 0; j < VPL; ++j)
+        out[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, 4 * lg + 256 * j, 0, 0));
+}
+
 template <int LOGD, int MODE, bool COS>
 __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
     constexpr int D = 1 << LOGD;
     constexpr int VPL = D / 64;
-    constexpr int NW = (VPL + 31) / 32;
     const int lane = lane_id();
-    const int wib = threadIdx.x / kWave;
+    const int lg = lane_logical(lane);  // mirror layout: this lane holds elements lg + 64 j
+    const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     const long long gw = (long long)blockIdx.x * 4 + wib;
     const long long nw = (long long)gridDim.x * 4;
     const int P = a.P;
+    constexpr int NW = (VPL + 31) / 32;
 
     extern __shared__ __align__(16) unsigned char smem[];
-    // LDS: [top-K scratch 4 x 1 KB (kFull)] [2P keys] [embedding 4 x D x 4 B (!COS)]
-    const size_t off_red = MODE == kFull ? (size_t)4 * 1024 : 0;
+    // LDS: [top-K scratch 4 x 2 KB (kFull)] [2P keys] [embedding 4 x D x 4 B (!COS)]
+    const size_t off_red = MODE == kFull ? (size_t)4 * 2048 : 0;
     const size_t off_emb = off_red + (size_t)2 * P * 4;
-    uint64_t* tks = (uint64_t*)smem + (size_t)wib * 128;
+    uint64_t* tks = (uint64_t*)smem + (size_t)wib * 256;
     uint32_t* red = (uint32_t*)(smem + off_red);
     float* emb = (float*)(smem + off_emb) + (size_t)wib * D;
 
@@ -430,6 +502,7 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int w = 0; w < NW; ++w) sg[r][w] = a.signmask[(r * NW + w) * 64 + lane];
+
 
     if (MODE == kSample) {
         for (int i = threadIdx.x; i < P; i += blockDim.x) { red[i] = 0u; red[P + i] = 0xffffffffu; }
@@ -451,24 +524,18 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
     // raw rows of the next point are loaded one iteration ahead (cosine)
     float xpre[RAW ? VPL : 1];
     if constexpr (RAW) {
-        if (gw < npts) {
-            const float* x = a.X + gw * stride * a.d;
-#pragma unroll
-            for (int j = 0; j < VPL; ++j) { const int e = lane + 64 * j; xpre[j] = e < a.d ? x[e] : 0.0f; }
-        }
+        if (gw < npts) load_row<VPL>(a.X + gw * stride * a.d, a.d, lg, xpre);
     }
     for (long long q = gw; q < npts; q += nw) {
         const long long p = MODE == kPoints ? (long long)a.pt_ids[q] : q * stride;
         float v[VPL];
         if constexpr (COS) {
             if constexpr (!RAW) {
-                const float* x = a.Xn + p * a.ldx;
-#pragma unroll
-                for (int j = 0; j < VPL; ++j) { const int e = lane + 64 * j; v[j] = e < a.d ? x[e] : 0.0f; }
+                load_row<VPL>(a.Xn + p * a.ldx, a.d, lg, v);
             } else {
                 // transformData for Cosine (src/Utilities.h:295-299): x / ||x||,
-                // ||x||^2 summed in FP64 per lane (e = lane + 64 j) then by the
-                // xor butterfly -- the order of k_transform.
+                // ||x||^2 summed in FP64 per lane (elements e = lg + 64 j) then by
+                // the xor butterfly over lg -- the order of k_transform.
                 double acc = 0.0;
 #pragma unroll
                 for (int j = 0; j < VPL; ++j) {
@@ -476,12 +543,8 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
                     const double t = v[j];
                     acc = acc + t * t;
                 }
-                if (q + nw < npts) {
-                    const float* xn = a.X + (q + nw) * stride * a.d;
-#pragma unroll
-                    for (int j = 0; j < VPL; ++j) { const int e = lane + 64 * j; xpre[j] = e < a.d ? xn[e] : 0.0f; }
-                }
-                const float nrm = (float)sqrt(wave_sum_f64(acc, lane));
+                if (q + nw < npts) load_row<VPL>(a.X + (q + nw) * stride * a.d, a.d, lg, xpre);
+                const float nrm = (float)sqrt(wave_sum_f64(acc, lg));
                 if (!(nrm > 0.0f)) {
                     if (lane == 0) atomicOr(a.err, 2);
                 }
@@ -495,14 +558,16 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
 #pragma unroll
                 for (int j = 0; j < VPL; ++j) v[j] = (float)((double)v[j] * rinv);
                 if (MODE == kFull) {
-                    float* o = a.Xn_out + p * a.ldx + lane;
-                    uint16_t* ob = a.Xb_out ? a.Xb_out + p * a.ldx + lane : nullptr;
+                    // stores past ldx are dropped by the range check
+                    const __amdgpu_buffer_rsrc_t ro = row_rsrc(a.Xn_out + p * a.ldx, a.ldx * 4);
 #pragma unroll
-                    for (int j = 0; j < VPL; ++j) {
-                        if (lane + 64 * j < a.ldx) {
-                            o[64 * j] = v[j];
-                            if (ob) ob[64 * j] = (uint16_t)f32_to_bf16_rne(v[j]);
-                        }
+                    for (int j = 0; j < VPL; ++j)
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v[j]), ro, 4 * lg + 256 * j, 0, 0);
+                    if (a.Xb_out) {
+                        const __amdgpu_buffer_rsrc_t rb = row_rsrc(a.Xb_out + p * a.ldx, a.ldx * 2);
+#pragma unroll
+                        for (int j = 0; j < VPL; ++j)
+                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)f32_to_bf16_rne(v[j]), rb, 2 * lg + 128 * j, 0, 0);
                     }
                 }
             }
@@ -510,15 +575,15 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
             embed_lds<D>(a, p, lane, emb);
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int j = 0; j < VPL; ++j) v[j] = emb[lane + 64 * j];
+            for (int j = 0; j < VPL; ++j) v[j] = emb[lg + 64 * j];
             __builtin_amdgcn_wave_barrier();
         }
-        hd3_rotate<LOGD>(v, lane, sg);
+        hd3_rotate<LOGD>(v, lg, sg);
 
         if constexpr (MODE == kSample) {
 #pragma unroll
             for (int j = 0; j < VPL; ++j) {
-                if (lane + 64 * j < P) {
+                if (lg + 64 * j < P) {
                     const uint32_t k = ord_key(v[j]);
                     cmax[j] = k > cmax[j] ? k : cmax[j];
                     cmin[j] = k < cmin[j] ? k : cmin[j];
@@ -529,11 +594,11 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
 #pragma unroll
             for (int j = 0; j < VPL; ++j) {
                 kc[j] = ord_key(v[j]);
-                if (64 * j + 63 >= P && lane + 64 * j >= P) kc[j] = 0u;
+                if (64 * j + 63 >= P && lg + 64 * j >= P) kc[j] = 0u;
             }
             int* tk = a.topK + p * 2 * a.K;
             bool t1, t2;
-            wave_topk_both<VPL>(kc, P, a.K, lane, (uint32_t*)tks, tk, tk + a.K, t1, t2);
+            wave_topk_both<VPL>(kc, P, a.K, lane, lg, (uint32_t*)tks, tk, tk + a.K, t1, t2);
             if (t1 || t2) {
                 int slot = 0;
                 if (lane == 0) slot = atomicAdd(a.tie_count, 1);
@@ -543,7 +608,7 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
                     float* tv = a.tie_vals + (size_t)slot * P;
 #pragma unroll
                     for (int j = 0; j < VPL; ++j) {
-                        const int e = lane + 64 * j;
+                        const int e = lg + 64 * j;
                         if (e < P) tv[e] = key_to_float(kc[j]);
                     }
                 }
@@ -555,7 +620,7 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
             if (__any(anyc))
 #pragma unroll 1
             for (int j = 0; j < VPL; ++j) {
-                const int e = lane + 64 * j;
+                const int e = lg + 64 * j;
                 const uint32_t k = kc[j];
                 if (k == 0u) continue;
                 if (k >= any_hi && k >= red[e]) {
@@ -578,7 +643,7 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
         } else if constexpr (MODE == kRows) {
 #pragma unroll
             for (int j = 0; j < VPL; ++j) {
-                const int e = lane + 64 * j;
+                const int e = lg + 64 * j;
                 if (e < P) {
                     const int s = a.row_slot[e];
                     if (s >= 0) a.row_buf[(size_t)s * a.n + p] = v[j];
@@ -587,7 +652,7 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
         } else {
 #pragma unroll
             for (int j = 0; j < VPL; ++j) {
-                const int e = lane + 64 * j;
+                const int e = lg + 64 * j;
                 if (e < P) a.pt_buf[(size_t)e * a.n_ids + q] = v[j];
             }
         }
@@ -595,7 +660,7 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
     if constexpr (MODE == kSample) {
 #pragma unroll
         for (int j = 0; j < VPL; ++j) {
-            const int e = lane + 64 * j;
+            const int e = lg + 64 * j;
             if (e < P) { atomicMax(&red[e], cmax[j]); atomicMin(&red[P + e], cmin[j]); }
         }
         __syncthreads();
@@ -940,7 +1005,7 @@ int launch_transform(const float* X, long long n, int d, int ldx, int metric, fl
 template <int LOGD, int MODE, bool COS>
 static int launch_index_t(const IndexArgs& a, int grid, hipStream_t st) {
     constexpr int D = 1 << LOGD;
-    size_t sm = (MODE == kFull ? (size_t)4 * 1024 : 0) + (size_t)2 * a.P * 4 + (COS ? 0 : (size_t)4 * D * 4);
+    size_t sm = (MODE == kFull ? (size_t)4 * 2048 : 0) + (size_t)2 * a.P * 4 + (COS ? 0 : (size_t)4 * D * 4);
     if (sm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE, COS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);

@@ -13,7 +13,7 @@ struct IndexArgs {
     float* Xn_out;    // cosine full pass writes the transformed rows here
     uint16_t* Xb_out; // and their BF16 copy (optional)
     int* err;         // bit 2: zero point
-    const uint32_t* signmask;  // [3][NW][64] per-lane sign bits
+    const uint32_t* signmask;  // [3][NW][64] per-lane sign bits (lanes in the mirror layout)
     long long sample_stride;
     const uint32_t* thr_any;   // [0] min of thr_hi, [1] max of thr_lo
     long long n;
