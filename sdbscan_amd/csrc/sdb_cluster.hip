@@ -148,36 +148,10 @@ __global__ void k_uf_init(ClusterArgs c) {
     c.border_min[a] = 0x7fffffff;
 }
 
-// Per (core point, slot): all core hits of the slot's list are connected
-// through the point.  adj[l][first] collects them (star edges from the first
-// hit) for the list-local unions of k_uf_lists; k_uf_points then joins each
-// point to the first hit of each slot.
-__global__ void k_uf_adj(ClusterArgs c) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= c.n * c.S) return;
-    const long long a = t / c.S;
-    if (!c.core[a]) return;
-    const int s = (int)(t % c.S);
-    const int l = slot_list_c(c.topK, a, s, c.K, c.P);
-    uint32_t m[kMaxW];
-    entry_words(c.E, (size_t)t, c.W, m);
-    int first = -1;
-#pragma unroll
-    for (int w = 0; w < kMaxW; ++w) {
-        if (w < c.W) m[w] &= c.corehub[(size_t)l * c.W + w];
-        if (first < 0 && m[w]) first = w * 32 + __ffs(m[w]) - 1;
-    }
-    if (first < 0) return;
-    // only the hits after the first one add edges
-#pragma unroll
-    for (int w = 0; w < kMaxW; ++w)
-        if (w == (first >> 5)) m[w] &= ~(1u << (first & 31));
-    uint32_t* ad = c.adj + ((size_t)l * c.M + first) * c.W;
-#pragma unroll
-    for (int w = 0; w < kMaxW; ++w)
-        if (w < c.W && m[w] && (load_relaxed((const int*)&ad[w]) & m[w]) != m[w]) atomicOr(&ad[w], m[w]);
-}
-
+// Per core point, slot by slot: all core hits of the slot's list are
+// connected through the point.  The point is joined to the first core hit;
+// the later ones are recorded as star edges from the first (adj[l][first])
+// for the list-local unions of k_uf_lists.
 __global__ void k_uf_points(ClusterArgs c) {
     long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= c.n || !c.core[a]) return;
@@ -185,16 +159,22 @@ __global__ void k_uf_points(ClusterArgs c) {
     for (int s = 0; s < c.S; ++s) {
         const int l = slot_list_c(c.topK, a, s, c.K, c.P);
         const size_t t = (size_t)a * c.S + s;
-        uint32_t ew[kMaxW];
-        entry_words(c.E, t, c.W, ew);
+        uint32_t m[kMaxW];
+        entry_words(c.E, t, c.W, m);
         int first = -1;
 #pragma unroll
         for (int w = 0; w < kMaxW; ++w) {
-            if (first >= 0 || w >= c.W || !ew[w]) continue;
-            const uint32_t m = ew[w] & c.corehub[(size_t)l * c.W + w];
-            if (m) first = w * 32 + __ffs(m) - 1;
+            if (w < c.W && m[w]) m[w] &= c.corehub[(size_t)l * c.W + w];
+            if (first < 0 && m[w]) first = w * 32 + __ffs(m[w]) - 1;
         }
         if (first < 0) continue;
+#pragma unroll
+        for (int w = 0; w < kMaxW; ++w)
+            if (w == (first >> 5)) m[w] &= ~(1u << (first & 31));
+        uint32_t* ad = c.adj + ((size_t)l * c.M + first) * c.W;
+#pragma unroll
+        for (int w = 0; w < kMaxW; ++w)
+            if (w < c.W && m[w] && (load_relaxed((const int*)&ad[w]) & m[w]) != m[w]) atomicOr(&ad[w], m[w]);
         const int x = c.lists[(size_t)l * c.M + first];
         if (uf_find(c.parent, x) != ra) {
             uf_union(c.parent, (int)a, x);
@@ -284,10 +264,6 @@ int launch_corehub(const ClusterArgs& c, hipStream_t st) {
 }
 int launch_uf_init(const ClusterArgs& c, hipStream_t st) {
     hipLaunchKernelGGL(k_uf_init, dim3(nblk(c.n, 256)), dim3(256), 0, st, c);
-    return (int)hipGetLastError();
-}
-int launch_uf_slots(const ClusterArgs& c, hipStream_t st) {
-    hipLaunchKernelGGL(k_uf_adj, dim3(nblk(c.n * c.S, 256)), dim3(256), 0, st, c);
     return (int)hipGetLastError();
 }
 int launch_uf_points(const ClusterArgs& c, hipStream_t st) {

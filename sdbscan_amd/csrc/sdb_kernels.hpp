@@ -134,7 +134,6 @@ struct ClusterArgs {
 int launch_degree(const ClusterArgs& c, hipStream_t st);
 int launch_corehub(const ClusterArgs& c, hipStream_t st);
 int launch_uf_init(const ClusterArgs& c, hipStream_t st);
-int launch_uf_slots(const ClusterArgs& c, hipStream_t st);
 int launch_uf_lists(const ClusterArgs& c, hipStream_t st);
 int launch_uf_points(const ClusterArgs& c, hipStream_t st);
 int launch_uf_compress(const ClusterArgs& c, hipStream_t st);
