@@ -80,7 +80,7 @@ struct sdb_ctx {
     // transformed
     DevBuf Xn, Xb;
     // index
-    DevBuf bits, R, thr_any, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
+    DevBuf bits, R, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
         cand_idx, topM, flags, lists, is_hub, hub_flags, hub_excl, hub_id, hub_cnt, hub_ptr,
         hub_cur, hub_mem, inv, pos_hub, pcnt, poff, pairs, F, wgcnt, wgoff, grp_ptr, grp, scan_tmp, total, err_flag,
         row_slot, row_buf;
@@ -231,7 +231,7 @@ void sdb_destroy(sdb_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
-    DevBuf* all[] = {&c->Xown, &c->Xn, &c->Xb, &c->bits, &c->R, &c->thr_any, &c->topK, &c->tie_idx, &c->tie_vals,
+    DevBuf* all[] = {&c->Xown, &c->Xn, &c->Xb, &c->bits, &c->R, &c->topK, &c->tie_idx, &c->tie_vals,
                      &c->tie_count, &c->cmax, &c->cmin, &c->thi, &c->tlo, &c->cand_cnt, &c->cand_key,
                      &c->cand_idx, &c->topM, &c->flags, &c->lists, &c->is_hub, &c->hub_flags,
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
@@ -403,7 +403,6 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ENSURE(c->cmin, (size_t)Gs * P * 4);
     ENSURE(c->thi, (size_t)P * 4);
     ENSURE(c->tlo, (size_t)P * 4);
-    ENSURE(c->thr_any, 8);
     const int cap = 4096;
     ENSURE(c->cand_cnt, (size_t)2 * P * 4);
     ENSURE(c->cand_key, (size_t)2 * P * cap * 4);
@@ -421,7 +420,6 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ia.err = P_<int>(c->err_flag);
     ia.signmask = P_<uint32_t>(c->bits);
     ia.sample_stride = stride;
-    ia.thr_any = P_<uint32_t>(c->thr_any);
     ia.n = n; ia.ldx = ldx; ia.d = d; ia.metric = q.metric; ia.ker = q.ker_n_features;
     ia.kerL = q.ker_intervalSampling;
     ia.R = l12 ? P_<float>(c->R) : nullptr;
@@ -436,7 +434,6 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     const int logD = ilog2(D);
     LCHK(launch_index(ia, logD, 0, Gs, st));
     LCHK(launch_topm_threshold(ia.chunk_max, ia.chunk_min, Gs, P, M, P_<uint32_t>(c->thi), P_<uint32_t>(c->tlo), st));
-    LCHK(launch_thr_any(P_<uint32_t>(c->thi), P_<uint32_t>(c->tlo), P, P_<uint32_t>(c->thr_any), st));
     // ---- full pass: (transform,) projections, top-K, top-M candidates
     LCHK(launch_index(ia, logD, 1, Gf, st));
     int ntie = 0;

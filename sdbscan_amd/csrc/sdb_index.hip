@@ -462,6 +462,9 @@ __device__ __forceinline__ double wave_sum_f64(double v, int lg) {
     return v;
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
 // Row loads and stores through buffer resources (base = the row, num_records
 // = its valid bytes): element e sits at byte 4 e, and accesses at or past the
 // end read 0 / are dropped by the hardware range check, so the ragged last
@@ -492,7 +495,9 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     // LDS: [top-K scratch 4 x 2 KB (kFull)] [2P keys] [embedding 4 x D x 4 B (!COS)]
     const size_t off_red = MODE == kFull ? (size_t)4 * 2048 : 0;
-    const size_t off_emb = off_red + (size_t)2 * P * 4;
+    // threshold area: kSample row max / min accumulators [P] + [P]; kFull
+    // per-row (hi, lo) threshold pairs for all D elements (past P: never hit)
+    const size_t off_emb = off_red + (size_t)8 * (MODE == kFull ? D : P);
     uint64_t* tks = (uint64_t*)smem + (size_t)wib * 256;
     uint32_t* red = (uint32_t*)(smem + off_red);
     float* emb = (float*)(smem + off_emb) + (size_t)wib * D;
@@ -507,11 +512,11 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
     if (MODE == kSample) {
         for (int i = threadIdx.x; i < P; i += blockDim.x) { red[i] = 0u; red[P + i] = 0xffffffffu; }
     } else if (MODE == kFull) {
-        for (int i = threadIdx.x; i < P; i += blockDim.x) { red[i] = a.thr_hi[i]; red[P + i] = a.thr_lo[i]; }
+        uint2* thr = (uint2*)red;
+        for (int i = threadIdx.x; i < D; i += blockDim.x)
+            thr[i] = i < P ? uint2{a.thr_hi[i], a.thr_lo[i]} : uint2{0xffffffffu, 0u};
     }
     __syncthreads();
-    const uint32_t any_hi = MODE == kFull ? a.thr_any[0] : 0u;
-    const uint32_t any_lo = MODE == kFull ? a.thr_any[1] : 0u;
 
     uint32_t cmax[MODE == kSample ? VPL : 1], cmin[MODE == kSample ? VPL : 1];
     if constexpr (MODE == kSample) {
@@ -564,10 +569,17 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
                     for (int j = 0; j < VPL; ++j)
                         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v[j]), ro, 4 * lg + 256 * j, 0, 0);
                     if (a.Xb_out) {
+                        // BF16 copy, round to nearest even (hardware pair conversion)
                         const __amdgpu_buffer_rsrc_t rb = row_rsrc(a.Xb_out + p * a.ldx, a.ldx * 2);
 #pragma unroll
-                        for (int j = 0; j < VPL; ++j)
-                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)f32_to_bf16_rne(v[j]), rb, 2 * lg + 128 * j, 0, 0);
+                        for (int j = 0; j + 1 < VPL; j += 2) {
+                            const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){v[j], v[j + 1]}, bf16x2));
+                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)u, rb, 2 * lg + 128 * j, 0, 0);
+                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(u >> 16), rb, 2 * lg + 128 * (j + 1), 0, 0);
+                        }
+                        if constexpr (VPL % 2 == 1)
+                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)f32_to_bf16_rne(v[VPL - 1]), rb,
+                                                                  2 * lg + 128 * (VPL - 1), 0, 0);
                     }
                 }
             }
@@ -613,17 +625,32 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
                     }
                 }
             }
-            // values beyond every row's threshold are rare: one wave-wide test
+            // values beyond their row's threshold are rare: one unrolled test
+            // against the per-row (hi, lo) pairs, then the appends.  Elements
+            // past P have key 0 and the pair (~0, 0).
+            const uint2* thr = (const uint2*)red;
             bool anyc = false;
+            if (P >= VPL * 64) {
 #pragma unroll
-            for (int j = 0; j < VPL; ++j) anyc |= kc[j] != 0u && (kc[j] >= any_hi || kc[j] <= any_lo);
+                for (int j = 0; j < VPL; ++j) {
+                    const uint2 th = thr[lg + 64 * j];
+                    anyc |= kc[j] >= th.x || kc[j] <= th.y;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < VPL; ++j) {
+                    const uint2 th = thr[lg + 64 * j];
+                    anyc |= kc[j] >= th.x || (kc[j] <= th.y && kc[j] != 0u);
+                }
+            }
             if (__any(anyc))
 #pragma unroll 1
             for (int j = 0; j < VPL; ++j) {
                 const int e = lg + 64 * j;
                 const uint32_t k = kc[j];
                 if (k == 0u) continue;
-                if (k >= any_hi && k >= red[e]) {
+                const uint2 th = thr[e];
+                if (k >= th.x) {
                     const int pos = atomicAdd(&a.cand_cnt[2 * e], 1);
                     if (pos < a.cand_cap) {
                         const size_t o = (size_t)(2 * e) * a.cand_cap + pos;
@@ -631,7 +658,7 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
                         a.cand_idx[o] = (int)p;
                     }
                 }
-                if (k <= any_lo && k <= red[P + e]) {
+                if (k <= th.y) {
                     const int pos = atomicAdd(&a.cand_cnt[2 * e + 1], 1);
                     if (pos < a.cand_cap) {
                         const size_t o = (size_t)(2 * e + 1) * a.cand_cap + pos;
@@ -707,18 +734,6 @@ __global__ __launch_bounds__(256) void k_topm_threshold(const uint32_t* __restri
     }
     __syncthreads();
     if (threadIdx.x == 0) { thi[r] = res_hi; tlo[r] = res_lo; }
-}
-
-// thr_any[0] = min over rows of thr_hi, thr_any[1] = max over rows of thr_lo:
-// a value below / above both cannot be a candidate of any row.
-__global__ __launch_bounds__(256) void k_thr_any(const uint32_t* thi, const uint32_t* tlo, int P,
-                                                 uint32_t* thr_any) {
-    __shared__ uint32_t mn, mx;
-    if (threadIdx.x == 0) { mn = 0xffffffffu; mx = 0u; }
-    __syncthreads();
-    for (int i = threadIdx.x; i < P; i += blockDim.x) { atomicMin(&mn, thi[i]); atomicMax(&mx, tlo[i]); }
-    __syncthreads();
-    if (threadIdx.x == 0) { thr_any[0] = mn; thr_any[1] = mx; }
 }
 
 // Exact top-M per row and side from the candidate lists: bitonic sort of the
@@ -1005,7 +1020,7 @@ int launch_transform(const float* X, long long n, int d, int ldx, int metric, fl
 template <int LOGD, int MODE, bool COS>
 static int launch_index_t(const IndexArgs& a, int grid, hipStream_t st) {
     constexpr int D = 1 << LOGD;
-    size_t sm = (MODE == kFull ? (size_t)4 * 2048 : 0) + (size_t)2 * a.P * 4 + (COS ? 0 : (size_t)4 * D * 4);
+    size_t sm = (MODE == kFull ? (size_t)4 * 2048 + (size_t)8 * D : (size_t)8 * a.P) + (COS ? 0 : (size_t)4 * D * 4);
     if (sm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE, COS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -1037,10 +1052,6 @@ int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t s
     }
 }
 
-int launch_thr_any(const uint32_t* thi, const uint32_t* tlo, int P, uint32_t* thr_any, hipStream_t st) {
-    hipLaunchKernelGGL(k_thr_any, dim3(1), dim3(256), 0, st, thi, tlo, P, thr_any);
-    return (int)hipGetLastError();
-}
 
 int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,
                           uint32_t* thi, uint32_t* tlo, hipStream_t st) {

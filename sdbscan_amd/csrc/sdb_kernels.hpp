@@ -15,7 +15,6 @@ struct IndexArgs {
     int* err;         // bit 2: zero point
     const uint32_t* signmask;  // [3][NW][64] per-lane sign bits (lanes in the mirror layout)
     long long sample_stride;
-    const uint32_t* thr_any;   // [0] min of thr_hi, [1] max of thr_lo
     long long n;
     int ldx, d, metric, ker;
     float kerL;        // ker_intervalSampling
@@ -74,7 +73,6 @@ struct DistArgs {
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
                      uint16_t* Xb, int* err, hipStream_t st);
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st);
-int launch_thr_any(const uint32_t* thi, const uint32_t* tlo, int P, uint32_t* thr_any, hipStream_t st);
 int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,
                           uint32_t* thi, uint32_t* tlo, hipStream_t st);
 int launch_topm_select(const uint32_t* key, const int* idx, const int* cnt, int cap, int P, int M,
