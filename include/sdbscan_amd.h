@@ -133,6 +133,11 @@ int sdb_get_index(sdb_ctx* ctx, int* topK_out, int* topM_out);
 /* Transformed points (n x n_features) as used by the distance tiles. */
 int sdb_get_transformed(sdb_ctx* ctx, float* X_out);
 
+/* Their BF16 copy (n x n_features, round to nearest even) read by the BF16
+ * distance tiles; SDB_ESTATE unless the precision is SDB_BF16.  No
+ * counterpart in the reference (it has no BF16 path). */
+int sdb_get_transformed_bf16(sdb_ctx* ctx, uint16_t* X_out);
+
 /* Deduplicated neighbourhoods after sdb_find_core as CSR sorted by index:
  * ptr has n+1 entries; idx/dist hold ptr[n] entries (dist = FP32 distance,
  * may be NULL).  Call with idx == NULL to get ptr only. */

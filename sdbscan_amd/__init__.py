@@ -62,6 +62,7 @@ _SIGS = {
                           ctypes.c_int),
     "sdb_get_index": ([_P, _P, _P], ctypes.c_int),
     "sdb_get_transformed": ([_P, _P], ctypes.c_int),
+    "sdb_get_transformed_bf16": ([_P, _P], ctypes.c_int),
     "sdb_get_neighbors": ([_P, _P, _P, _P], ctypes.c_int),
     "sdb_core_dist": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "sdb_optics_reach": ([_P, _P, _P], ctypes.c_int),
@@ -222,6 +223,14 @@ class Context:
         p = self.params
         X = np.zeros((p.n_points, p.n_features), np.float32)
         self._chk(self.lib.sdb_get_transformed(self.h, X.ctypes.data))
+        return X
+
+    def get_transformed_bf16(self):
+        """BF16 bit patterns (uint16) of the transformed points."""
+        import numpy as np
+        p = self.params
+        X = np.zeros((p.n_points, p.n_features), np.uint16)
+        self._chk(self.lib.sdb_get_transformed_bf16(self.h, X.ctypes.data))
         return X
 
     def get_neighbors(self):

@@ -592,6 +592,15 @@ int sdb_get_transformed(sdb_ctx* c, float* X_out) {
     return SDB_OK;
 }
 
+int sdb_get_transformed_bf16(sdb_ctx* c, uint16_t* X_out) {
+    if (!c || !X_out) return SDB_EINVAL;
+    if (!c->transformed || c->transformed_prec != SDB_BF16 || !c->Xb.p)
+        return fail(c, SDB_ESTATE, "no BF16 rows: build the index (or sng_find_core) with precision BF16 first");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpy2D(X_out, (size_t)c->d * 2, c->Xb.p, (size_t)c->ldx * 2, (size_t)c->d * 2, (size_t)c->n, hipMemcpyDeviceToHost));
+    return SDB_OK;
+}
+
 // Error band of the fast distance, derived from the standard bound
 // |fl(sum) - sum| <= gamma_k * sum |terms| (any summation order, one rounding of
 // at most 2^-23 relative per operation), with a safety factor of 2.

@@ -57,6 +57,23 @@ def test_transform_bit_exact(ctx):
     assert np.array_equal(Xg.view(np.uint32), Xo.view(np.uint32))
 
 
+def test_bf16_rows_round_to_nearest_even(ctx):
+    """The BF16 copy of the transformed rows is the round-to-nearest-even of
+    the FP32 rows, exact halfway cases included (the recheck band of the BF16
+    tiles assumes half-ulp rounding)."""
+    X, _ = datagen.mixture(3000, 784, seed=19)
+    X = X * np.float32(0.37)
+    print("bf16 rows cosine n=3000 d=784", flush=True)
+    run_index(ctx, X, precision=S.BF16)
+    Xn = ctx.get_transformed()
+    Xb = ctx.get_transformed_bf16()
+    u = Xn.view(np.uint32).astype(np.uint64)
+    low = u & 0xFFFF
+    up = (u >> 16) + ((low > 0x8000) | ((low == 0x8000) & ((u >> 16) & 1 == 1)))
+    assert np.count_nonzero(low == 0x8000) > 0   # halfway cases present
+    assert np.array_equal(Xb, up.astype(np.uint16))
+
+
 def test_index_matches_oracle(ctx):
     X, _ = datagen.mixture(4000, 784, seed=11)
     print("index cosine n=4000 K=5 M=50", flush=True)
