@@ -371,15 +371,18 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     HIPCHK(hipEventRecord(c->ev[1], st));
 
     // ---- random structures: per-lane sign masks of the 3 x D sign bits (lane
-    // p holds elements lane_logical(p) + 64 j in the index kernels)
+    // p holds elements lane_logical(p) + 64 j in the index kernels).  Rounds 1
+    // and 2 carry the pending sign of the previous FWHT of the index kernels:
+    // lanes with odd popcount(lane_logical(p) & 15) hold negated outputs.
     const int VPL = D / 64, NWs = (VPL + 31) / 32;
     std::vector<uint32_t> smask((size_t)3 * NWs * 64, 0u);
     for (int r = 0; r < 3; ++r)
         for (int w = 0; w < NWs; ++w)
             for (int lane = 0; lane < 64; ++lane) {
+                const uint32_t pend = r > 0 ? (uint32_t)(__builtin_popcount(lane_logical(lane) & 15) & 1) : 0u;
                 uint32_t m = 0;
                 for (int b = 0; b < 32 && w * 32 + b < VPL; ++b)
-                    m |= (uint32_t)(hd3_bits[(size_t)r * D + lane_logical(lane) + 64 * (w * 32 + b)] & 1u) << b;
+                    m |= ((uint32_t)(hd3_bits[(size_t)r * D + lane_logical(lane) + 64 * (w * 32 + b)] & 1u) ^ pend) << b;
                 smask[((size_t)r * NWs + w) * 64 + lane] = m;
             }
     ENSURE(c->bits, smask.size() * 4);

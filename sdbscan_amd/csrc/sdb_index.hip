@@ -112,19 +112,70 @@ __device__ __forceinline__ void fwht_inreg(float (&v)[VPL], int js) {
     }
 }
 
+// Walsh-Hadamard levels at logical strides 1, 2, 4, 8 (DPP partners in the
+// mirror layout) with one v_fmac_f32_dpp per element: v += sg * v[partner],
+// sg = -1 on the upper lanes of the level.  An upper lane then holds
+// mine - partner = -(partner - mine), the negated level output with the same
+// rounding (round to nearest even is symmetric), so after the four levels a
+// lane holds its outputs times (-1)^popcount(lg & 15); hd3_rotate folds that
+// pending sign into the next sign flip.  The partners of a level have equal
+// pending signs (they differ only in the level's bit), so sg is the plain
+// per-level sign.
+// Hazards: a DPP read needs 2 wait states after a VALU write of its source
+// and 5 after a VALU write of EXEC.  Only the stride-1 blocks can follow
+// compiler code that wrote v (the sign flip), so they start with s_nop 4; in
+// blocks of 8 the other levels read registers written at least 7 DPP
+// instructions earlier.  The stride-8 blocks end with s_nop 4 for the
+// permlane swaps that follow.  Single-register blocks (fewer than 8 registers)
+// wait on both sides.
+#define SDB_FMAC_DPP(i, ctrl) "v_fmac_f32_dpp %" #i ", %" #i ", %8 " ctrl " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+#define SDB_FMAC_DPP8(lead, ctrl, trail)                                                               \
+    lead SDB_FMAC_DPP(0, ctrl) SDB_FMAC_DPP(1, ctrl) SDB_FMAC_DPP(2, ctrl) SDB_FMAC_DPP(3, ctrl)     \
+        SDB_FMAC_DPP(4, ctrl) SDB_FMAC_DPP(5, ctrl) SDB_FMAC_DPP(6, ctrl) SDB_FMAC_DPP(7, ctrl) trail
+#define SDB_FMAC_DPP1(ctrl) "s_nop 4\n\tv_fmac_f32_dpp %0, %0, %1 " ctrl " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\ts_nop 4"
+
+template <int S>
+__device__ __forceinline__ void fmac_dpp8(float* v, float sg) {
+#define SDB_OPS "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), "+v"(v[6]), "+v"(v[7]) : "v"(sg)
+    if constexpr (S == 1) asm volatile(SDB_FMAC_DPP8("s_nop 4\n\t", "quad_perm:[1,0,3,2]", "") : SDB_OPS);
+    if constexpr (S == 2) asm volatile(SDB_FMAC_DPP8("", "quad_perm:[2,3,0,1]", "") : SDB_OPS);
+    if constexpr (S == 4) asm volatile(SDB_FMAC_DPP8("", "row_half_mirror", "") : SDB_OPS);
+    if constexpr (S == 8) asm volatile(SDB_FMAC_DPP8("", "row_mirror", "s_nop 4") : SDB_OPS);
+#undef SDB_OPS
+}
+template <int S>
+__device__ __forceinline__ void fmac_dpp1(float& v, float sg) {
+    if constexpr (S == 1) asm volatile(SDB_FMAC_DPP1("quad_perm:[1,0,3,2]") : "+v"(v) : "v"(sg));
+    if constexpr (S == 2) asm volatile(SDB_FMAC_DPP1("quad_perm:[2,3,0,1]") : "+v"(v) : "v"(sg));
+    if constexpr (S == 4) asm volatile(SDB_FMAC_DPP1("row_half_mirror") : "+v"(v) : "v"(sg));
+    if constexpr (S == 8) asm volatile(SDB_FMAC_DPP1("row_mirror") : "+v"(v) : "v"(sg));
+}
+template <int S, int VPL>
+__device__ __forceinline__ void fwht_dpp(float (&v)[VPL], int lg) {
+    const float sg = (lg & S) ? -1.0f : 1.0f;
+    if constexpr (VPL % 8 == 0) {
+#pragma unroll
+        for (int j = 0; j < VPL; j += 8) fmac_dpp8<S>(&v[j], sg);
+    } else {
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) fmac_dpp1<S>(v[j], sg);
+    }
+}
+
 // Unnormalised FWHT with increasing strides (the level order of FFHT,
 // src/fht_avx.c:1013-1251).  Element e lives in register e >> 6 of the lane
 // whose logical index (mirror layout, lane_logical) is e & 63.
-// Outputs equal FFHT's as IEEE values; FFHT's (0 - v) butterflies at strides
-// 1, 2, 4 can only change the sign of an exact zero, which no comparison of
-// the index distinguishes.
+// Outputs equal FFHT's as IEEE values times the pending sign of fwht_dpp;
+// FFHT's (0 - v) butterflies at strides 1, 2, 4 and the negated levels can
+// only change the sign of an exact zero, which no comparison of the index
+// distinguishes.
 template <int LOGD>
 __device__ __forceinline__ void ffht_wave(float (&v)[(1 << LOGD) / 64], int lg) {
     constexpr int VPL = (1 << LOGD) / 64;
-    fwht_cross<1, VPL>(v, lg);
-    fwht_cross<2, VPL>(v, lg);
-    fwht_cross<4, VPL>(v, lg);
-    fwht_cross<8, VPL>(v, lg);
+    fwht_dpp<1, VPL>(v, lg);
+    fwht_dpp<2, VPL>(v, lg);
+    fwht_dpp<4, VPL>(v, lg);
+    fwht_dpp<8, VPL>(v, lg);
     if constexpr (VPL % 2 == 0) {
         fwht_cross_pairs<16, VPL>(v);
         fwht_cross_pairs<32, VPL>(v);
@@ -190,7 +241,10 @@ __device__ __noinline__ void embed_lds(const IndexArgs& a, long long p, int lane
 
 // HD3 rotation in registers (src/sDbscan.cpp:77-88): 3 x {sign flip, FWHT};
 // the (2 bit - 1) multiplier negates where the bit is 0.  sg[r][w] bit b is
-// the sign bit of register 32 w + b in round r for this lane's elements.
+// the sign bit of register 32 w + b in round r for this lane's elements; the
+// masks of rounds 1 and 2 come with the pending sign of the previous FWHT
+// folded in (bit flipped on lanes with odd popcount(lg & 15)), and the last
+// FWHT's pending sign is removed at the end.
 template <int LOGD>
 __device__ __forceinline__ void hd3_rotate(float (&v)[(1 << LOGD) / 64], int lg,
                                            const uint32_t (&sg)[3][((1 << LOGD) / 64 + 31) / 32]) {
@@ -204,6 +258,9 @@ __device__ __forceinline__ void hd3_rotate(float (&v)[(1 << LOGD) / 64], int lg,
         }
         ffht_wave<LOGD>(v, lg);
     }
+    const uint32_t pend = (uint32_t)(__popc(lg & 15) & 1) << 31;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) v[j] = __uint_as_float(__float_as_uint(v[j]) ^ pend);
 }
 
 // Bitonic sorting network across the 64 lanes of a wavefront, descending:
