@@ -396,7 +396,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     const long long stride = n >= 65536 ? 8 : 1;
     const long long npts = (n + stride - 1) / stride;
     const int Gs = (int)std::max<long long>(1, std::min<long long>(1024, (npts + 3) / 4));
-    const int Gf = (int)std::max<long long>(1, std::min<long long>(2048, (n + 3) / 4));
+    const int Gf = (int)std::max<long long>(1, std::min<long long>(8192, (n + 3) / 4));
     const int tie_cap = (int)std::min<long long>(n, 65536);
     ENSURE(c->topK, (size_t)n * 2 * K * 4);
     ENSURE(c->tie_idx, (size_t)tie_cap * 4);

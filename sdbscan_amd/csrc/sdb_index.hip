@@ -538,7 +538,9 @@ __device__ __forceinline__ void load_row(const float* row, int d, int lg, float 
 }
 
 template <int LOGD, int MODE, bool COS>
-__global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
+// Occupancy: 3 workgroups per CU up to D = 1024 (at most 168 registers),
+// fewer for the longer rows so that their registers do not spill.
+__global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_index(IndexArgs a) {
     constexpr int D = 1 << LOGD;
     constexpr int VPL = D / 64;
     const int lane = lane_id();
@@ -583,14 +585,15 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
     const long long stride = MODE == kSample ? a.sample_stride : 1;
     const long long npts = MODE == kPoints ? a.n_ids : (a.n + stride - 1) / stride;
     constexpr bool RAW = COS && (MODE == kSample || MODE == kFull);  // reads untransformed rows
-    // raw rows of the next point are loaded one iteration ahead (cosine)
-    float xpre[RAW ? VPL : 1];
-    if constexpr (RAW) {
-        if (gw < npts) load_row<VPL>(a.X + gw * stride * a.d, a.d, lg, xpre);
-    }
     for (long long q = gw; q < npts; q += nw) {
         const long long p = MODE == kPoints ? (long long)a.pt_ids[q] : q * stride;
         float v[VPL];
+        // the sign masks are re-derived per point rather than kept expanded
+        // across the loop (3 registers instead of 3 x VPL)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int w = 0; w < NW; ++w) asm volatile("" : "+v"(sg[r][w]));
         if constexpr (COS) {
             if constexpr (!RAW) {
                 load_row<VPL>(a.Xn + p * a.ldx, a.d, lg, v);
@@ -598,14 +601,13 @@ __global__ __launch_bounds__(256, 2) void k_index(IndexArgs a) {
                 // transformData for Cosine (src/Utilities.h:295-299): x / ||x||,
                 // ||x||^2 summed in FP64 per lane (elements e = lg + 64 j) then by
                 // the xor butterfly over lg -- the order of k_transform.
+                load_row<VPL>(a.X + p * a.d, a.d, lg, v);
                 double acc = 0.0;
 #pragma unroll
                 for (int j = 0; j < VPL; ++j) {
-                    v[j] = xpre[j];
                     const double t = v[j];
                     acc = acc + t * t;
                 }
-                if (q + nw < npts) load_row<VPL>(a.X + (q + nw) * stride * a.d, a.d, lg, xpre);
                 const float nrm = (float)sqrt(wave_sum_f64(acc, lg));
                 if (!(nrm > 0.0f)) {
                     if (lane == 0) atomicOr(a.err, 2);
