@@ -86,6 +86,47 @@ def cpu_baseline(ctx, bits):
                       f"core points, clusters) with {thr} OpenMP threads; {dt:.2f} s"}
 
 
+def batch_bench(args):
+    """Many-dataset sweep: 64 datasets of n log-uniform in [2,000, 500,000]
+    and d in {32, 128, 784} (3-30-component Gaussian mixtures on the unit
+    sphere), each swept over 8 eps values in [0.1, 0.4] x minPts {10, 50}
+    with sOPTICS at the largest eps, FP32 distances.  One JSON line: the
+    batched run (several streams, autotuned tile rows) against the same jobs
+    one dataset at a time (one stream, default tiles)."""
+    import sdbscan_amd as S
+    from oracle import oracle as O
+    from sdbscan_amd import datagen
+    rng = np.random.default_rng(1003)
+    nds = args.batch_datasets
+    ns = np.exp(rng.uniform(np.log(2000), np.log(500000), nds)).astype(np.int64)
+    ds = rng.choice([32, 128, 784], nds)
+    ks = rng.integers(3, 31, nds)
+    eps = np.sort(rng.uniform(0.1, 0.4, 8)).astype(np.float32)
+    minPts = [10, 50]
+    t0 = time.perf_counter()
+    data = [datagen.sphere_mixture(int(n), int(d), int(k), seed=2000 + i)[0]
+            for i, (n, d, k) in enumerate(zip(ns, ds, ks))]
+    log(f"[batch] generated {nds} datasets, {int(ns.sum())} points in {time.perf_counter() - t0:.1f} s")
+    kw = dict(optics=True, n_proj=P, topK=10, topM=50, hd3=O.hd3_bits)
+    S.batch_run(data[:2], eps, minPts, n_streams=2, autotune=False, **kw)  # warm-up
+    r1, wall1 = S.batch_run(data, eps, minPts, n_streams=1, autotune=False, **kw)
+    log(f"[batch] one dataset at a time: {wall1:.0f} ms")
+    rb, wallb = S.batch_run(data, eps, minPts, n_streams=args.streams, autotune=True, **kw)
+    log(f"[batch] {args.streams} streams, autotuned tiles: {wallb:.0f} ms; tile rows "
+        f"{sorted(set(r['tile_rows'] for r in rb))}")
+    same = all(np.array_equal(a["labels"], b["labels"]) and np.array_equal(a["ordering"], b["ordering"])
+               and np.array_equal(a["reach"], b["reach"]) for a, b in zip(r1, rb))
+    tot = int(ns.sum())
+    print(json.dumps({
+        "metric": "points/sec, 64-dataset sweep (8 eps x 2 minPts + sOPTICS), FP32", "value": tot / (wallb * 1e-3),
+        "unit": "points/sec", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"{nds} datasets, n log-uniform [2000, 500000] (sum {tot}), d in {{32,128,784}}, "
+                               "eps 8 x U[0.1, 0.4], minPts {10, 50}, sOPTICS at max eps, n_proj 1024, topK 10, "
+                               "topM 50", "streams": args.streams},
+        "wall_ms": wallb, "one_by_one_wall_ms": wall1, "one_by_one_points_per_sec": tot / (wall1 * 1e-3),
+        "speedup_vs_one_by_one": wall1 / wallb, "outputs_identical": same}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +135,10 @@ def main():
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", action="store_true",
+                    help="run the many-dataset sweep (sdb_batch_run) instead of the 8M x 784 step")
+    ap.add_argument("--batch-datasets", type=int, default=64)
+    ap.add_argument("--streams", type=int, default=8)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 4 GiB device-to-device copy (profiler counter calibration)")
     args = ap.parse_args()
@@ -105,6 +150,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": src.numel() * 4, "copies": 3}))
+        return
+    if args.batch:
+        batch_bench(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "

@@ -15,6 +15,10 @@
  *   sdb_core_dist      <- rp_findCoreDist (src/sDbscan.cpp:1314-1474)
  *   sdb_optics_reach   <- the reachability values of formOptics_scikit
  *                         (src/sDbscan.cpp:1588-1703) for a given ordering
+ *   sdb_optics_order   <- formOptics_scikit (src/sDbscan.cpp:1588-1703): ordering
+ *                         and reachability
+ *   sdb_batch_run      <- test_sDbscan's eps sweep (src/sDbscan.cpp:980-1039) over
+ *                         many datasets at once (no batch entry in the reference)
  *   sdb_get_index      <- the protected matrix_topK / matrix_topM (src/sDbscan.h:54-55)
  *
  * All entry points return 0 on success and a nonzero SDB_E* code otherwise;
@@ -160,6 +164,66 @@ int sdb_generate(sdb_ctx* ctx, int kind, long long n, int d, unsigned long long 
 
 /* Copy rows [row0, row0 + rows) of the current input to the host. */
 int sdb_copy_input(sdb_ctx* ctx, long long row0, long long rows, float* out);
+
+/* sOPTICS at the eps of the last sdb_find_core: formOptics_scikit
+ * (src/sDbscan.cpp:1588-1703) walks the device neighbourhoods on the host
+ * (ascending start points, lazy min-heap on reach) and writes the ordering
+ * (n ints); reach_out (n floats, may be NULL) gets the reachability values
+ * of sdb_optics_reach.  minPts as in sdb_core_dist (at least 2 is used). */
+int sdb_optics_order(sdb_ctx* ctx, int minPts, int* ordering_out, float* reach_out);
+
+/* The same for several minPts at the eps of the last sdb_find_core: the
+ * neighbourhoods are fetched once and the host walks of the settings run on
+ * parallel threads.  orderings (and reaches, may be NULL) hold n_minpts
+ * consecutive runs of n values.  After sdb_sng_find_core only n_minpts = 1
+ * is accepted (its core flags stand). */
+int sdb_optics_orders(sdb_ctx* ctx, int n_minpts, const int* minPts, int* orderings, float* reaches);
+
+/* Rows of the (point, list) entries per distance tile; 0 restores the
+ * default for the row length.  No counterpart in the reference. */
+int sdb_set_tile_rows(sdb_ctx* ctx, int rows);
+
+/* Tile-shape autotuner: runs the distance pass of the built index at eps
+ * once per candidate row count, keeps the fastest on the context and returns
+ * it (and its kernel time).  The next sdb_find_core recomputes distances.
+ * No counterpart in the reference. */
+int sdb_tune_tile_rows(sdb_ctx* ctx, float eps, const int* candidates, int n_candidates, int* best_rows,
+                       double* best_ms);
+
+/* Batched sweeps (no counterpart in the reference, whose test_sDbscan sweeps
+ * the eps values of one dataset, src/sDbscan.cpp:980-1039).  Each job is one
+ * dataset with its own parameters and random structures, swept over n_eps
+ * eps values x n_minpts minPts values (labels of every pair, as fit_sDbscan)
+ * and, with optics set, sOPTICS at the largest eps for every minPts (as
+ * fit_sOptics).  The index is built once per job, the distances once per eps
+ * (the minPts settings reuse them).  n_streams contexts, one HIP stream each,
+ * take jobs from a shared queue; with autotune set the first job of every
+ * (n_features, metric, precision) class tunes the tile rows for the rest.
+ * Output arrays are caller-owned and may be NULL. */
+typedef struct sdb_batch_job {
+    const float* X;            /* n x d host rows (the reference's d x n column-major) */
+    sdb_params params;         /* n_points / n_features are n and d */
+    const uint8_t* hd3_bits;   /* 3 * fhtDim sign bytes (bitHD3Generator) */
+    const float* R;            /* (ker_n_features / 2) x d for L1 / L2, else NULL */
+    int n_eps;
+    const float* eps;          /* n_eps values */
+    int n_minpts;
+    const int* minPts;         /* n_minpts values */
+    int optics;                /* 1: sOPTICS at max(eps) for every minPts */
+    int* labels;               /* [n_eps][n_minpts][n] */
+    int* n_clusters;           /* [n_eps][n_minpts] */
+    int* ordering;             /* [n_minpts][n] (optics) */
+    float* reach;              /* [n_minpts][n] (optics) */
+    int status;                /* out: SDB_OK or the job's error code */
+    int tile_rows;             /* out: distance tile rows used */
+    double ms;                 /* out: wall time of the job */
+    char error[192];           /* out: message when status != SDB_OK */
+} sdb_batch_job;
+
+/* Runs the jobs; returns SDB_OK when every job succeeded, else the first
+ * failing job's code (each job keeps its own status).  wall_ms (may be NULL)
+ * gets the wall time of the whole batch. */
+int sdb_batch_run(int device, sdb_batch_job* jobs, int n_jobs, int n_streams, int autotune, double* wall_ms);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,16 @@ class Stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BatchJob(ctypes.Structure):
+    """sdb_batch_job of include/sdbscan_amd.h."""
+    _fields_ = [("X", ctypes.c_void_p), ("params", Params), ("hd3_bits", ctypes.c_void_p),
+                ("R", ctypes.c_void_p), ("n_eps", ctypes.c_int), ("eps", ctypes.c_void_p),
+                ("n_minpts", ctypes.c_int), ("minPts", ctypes.c_void_p), ("optics", ctypes.c_int),
+                ("labels", ctypes.c_void_p), ("n_clusters", ctypes.c_void_p),
+                ("ordering", ctypes.c_void_p), ("reach", ctypes.c_void_p), ("status", ctypes.c_int),
+                ("tile_rows", ctypes.c_int), ("ms", ctypes.c_double), ("error", ctypes.c_char * 192)]
+
+
 _P = ctypes.c_void_p
 _SIGS = {
     "sdb_create": ([ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
@@ -66,6 +76,13 @@ _SIGS = {
     "sdb_get_neighbors": ([_P, _P, _P, _P], ctypes.c_int),
     "sdb_core_dist": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "sdb_optics_reach": ([_P, _P, _P], ctypes.c_int),
+    "sdb_optics_order": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
+    "sdb_optics_orders": ([_P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+    "sdb_set_tile_rows": ([_P, ctypes.c_int], ctypes.c_int),
+    "sdb_tune_tile_rows": ([_P, ctypes.c_float, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "sdb_batch_run": ([ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                       ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "sdb_get_stats": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
     "sdb_generate": ([_P, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_ulonglong,
                       ctypes.c_float], ctypes.c_int),
@@ -258,6 +275,38 @@ class Context:
         self._chk(self.lib.sdb_optics_reach(self.h, ordering.ctypes.data, out.ctypes.data))
         return out
 
+    def optics_order(self, minPts):
+        """sOPTICS ordering and reachability at the last find_core's eps."""
+        import numpy as np
+        n = self.params.n_points
+        order = np.zeros(n, np.int32)
+        reach = np.zeros(n, np.float32)
+        self._chk(self.lib.sdb_optics_order(self.h, minPts, order.ctypes.data, reach.ctypes.data))
+        return order, reach
+
+    def optics_orders(self, minPts):
+        """optics_order for several minPts at once: arrays [len(minPts), n]."""
+        import numpy as np
+        n = self.params.n_points
+        mp = np.ascontiguousarray(minPts, dtype=np.int32)
+        order = np.zeros((len(mp), n), np.int32)
+        reach = np.zeros((len(mp), n), np.float32)
+        self._chk(self.lib.sdb_optics_orders(self.h, len(mp), mp.ctypes.data, order.ctypes.data,
+                                             reach.ctypes.data))
+        return order, reach
+
+    def set_tile_rows(self, rows):
+        self._chk(self.lib.sdb_set_tile_rows(self.h, rows))
+
+    def tune_tile_rows(self, eps, candidates):
+        import numpy as np
+        cand = np.ascontiguousarray(candidates, dtype=np.int32)
+        best = ctypes.c_int()
+        ms = ctypes.c_double()
+        self._chk(self.lib.sdb_tune_tile_rows(self.h, eps, cand.ctypes.data, len(cand), ctypes.byref(best),
+                                              ctypes.byref(ms)))
+        return best.value, ms.value
+
     def stats(self):
         s = Stats()
         self._chk(self.lib.sdb_get_stats(self.h, ctypes.byref(s)))
@@ -268,3 +317,56 @@ class Context:
         out = np.zeros((rows, self.params.n_features), np.float32)
         self._chk(self.lib.sdb_copy_input(self.h, row0, rows, out.ctypes.data))
         return out
+
+
+def batch_run(datasets, eps, minPts, optics=True, n_streams=4, autotune=True, device=0, n_proj=1024, topK=10,
+              topM=50, metric=COSINE, seed=42, precision=FP32, hd3=None):
+    """Run sdb_batch_run over a list of (n x d) float32 arrays, every dataset
+    swept over the same eps and minPts values.  ``hd3(fhtDim, seed)`` returns
+    the 3 * fhtDim sign bytes (the reference's bitHD3Generator); it is
+    required.  Returns (results, wall_ms) with one dict per dataset:
+    labels [n_eps, n_minpts, n], n_clusters [n_eps, n_minpts], ordering and
+    reach [n_minpts, n] (optics), tile_rows and ms."""
+    import numpy as np
+    if hd3 is None:
+        raise ValueError("hd3(fhtDim, seed) -> sign bytes is required")
+    lib = capi()
+    eps_a = np.ascontiguousarray(eps, dtype=np.float32)
+    mp_a = np.ascontiguousarray(minPts, dtype=np.int32)
+    jobs = (BatchJob * len(datasets))()
+    keep = []
+    out = []
+    for i, X in enumerate(datasets):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n, d = X.shape
+        ker = d if metric == COSINE else 1024
+        D = fht_dim(n_proj, ker)
+        bits = np.ascontiguousarray(hd3(3 * D, seed), dtype=np.uint8)
+        lab = np.zeros((len(eps_a), len(mp_a), n), np.int32)
+        ncl = np.zeros((len(eps_a), len(mp_a)), np.int32)
+        order = np.zeros((len(mp_a), n), np.int32)
+        reach = np.zeros((len(mp_a), n), np.float32)
+        keep += [X, bits, lab, ncl, order, reach]
+        j = jobs[i]
+        j.X = X.ctypes.data
+        j.params = Params(n, d, n_proj, topK, topM, metric, ker, 1.0, 0.4, D, precision)
+        j.hd3_bits = bits.ctypes.data
+        j.R = None
+        j.n_eps, j.eps = len(eps_a), eps_a.ctypes.data
+        j.n_minpts, j.minPts = len(mp_a), mp_a.ctypes.data
+        j.optics = 1 if optics else 0
+        j.labels, j.n_clusters = lab.ctypes.data, ncl.ctypes.data
+        j.ordering, j.reach = order.ctypes.data, reach.ctypes.data
+        out.append({"labels": lab, "n_clusters": ncl, "ordering": order, "reach": reach})
+    wall = ctypes.c_double()
+    rc = lib.sdb_batch_run(device, ctypes.cast(jobs, ctypes.c_void_p), len(datasets), n_streams,
+                           1 if autotune else 0, ctypes.byref(wall))
+    for i in range(len(datasets)):
+        out[i]["status"] = jobs[i].status
+        out[i]["error"] = jobs[i].error.decode()
+        out[i]["tile_rows"] = jobs[i].tile_rows
+        out[i]["ms"] = jobs[i].ms
+    if rc != 0:
+        bad = [o["error"] for o in out if o["status"] != 0]
+        raise SdbError(rc, "; ".join(bad[:3]))
+    return out, wall.value

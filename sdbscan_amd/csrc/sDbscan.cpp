@@ -42,14 +42,6 @@ unsigned seed_of(int random_seed) {
     return s;
 }
 
-// (index, value) pair ordered by value only (src/Header.h:48-75, :100).
-struct OPair {
-    int idx;
-    float val;
-    OPair(int i, double v) : idx(i), val((float)v) {}
-    bool operator>(const OPair& o) const { return val > o.val; }
-};
-typedef std::priority_queue<OPair, std::vector<OPair>, std::greater<OPair>> OMinHeap;
 
 }  // namespace
 
@@ -358,49 +350,10 @@ void sDbscan::fit_sOptics(const float* X, float eps, int minPts) {
 // neighbourhoods: the ordering is the sequential heap walk on the host, the
 // reachability values are computed on the device for that ordering.
 void sDbscan::optics_order(const std::vector<uint8_t>& core, int minPts) {
-    std::vector<long long> ptr(n_points + 1);
-    check(sdb_get_neighbors(ctx, ptr.data(), nullptr, nullptr), "neighbors");
-    std::vector<int> idx(std::max<long long>(ptr[n_points], 1));
-    std::vector<float> dd(std::max<long long>(ptr[n_points], 1));
-    check(sdb_get_neighbors(ctx, ptr.data(), idx.data(), dd.data()), "neighbors");
-    std::vector<float> cdist(n_points);
-    check(sdb_core_dist(ctx, std::max(minPts, 2), cdist.data()), "core_dist");
-
-    const float NEG = -2147483648.0f;  // src/Header.h:20
-    std::vector<float> reach(n_points, NEG);
-    std::vector<uint8_t> done(n_points, 0);
-    ordering_.clear();
-    ordering_.reserve(n_points);
-    auto expand = [&](int xi, OMinHeap& seeds) {
-        const float cd = cdist[xi];
-        for (long long j = ptr[xi]; j < ptr[xi + 1]; ++j) {
-            const int xj = idx[j];
-            if (done[xj]) continue;
-            const float r = std::max(cd, dd[j]);
-            if (reach[xj] == NEG || reach[xj] > r) {
-                reach[xj] = r;
-                seeds.emplace(xj, reach[xj]);
-            }
-        }
-    };
-    for (int xi = 0; xi < n_points; ++xi) {
-        if (done[xi]) continue;
-        done[xi] = 1;
-        ordering_.push_back(xi);
-        if (!core[xi]) continue;
-        OMinHeap seeds;
-        expand(xi, seeds);
-        while (!seeds.empty()) {
-            const int xj = seeds.top().idx;
-            seeds.pop();
-            if (done[xj]) continue;
-            done[xj] = 1;
-            ordering_.push_back(xj);
-            if (core[xj]) expand(xj, seeds);
-        }
-    }
-    reachability_.assign(n_points, NEG);
-    check(sdb_optics_reach(ctx, ordering_.data(), reachability_.data()), "optics_reach");
+    (void)core;  // the device core flags of the last find_core are the same
+    ordering_.assign(n_points, 0);
+    reachability_.assign(n_points, 0.0f);
+    check(sdb_optics_order(ctx, minPts, ordering_.data(), reachability_.data()), "optics_order");
 }
 
 // ---------------------------------------------------------------------------

@@ -12,6 +12,7 @@
 #include <functional>
 #include <queue>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sdbscan_amd.h"
@@ -95,6 +96,7 @@ struct sdb_ctx {
     // find_core
     DevBuf E, colsum, queue, qcount, tiles, degree, core, stats64;
     long long qcap = 0;
+    int tile_rows = 0;   // distance tile rows (0: the default for the row length)
     bool cored = false;
     float eps = 0.f;
     int minPts = 0;
@@ -103,6 +105,7 @@ struct sdb_ctx {
     // neighbourhoods / optics
     DevBuf nb_deg, nb_ptr, nb_cur, nb_idx, nb_dist, nb_sorted, core_dist, pos, reach;
     bool have_nb = false;
+    bool sorted_dist = false;  // nb_sorted holds the row-sorted neighbour distances
     bool clustered = false;
     // noise labelling
     DevBuf ns_ids, cproj, noise, nbest;
@@ -693,6 +696,7 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
         da.nchunk = (da.dk + da.kc - 1) / da.kc;
         QB = da.nchunk == 1 ? 4096 : 256;
         if (const char* e = std::getenv("SDB_TILE_ROWS")) QB = std::max(256, std::atoi(e));
+        if (c->tile_rows > 0) QB = c->tile_rows;
         QB = std::min(QB, 16 * 8 * 255);  // per-lane byte counters of the single-chunk tiles
     } else {
         int kc = (kLdsBytes - 8 * M) / (4 * M + 128);
@@ -778,6 +782,26 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
     const long long n = c->n;
     const int M = c->Mt, W = c->W, S = c->S, L2 = c->L2;
     hipStream_t st = c->st;
+    if (minPts < 1) return fail(c, SDB_EINVAL, "minPts must be positive");
+    if (c->cored && c->eps == eps) {
+        // same eps: the hit masks and degrees stand; only the core test moves
+        HIPCHK(hipEventRecord(c->ev[3], st));
+        HIPCHK(hipMemsetAsync((unsigned long long*)c->stats64.p + 3, 0, 8, st));
+        LCHK(launch_core_minpts(P_<int>(c->degree), n, minPts, P_<uint8_t>(c->core),
+                                (unsigned long long*)c->stats64.p + 3, st));
+        HIPCHK(hipEventRecord(c->ev[6], st));
+        unsigned long long nc = 0;
+        HIPCHK(hipMemcpyAsync(&nc, (unsigned long long*)c->stats64.p + 3, 8, hipMemcpyDeviceToHost, st));
+        if (core_out) HIPCHK(hipMemcpyAsync(core_out, c->core.p, (size_t)n, hipMemcpyDeviceToHost, st));
+        if (degree_out) HIPCHK(hipMemcpyAsync(degree_out, c->degree.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        c->stats.t_core_ms = elapsed(c->ev[3], c->ev[6]);
+        c->stats.t_dist_ms = 0.f;
+        c->stats.n_core = (long long)nc;
+        c->minPts = minPts;
+        c->clustered = false;
+        return SDB_OK;
+    }
     int nq = 0;
     double band = 0, band32 = 0;
     int r = run_distances(c, eps, &nq, &band, &band32);
@@ -992,6 +1016,7 @@ static int build_neighbors(sdb_ctx* c) {
     }
     HIPCHK(hipStreamSynchronize(st));
     c->have_nb = true;
+    c->sorted_dist = false;
     return 0;
 }
 
@@ -1022,7 +1047,7 @@ int sdb_core_dist(sdb_ctx* c, int minPts, float* core_dist_out) {
     HIPCHK(hipMemcpy(&tot, P_<long long>(c->nb_ptr) + n, 8, hipMemcpyDeviceToHost));
     ENSURE(c->core_dist, (size_t)n * 4);
     float* sorted = (float*)c->nb_sorted.p;
-    if (tot > 0) {
+    if (tot > 0 && !c->sorted_dist) {
         size_t tb = 0;
         hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, P_<float>(c->nb_dist), sorted, (int)tot, (int)n,
                                                    P_<long long>(c->nb_ptr), P_<long long>(c->nb_ptr) + 1, 0, 32, st);
@@ -1033,6 +1058,7 @@ int sdb_core_dist(sdb_ctx* c, int minPts, float* core_dist_out) {
         HIPCHK(hipStreamSynchronize(st));
         hipFree(tmp.p);
         if (e != hipSuccess) return fail(c, SDB_EHIP, "segmented sort failed: %s", hipGetErrorString(e));
+        c->sorted_dist = true;
     }
     LCHK(launch_kth(n, P_<long long>(c->nb_ptr), sorted, P_<uint8_t>(c->core), minPts - 2, P_<float>(c->core_dist), st));
     HIPCHK(hipMemcpyAsync(core_dist_out, c->core_dist.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
@@ -1059,6 +1085,143 @@ int sdb_optics_reach(sdb_ctx* c, const int* ordering, float* reach_out) {
                       P_<float>(c->core_dist), P_<int>(c->pos), P_<uint32_t>(c->reach), st));
     HIPCHK(hipMemcpyAsync(reach_out, c->reach.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return SDB_OK;
+}
+
+// formOptics_scikit (src/sDbscan.cpp:1588-1703) on the host over the device
+// neighbourhoods: points in ascending order start a walk; a core point pushes
+// its unprocessed neighbours with reach max(core_dist, d) into a lazy
+// min-heap keyed on the FP32 reach only (ties in the heap's own order, as the
+// reference's priority_queue of (index, value) pairs ordered by value).
+namespace {
+struct ReachPair {
+    int idx;
+    float val;
+    ReachPair(int i, double v) : idx(i), val((float)v) {}
+    bool operator>(const ReachPair& o) const { return val > o.val; }
+};
+typedef std::priority_queue<ReachPair, std::vector<ReachPair>, std::greater<ReachPair>> ReachHeap;
+}  // namespace
+
+static void optics_walk(long long n, const long long* ptr, const int* idx, const float* dd, const uint8_t* core,
+                        const float* cdist, int* ordering_out) {
+    const float NEG = -2147483648.0f;  // src/Header.h:20
+    std::vector<float> reach(n, NEG);
+    std::vector<uint8_t> done(n, 0);
+    long long k = 0;
+    auto expand = [&](int xi, ReachHeap& seeds) {
+        const float cd = cdist[xi];
+        for (long long j = ptr[xi]; j < ptr[xi + 1]; ++j) {
+            const int xj = idx[j];
+            if (done[xj]) continue;
+            const float rr = std::max(cd, dd[j]);
+            if (reach[xj] == NEG || reach[xj] > rr) {
+                reach[xj] = rr;
+                seeds.emplace(xj, reach[xj]);
+            }
+        }
+    };
+    for (long long xi = 0; xi < n; ++xi) {
+        if (done[xi]) continue;
+        done[xi] = 1;
+        ordering_out[k++] = (int)xi;
+        if (!core[xi]) continue;
+        ReachHeap seeds;
+        expand((int)xi, seeds);
+        while (!seeds.empty()) {
+            const int xj = seeds.top().idx;
+            seeds.pop();
+            if (done[xj]) continue;
+            done[xj] = 1;
+            ordering_out[k++] = xj;
+            if (core[xj]) expand(xj, seeds);
+        }
+    }
+}
+
+int sdb_optics_orders(sdb_ctx* c, int n_minpts, const int* minPts, int* orderings, float* reaches) {
+    if (!c || n_minpts < 1 || !minPts || !orderings) return SDB_EINVAL;
+    if (!c->cored) return fail(c, SDB_ESTATE, "find_core first");
+    // sample-sequence neighbourhoods carry the core flags of their own call
+    if (c->sng && n_minpts != 1) return fail(c, SDB_EUNSUPPORTED, "one minPts per call after sng_find_core");
+    const long long n = c->n;
+    const float eps = c->eps;
+    const int keep = c->minPts;
+    // the neighbourhoods of eps do not depend on minPts: fetched once
+    std::vector<long long> ptr(n + 1);
+    int r = sdb_get_neighbors(c, ptr.data(), nullptr, nullptr);
+    if (r) return r;
+    std::vector<int> idx(std::max<long long>(ptr[n], 1));
+    std::vector<float> dd(std::max<long long>(ptr[n], 1));
+    r = sdb_get_neighbors(c, ptr.data(), idx.data(), dd.data());
+    if (r) return r;
+    std::vector<std::vector<uint8_t>> core(n_minpts, std::vector<uint8_t>(n));
+    std::vector<std::vector<float>> cdist(n_minpts, std::vector<float>(n));
+    for (int m = 0; m < n_minpts; ++m) {
+        if (c->sng) {
+            HIPCHK(hipMemcpy(core[m].data(), c->core.p, (size_t)n, hipMemcpyDeviceToHost));
+        } else {
+            r = sdb_find_core(c, eps, minPts[m], core[m].data(), nullptr);
+            if (r) return r;
+        }
+        r = sdb_core_dist(c, std::max(minPts[m], 2), cdist[m].data());
+        if (r) return r;
+    }
+    // the walks are serial each; the settings run side by side
+    std::vector<std::thread> th;
+    for (int m = 1; m < n_minpts; ++m)
+        th.emplace_back(optics_walk, n, ptr.data(), idx.data(), dd.data(), core[m].data(), cdist[m].data(),
+                        orderings + (size_t)m * n);
+    optics_walk(n, ptr.data(), idx.data(), dd.data(), core[0].data(), cdist[0].data(), orderings);
+    for (auto& t : th) t.join();
+    if (reaches) {
+        for (int m = 0; m < n_minpts; ++m) {
+            // core flags and core distances of this setting back on the device
+            HIPCHK(hipMemcpyAsync(c->core.p, core[m].data(), (size_t)n, hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipMemcpyAsync(c->core_dist.p, cdist[m].data(), (size_t)n * 4, hipMemcpyHostToDevice, c->st));
+            r = sdb_optics_reach(c, orderings + (size_t)m * n, reaches + (size_t)m * n);
+            if (r) return r;
+        }
+    }
+    if (c->sng) return SDB_OK;
+    return sdb_find_core(c, eps, keep, nullptr, nullptr);  // the core flags of the last find_core again
+}
+
+int sdb_optics_order(sdb_ctx* c, int minPts, int* ordering_out, float* reach_out) {
+    return sdb_optics_orders(c, 1, &minPts, ordering_out, reach_out);
+}
+
+int sdb_set_tile_rows(sdb_ctx* c, int rows) {
+    if (!c) return SDB_EINVAL;
+    if (rows < 0) return fail(c, SDB_EINVAL, "tile rows must be >= 0");
+    c->tile_rows = rows == 0 ? 0 : std::min(std::max(rows, 64), 16 * 8 * 255);
+    return SDB_OK;
+}
+
+int sdb_tune_tile_rows(sdb_ctx* c, float eps, const int* cand, int ncand, int* best_rows, double* best_ms) {
+    if (!c || !cand || ncand < 1) return SDB_EINVAL;
+    if (!c->indexed || c->sng) return fail(c, SDB_ESTATE, "build_index first");
+    HIPCHK(hipSetDevice(c->device));
+    int best = 0;
+    double bms = 0.0;
+    for (int i = 0; i < ncand; ++i) {
+        if (cand[i] <= 0) return fail(c, SDB_EINVAL, "tile row candidates must be positive");
+        int r = sdb_set_tile_rows(c, cand[i]);
+        if (r) return r;
+        int nq = 0;
+        double band = 0, band32 = 0;
+        r = run_distances(c, eps, &nq, &band, &band32);
+        if (r) return r;
+        HIPCHK(hipStreamSynchronize(c->st));
+        const double ms = c->stats.t_dist_ms;
+        if (i == 0 || ms < bms) { bms = ms; best = c->tile_rows; }
+    }
+    c->tile_rows = best;
+    c->cored = false;  // the hit masks belong to the last candidate; degrees were not counted
+    c->have_nb = false;
+    c->clustered = false;
+    if (best_rows) *best_rows = best;
+    if (best_ms) *best_ms = bms;
     return SDB_OK;
 }
 
@@ -1141,6 +1304,7 @@ static int build_neighbors_sng(sdb_ctx* c) {
                              P_<int>(c->nb_idx), P_<float>(c->nb_dist), st));
     HIPCHK(hipStreamSynchronize(st));
     c->have_nb = true;
+    c->sorted_dist = false;
     return 0;
 }
 

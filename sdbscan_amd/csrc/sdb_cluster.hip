@@ -252,7 +252,23 @@ __global__ void k_labels(ClusterArgs c, const int* rank) {
     c.labels[a] = lab;
 }
 
+// Core test of new minPts on degrees already counted for the same eps.
+__global__ void k_core_minpts(const int* __restrict__ degree, long long n, int minPts, uint8_t* __restrict__ core,
+                              unsigned long long* __restrict__ n_core) {
+    const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool is_core = a < n && degree[a] >= minPts - 1;
+    if (a < n) core[a] = is_core ? 1 : 0;
+    const unsigned long long b = __ballot(is_core);
+    if (lane_id() == 0 && b) atomicAdd(n_core, (unsigned long long)__popcll(b));
+}
+
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int launch_core_minpts(const int* degree, long long n, int minPts, uint8_t* core, unsigned long long* n_core,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_core_minpts, dim3(nblk(n, 256)), dim3(256), 0, st, degree, n, minPts, core, n_core);
+    return (int)hipGetLastError();
+}
 
 int launch_degree(const ClusterArgs& c, hipStream_t st) {
     hipLaunchKernelGGL(k_degree, dim3(nblk(c.n, 256)), dim3(256), 0, st, c);

@@ -133,6 +133,8 @@ int launch_degree(const ClusterArgs& c, hipStream_t st);
 int launch_corehub(const ClusterArgs& c, hipStream_t st);
 int launch_uf_init(const ClusterArgs& c, hipStream_t st);
 int launch_uf_lists(const ClusterArgs& c, hipStream_t st);
+int launch_core_minpts(const int* degree, long long n, int minPts, uint8_t* core, unsigned long long* n_core,
+                       hipStream_t st);
 int launch_uf_points(const ClusterArgs& c, hipStream_t st);
 int launch_uf_compress(const ClusterArgs& c, hipStream_t st);
 int launch_root_flags(const ClusterArgs& c, int* flags, hipStream_t st);

@@ -4,6 +4,8 @@ mixture()   -- 10-component sparse nonnegative mixture, L2-normalised
                (the 20k x 784 cosine workload and its 8M x 784 scale-up).
 blobs()     -- anisotropic Gaussian blobs plus uniform noise (L2 workload).
 dirichlet() -- Dirichlet histograms with cluster-specific boosted bins (Chi2).
+sphere_mixture() -- Gaussian mixture normalised to the unit sphere (the
+               many-dataset batch sweep).
 """
 import numpy as np
 
@@ -49,4 +51,18 @@ def dirichlet(n, d=784, k=10, base=0.05, boost=2.0, n_boost=20, seed=1004):
     for c in range(k):
         idx = np.where(comp == c)[0]
         X[idx] = rng.dirichlet(alphas[c], len(idx)).astype(np.float32)
+    return X, comp
+
+
+def sphere_mixture(n, d, k, seed, spread=0.35):
+    """k Gaussian components around random unit centres, each point
+    normalised to the unit sphere (float32, n x d)."""
+    rng = np.random.default_rng(seed)
+    centers = rng.normal(size=(k, d))
+    centers /= np.linalg.norm(centers, axis=1, keepdims=True)
+    comp = rng.integers(0, k, n)
+    X = rng.standard_normal((n, d), dtype=np.float32)
+    X *= np.float32(spread / np.sqrt(d))
+    X += centers.astype(np.float32)[comp]
+    X /= np.linalg.norm(X, axis=1, keepdims=True)
     return X, comp

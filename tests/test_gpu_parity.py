@@ -245,6 +245,70 @@ def test_optics_core_dist_and_reachability(ctx):
     dreach = ctx.optics_reach(order)
     print("optics: unreached", int((reach == -2147483648.0).sum()), flush=True)
     assert np.array_equal(dreach, reach)
+    # the ordering built by the library over the device neighbourhoods
+    dorder, dreach2 = ctx.optics_order(minPts)
+    assert np.array_equal(dorder, order)
+    assert np.array_equal(dreach2, reach)
+
+
+def test_find_core_minpts_reuses_distances(ctx):
+    """A second find_core at the same eps only moves the core test; its
+    results equal a fresh index's."""
+    X, _ = datagen.mixture(4000, 128, k=5, sigma=0.3, seed=141)
+    print("minPts reuse n=4000 d=128", flush=True)
+    run_index(ctx, X, K=6, M=40, n_proj=256)
+    eps = 0.3
+    c1, d1 = ctx.find_core(eps, 12)
+    c2, d2 = ctx.find_core(eps, 25)
+    lab2, nc2 = ctx.form_clusters()
+    assert np.array_equal(d1, d2)
+    assert np.array_equal(c2, (d1 >= 24).astype(np.uint8))
+    assert 0 < int(c2.sum()) < int(c1.sum())
+    run_index(ctx, X, K=6, M=40, n_proj=256)
+    c3, d3 = ctx.find_core(eps, 25)
+    lab3, nc3 = ctx.form_clusters()
+    assert np.array_equal(c2, c3) and np.array_equal(d2, d3)
+    assert np.array_equal(lab2, lab3) and nc2 == nc3
+
+
+def test_tile_rows_autotune_keeps_results(ctx):
+    X, _ = datagen.mixture(6000, 784, seed=151)
+    print("tile rows autotune n=6000 d=784", flush=True)
+    for precision in (S.FP32, S.BF16):
+        run_index(ctx, X, K=5, M=50, precision=precision)
+        core0, deg0 = ctx.find_core(0.25, 20)
+        lab0, nc0 = ctx.form_clusters()
+        best, ms = ctx.tune_tile_rows(0.25, [256, 1024, 4096])
+        assert best in (256, 1024, 4096) and ms > 0
+        core1, deg1 = ctx.find_core(0.25, 20)
+        lab1, nc1 = ctx.form_clusters()
+        assert np.array_equal(core0, core1) and np.array_equal(deg0, deg1)
+        assert np.array_equal(lab0, lab1) and nc0 == nc1
+        ctx.set_tile_rows(0)
+
+
+def test_batch_matches_per_dataset_runs(ctx):
+    """sdb_batch_run (3 streams, autotuned tiles) gives every dataset's labels,
+    cluster counts, OPTICS ordering and reachability of one-by-one runs."""
+    shapes = [(2500, 32), (4000, 128), (3000, 784), (6000, 32), (1500, 128)]
+    data = [datagen.mixture(n, d, k=4, sigma=0.3, seed=170 + i)[0] for i, (n, d) in enumerate(shapes)]
+    eps, minPts = [0.3, 0.2], [10, 20]
+    print("batch 5 datasets x 2 eps x 2 minPts", flush=True)
+    res, wall = S.batch_run(data, eps, minPts, optics=True, n_streams=3, n_proj=256, topK=5, topM=40,
+                            hd3=O.hd3_bits)
+    assert wall > 0
+    for X, r in zip(data, res):
+        assert r["status"] == 0 and r["tile_rows"] >= 0
+        run_index(ctx, X, K=5, M=40, n_proj=256)
+        for ie, e in enumerate(eps):
+            for im, mp in enumerate(minPts):
+                ctx.find_core(e, mp)
+                lab, nc = ctx.form_clusters()
+                assert np.array_equal(r["labels"][ie, im], lab) and r["n_clusters"][ie, im] == nc
+        for im, mp in enumerate(minPts):
+            ctx.find_core(max(eps), mp)
+            o, rr = ctx.optics_order(mp)
+            assert np.array_equal(r["ordering"][im], o) and np.array_equal(r["reach"][im], rr)
 
 
 def test_label_noise_matches_oracle(ctx):
