@@ -94,7 +94,6 @@ def batch_bench(args):
     batched run (several streams, autotuned tile rows) against the same jobs
     one dataset at a time (one stream, default tiles)."""
     import sdbscan_amd as S
-    from oracle import oracle as O
     from sdbscan_amd import datagen
     rng = np.random.default_rng(1003)
     nds = args.batch_datasets
@@ -107,7 +106,7 @@ def batch_bench(args):
     data = [datagen.sphere_mixture(int(n), int(d), int(k), seed=2000 + i)[0]
             for i, (n, d, k) in enumerate(zip(ns, ds, ks))]
     log(f"[batch] generated {nds} datasets, {int(ns.sum())} points in {time.perf_counter() - t0:.1f} s")
-    kw = dict(optics=True, n_proj=P, topK=10, topM=50, hd3=O.hd3_bits)
+    kw = dict(optics=True, n_proj=P, topK=10, topM=50, hd3=S.hd3_bits)
     S.batch_run(data[:2], eps, minPts, n_streams=2, autotune=False, **kw)  # warm-up
     r1, wall1 = S.batch_run(data, eps, minPts, n_streams=1, autotune=False, **kw)
     log(f"[batch] one dataset at a time: {wall1:.0f} ms")
@@ -160,12 +159,11 @@ def main():
         sys.exit(2)
 
     import sdbscan_amd as S
-    from oracle import oracle as O
 
     n = args.n
     ctx = S.Context(0)
     p = ctx.set_params(n, D_FEAT, n_proj=P, topK=K, topM=M, metric=S.COSINE, precision=S.BF16)
-    bits = O.hd3_bits(3 * p.fhtDim, SEED_RNG)
+    bits = S.hd3_bits(3 * p.fhtDim, SEED_RNG)
     t0 = time.perf_counter()
     ctx.generate(n, D_FEAT, SEED_DATA, 0.1)
     log(f"generated {n} x {D_FEAT} on device in {time.perf_counter() - t0:.1f} s")

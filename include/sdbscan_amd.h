@@ -80,6 +80,17 @@ typedef struct sdb_stats {
     double band32;                /* error band of the FP32 recheck */
 } sdb_stats;
 
+/* Host random structures, the reference's std:: calls (no device work):
+ * sdb_hd3_bits  <- bitHD3Generator (src/Utilities.h:47-64): n_bits bytes of
+ *                  uniform_int_distribution<uint32_t>(0, 1) over
+ *                  default_random_engine(seed);
+ * sdb_rff_matrix <- gaussGenerator / cauchyGenerator (src/Utilities.h:76-122):
+ *                  the (ker_n_features / 2) x d matrix R (column c holds
+ *                  half consecutive values) of N(0, 1/sigma) (SDB_L2) or
+ *                  Cauchy(0, 1/sigma) (SDB_L1) draws. */
+int sdb_hd3_bits(int n_bits, unsigned int seed, uint8_t* out);
+int sdb_rff_matrix(int metric, int half, int d, float sigma, unsigned int seed, float* out);
+
 /* Context lifecycle (one HIP stream per context). */
 int sdb_create(int device, sdb_ctx** out);
 void sdb_destroy(sdb_ctx* ctx);

@@ -58,6 +58,9 @@ class BatchJob(ctypes.Structure):
 
 _P = ctypes.c_void_p
 _SIGS = {
+    "sdb_hd3_bits": ([ctypes.c_int, ctypes.c_uint, _P], ctypes.c_int),
+    "sdb_rff_matrix": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, _P],
+                       ctypes.c_int),
     "sdb_create": ([ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "sdb_destroy": ([_P], None),
     "sdb_last_error": ([_P], ctypes.c_char_p),
@@ -122,6 +125,26 @@ def fht_dim(n_proj, ker_n_features):
     """fhtDim of src/sDbscan.h:107."""
     import math
     return 1 << int(math.ceil(math.log2(max(n_proj, ker_n_features))))
+
+
+def hd3_bits(n_bits, seed):
+    """The 3 * fhtDim HD3 sign bytes of bitHD3Generator (host std:: calls)."""
+    import numpy as np
+    out = np.zeros(n_bits, np.uint8)
+    rc = capi().sdb_hd3_bits(n_bits, seed, out.ctypes.data)
+    if rc != 0:
+        raise SdbError(rc, "sdb_hd3_bits failed")
+    return out
+
+
+def rff_matrix(metric, half, d, sigma, seed):
+    """The (half x d column-major) random matrix of the L1 / L2 embeddings."""
+    import numpy as np
+    out = np.zeros(half * d, np.float32)
+    rc = capi().sdb_rff_matrix(metric, half, d, sigma, seed, out.ctypes.data)
+    if rc != 0:
+        raise SdbError(rc, "sdb_rff_matrix failed")
+    return out
 
 
 class SdbError(RuntimeError):
@@ -321,15 +344,18 @@ class Context:
 
 def batch_run(datasets, eps, minPts, optics=True, n_streams=4, autotune=True, device=0, n_proj=1024, topK=10,
               topM=50, metric=COSINE, seed=42, precision=FP32, hd3=None):
-    """Run sdb_batch_run over a list of (n x d) float32 arrays, every dataset
-    swept over the same eps and minPts values.  ``hd3(fhtDim, seed)`` returns
-    the 3 * fhtDim sign bytes (the reference's bitHD3Generator); it is
-    required.  Returns (results, wall_ms) with one dict per dataset:
+    """Run sdb_batch_run over a list of (n x d) float32 arrays (Cosine,
+    Chi2 or JS), every dataset swept over the same eps and minPts values.
+    ``hd3(n_bits, seed)`` gives the HD3 sign bytes (default: hd3_bits, the
+    reference's bitHD3Generator).  Returns (results, wall_ms) with one dict
+    per dataset:
     labels [n_eps, n_minpts, n], n_clusters [n_eps, n_minpts], ordering and
     reach [n_minpts, n] (optics), tile_rows and ms."""
     import numpy as np
     if hd3 is None:
-        raise ValueError("hd3(fhtDim, seed) -> sign bytes is required")
+        hd3 = hd3_bits
+    if metric in (L1, L2):
+        raise ValueError("batch_run builds Cosine / Chi2 / JS jobs; L1 / L2 jobs need an RFF matrix per job")
     lib = capi()
     eps_a = np.ascontiguousarray(eps, dtype=np.float32)
     mp_a = np.ascontiguousarray(minPts, dtype=np.int32)

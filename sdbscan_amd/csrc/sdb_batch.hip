@@ -19,6 +19,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <random>
 #include <thread>
 #include <tuple>
 #include <vector>
@@ -167,5 +168,29 @@ extern "C" int sdb_batch_run(int device, sdb_batch_job* jobs, int n_jobs, int n_
     if (wall_ms) *wall_ms = now_ms() - t0;
     for (int i = 0; i < n_jobs; ++i)
         if (jobs[i].status != SDB_OK) return jobs[i].status;
+    return SDB_OK;
+}
+
+extern "C" int sdb_hd3_bits(int n_bits, unsigned int seed, uint8_t* out) {
+    if (n_bits < 0 || (n_bits > 0 && !out)) return SDB_EINVAL;
+    std::default_random_engine gen(seed);
+    std::uniform_int_distribution<uint32_t> u(0, 1);
+    for (int i = 0; i < n_bits; ++i) out[i] = (uint8_t)(u(gen) & 1u);
+    return SDB_OK;
+}
+
+extern "C" int sdb_rff_matrix(int metric, int half, int d, float sigma, unsigned int seed, float* out) {
+    if (half < 0 || d < 0 || (half > 0 && d > 0 && !out) || !(sigma > 0.0f)) return SDB_EINVAL;
+    if (metric != SDB_L1 && metric != SDB_L2) return SDB_EINVAL;
+    std::default_random_engine gen(seed);
+    if (metric == SDB_L1) {
+        std::cauchy_distribution<float> cd(0, 1.0 / sigma);
+        for (int c = 0; c < d; ++c)
+            for (int r = 0; r < half; ++r) out[(size_t)c * half + r] = cd(gen);
+    } else {
+        std::normal_distribution<float> nd(0, 1.0 / sigma);
+        for (int c = 0; c < d; ++c)
+            for (int r = 0; r < half; ++r) out[(size_t)c * half + r] = nd(gen);
+    }
     return SDB_OK;
 }

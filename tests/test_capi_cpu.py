@@ -48,3 +48,18 @@ def test_dropin_module_surface():
     obj = cls(100, 8)
     obj.set_params(n_proj=64, topK=3, topM=10, random_seed=1)
     assert obj.labels_ == [] and obj.n_clusters_ == 0
+
+
+def test_host_rng_matches_oracle_kats():
+    """sdb_hd3_bits / sdb_rff_matrix (host std:: calls inside the library)
+    give the oracle's bitHD3Generator / gaussGenerator / cauchyGenerator
+    draws for the same seeds."""
+    import numpy as np
+    import sdbscan_amd
+    from oracle import oracle as O
+    for seed in (0, 42, 12345):
+        assert np.array_equal(sdbscan_amd.hd3_bits(3 * 1024, seed), O.hd3_bits(3 * 1024, seed))
+    r = sdbscan_amd.rff_matrix(sdbscan_amd.L2, 512, 51, 2.0, 7)
+    assert np.array_equal(r, np.asarray(O.gauss(512, 51, 0.0, 0.5, 7)).ravel())
+    r = sdbscan_amd.rff_matrix(sdbscan_amd.L1, 64, 13, 1.0, 3)
+    assert np.array_equal(r, np.asarray(O.cauchy(64, 13, 0.0, 1.0, 3)).ravel())
