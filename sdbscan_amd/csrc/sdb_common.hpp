@@ -54,9 +54,15 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
     return v;
 }
 
-// Relaxed device-scope atomic load (used for union-find parents).
+// Atomic load served from this XCD's L2 (workgroup scope: no cross-XCD
+// revalidation), used for union-find parents, the adjacency hints and the
+// border minima.  A value read here may be stale, never newer than memory;
+// those words only move one way (a parent only ever points to a smaller
+// member of the same set, a minimum only falls), so a stale value costs an
+// extra hop, a failed compare-and-swap that returns the fresh value, or a
+// redundant atomicMin -- never a wrong merge.
 __device__ __forceinline__ int load_relaxed(const int* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ uint32_t f32_to_bf16_rne(float f) {
