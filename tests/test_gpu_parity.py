@@ -88,6 +88,18 @@ def test_index_matches_oracle(ctx):
     assert np.mean(np.all(tk == tko, axis=1)) > 0.99
 
 
+@pytest.mark.parametrize("n_proj,d,K,M", [(1000, 100, 7, 40), (200, 33, 3, 25), (1024, 2000, 6, 30)])
+def test_index_ragged_projection_counts(ctx, n_proj, d, K, M):
+    """n_proj not a multiple of 64 (the last register of a lane half valid),
+    n_proj far below fhtDim, and rows longer than fhtDim's 1024 (d = 2000,
+    fhtDim 2048): index, cores and labels equal the oracle's."""
+    X, _ = datagen.mixture(3000, d, k=5, sigma=0.2, seed=200 + d)
+    print(f"ragged index n=3000 d={d} n_proj={n_proj} K={K} M={M}", flush=True)
+    p, bits, R, tk, tm, Xt, st = full_check(ctx, X, S.COSINE, 0.3, 10, K=K, M=M, n_proj=n_proj)
+    proj, tko, tmo = oracle_index(Xt, S.COSINE, p, bits, R)
+    assert index_equal(tk, tko, tm, tmo, K, M)
+
+
 def test_core_and_labels_match_oracle(ctx):
     X, _ = datagen.mixture(5000, 784, seed=13)
     print("core+labels cosine n=5000 eps=0.25 minPts=50", flush=True)
