@@ -1,12 +1,20 @@
 """Benchmark of sDbscan's neighbourhood-discovery path on one MI355X.
 
-Workload: 8,000,000 x 784 cosine points from the 10-component sparse
+Default workload: 8,000,000 x 784 cosine points from the 10-component sparse
 nonnegative mixture (generated on the device), n_proj = 1024 (fhtDim 1024),
 topK = 10, topM = 100, eps = 0.25, minPts = 100.  One step = the whole path on
 device-resident input: transform -> HD3 projections + top-K -> top-M ->
 candidate distances (BF16 MFMA tiles + FP32/FP64 boundary recheck) -> degrees
 and core test -> cluster labels.  The FP32 distance path runs the same steps
 afterwards; its labels must be identical (ARI = 1.0).
+
+--workload chi2: 8,000,000 x 784 Dirichlet histograms (10 components, 20
+boosted bins each; generated on the device), Chi2 random features to 1024
+dims, eps = 0.2, minPts = 100.  The BF16 MFMA tiles prune with the lower
+bound 1 - sum sqrt(x y) <= chi2 and the recheck decides the survivors
+(FP32, then FP64); compared with the FP32 plain path.
+
+--batch: the many-dataset sweep (see batch_bench).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -21,8 +29,21 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-N, D_FEAT, P, K, M, EPS, MINPTS = 8_000_000, 784, 1024, 10, 100, 0.25, 100
-SEED_DATA, SEED_RNG = 1002, 42
+N, D_FEAT, P, K, M = 8_000_000, 784, 1024, 10, 100
+SEED_RNG = 42
+# per workload: metric, device generator kind, data seed, eps, minPts,
+# kernel features, JSON metric name, data description
+WORKLOADS = {
+    "cosine": dict(metric=0, gen=0, seed=1002, eps=0.25, minPts=100, ker=D_FEAT,
+                   name="points/sec, 8M x 784 cosine: BF16+FP64-recheck vs FP32 plain, ARI=1.0",
+                   data="synthetic: 10-component sparse nonnegative mixture generated on the device (seed 1002)",
+                   tiles="BF16 distance tiles + FP32/FP64 recheck"),
+    "chi2": dict(metric=3, gen=1, seed=1004, eps=0.2, minPts=100, ker=1024,
+                 name="points/sec, 8M x 784 Chi2: BF16 lower-bound tiles + FP32/FP64 recheck vs FP32 plain",
+                 data="synthetic: Dirichlet histograms, 10 components with 20 bins at alpha 2 and the rest "
+                      "at alpha 0.05, generated on the device (seed 1004)",
+                 tiles="BF16 tiles on sqrt rows (chi2 >= 1 - sum sqrt(xy)) + FP32/FP64 recheck of survivors"),
+}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -36,11 +57,13 @@ def dist_bytes(n, d, s):
     return (n * 2 * K + 2 * P * M) * d * s + n * 2 * K * M / 8
 
 
-def measured_traffic(n):
+def measured_traffic(n, workload):
     """HBM bytes per launch of the BF16 distance kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None when the
     summary is missing or was measured at another size."""
-    path = os.path.join(ROOT, "profiles", "k_dist_pipe_bf16_8Mx784_hbm_counters.txt")
+    name = "k_dist_pipe_bf16_8Mx784_hbm_counters.txt" if workload == "cosine" else \
+        f"k_dist_pipe_bf16_8Mx784_{workload}_hbm_counters.txt"
+    path = os.path.join(ROOT, "profiles", name)
     if n != N or not os.path.exists(path):
         return None
     for line in open(path):
@@ -49,13 +72,13 @@ def measured_traffic(n):
     return None
 
 
-def run_path(ctx, bits, steps, warmup, tag):
+def run_path(ctx, bits, steps, warmup, tag, wl):
     times, dist_ms, phase = [], [], []
     lab = None
     for i in range(warmup + steps):
         t0 = time.perf_counter()
         ctx.build_index(bits)
-        ctx.find_core(EPS, MINPTS, want=False)
+        ctx.find_core(wl["eps"], wl["minPts"], want=False)
         lab, nc = ctx.form_clusters(want=(i == warmup + steps - 1))
         t1 = time.perf_counter()
         st = ctx.stats()
@@ -70,15 +93,16 @@ def run_path(ctx, bits, steps, warmup, tag):
     return times, dist_ms, phase, lab, nc
 
 
-def cpu_baseline(ctx, bits):
+def cpu_baseline(ctx, bits, wl, n_default):
     """The oracle (C++/OpenMP restatement with the reference's data structures)
     on the first rows of the same data, timed on this host."""
     from oracle import oracle as O
-    n_s = int(os.environ.get("SDB_CPU_SAMPLE", "20000"))
+    n_s = int(os.environ.get("SDB_CPU_SAMPLE", str(n_default)))
     X = ctx.copy_input(0, n_s)
     thr = O.max_threads()
     t0 = time.perf_counter()
-    lab, nc, core, t = O.fit_sdbscan(X, 0, P, K, M, D_FEAT, 0.4, None, bits, 1024, EPS, MINPTS, thr)
+    lab, nc, core, t = O.fit_sdbscan(X, wl["metric"], P, K, M, wl["ker"], 0.4, None, bits, 1024, wl["eps"],
+                                     wl["minPts"], thr)
     dt = time.perf_counter() - t0
     log(f"[cpu] n={n_s} threads={thr} {dt:.2f} s phases(ms)={t.tolist()} clusters={nc}")
     return {"value": n_s / dt, "unit": "points/sec", "cores": thr, "kind": "port",
@@ -134,6 +158,8 @@ def main():
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="cosine",
+                    help="cosine: the 8M x 784 cosine step; chi2: the 8M x 784 Chi2 step")
     ap.add_argument("--batch", action="store_true",
                     help="run the many-dataset sweep (sdb_batch_run) instead of the 8M x 784 step")
     ap.add_argument("--batch-datasets", type=int, default=64)
@@ -160,15 +186,17 @@ def main():
 
     import sdbscan_amd as S
 
+    wl = WORKLOADS[args.workload]
     n = args.n
     ctx = S.Context(0)
-    p = ctx.set_params(n, D_FEAT, n_proj=P, topK=K, topM=M, metric=S.COSINE, precision=S.BF16)
+    prm = dict(n_proj=P, topK=K, topM=M, metric=wl["metric"], ker_n_features=wl["ker"])
+    p = ctx.set_params(n, D_FEAT, precision=S.BF16, **prm)
     bits = S.hd3_bits(3 * p.fhtDim, SEED_RNG)
     t0 = time.perf_counter()
-    ctx.generate(n, D_FEAT, SEED_DATA, 0.1)
-    log(f"generated {n} x {D_FEAT} on device in {time.perf_counter() - t0:.1f} s")
+    ctx.generate(n, D_FEAT, wl["seed"], 0.1, kind=wl["gen"])
+    log(f"generated {n} x {D_FEAT} ({args.workload}) on device in {time.perf_counter() - t0:.1f} s")
 
-    times, dist_ms, phase, lab16, nc16 = run_path(ctx, bits, args.steps, args.warmup, "bf16")
+    times, dist_ms, phase, lab16, nc16 = run_path(ctx, bits, args.steps, args.warmup, "bf16", wl)
     total = float(sum(times))
     value = n * len(times) / total
     dist_avg = float(np.mean(dist_ms))
@@ -176,23 +204,26 @@ def main():
     achieved = B / (dist_avg * 1e-3) / 1e9
     extra = {}
     if not args.no_fp32:
-        ctx.set_params(n, D_FEAT, n_proj=P, topK=K, topM=M, metric=S.COSINE, precision=S.FP32)
-        t32, d32, ph32, lab32, nc32 = run_path(ctx, bits, max(1, args.steps), 1, "fp32")
+        ctx.set_params(n, D_FEAT, precision=S.FP32, **prm)
+        # the FP32 Chi2 tiles take ~20 s per 8M step: one untimed-warmup-free step
+        s32, w32 = (max(1, args.steps), 1) if wl["metric"] == S.COSINE else (1, 0)
+        t32, d32, ph32, lab32, nc32 = run_path(ctx, bits, s32, w32, "fp32", wl)
         v32 = n * len(t32) / float(sum(t32))
         same = bool(np.array_equal(lab16, lab32)) and nc16 == nc32
         extra = {"fp32_points_per_sec": v32, "speedup_vs_fp32": value / v32,
                  "labels_identical_to_fp32": same, "ari_vs_fp32": 1.0 if same else None,
-                 "fp32_dist_ms": float(np.mean(d32)),
-                 "fp32_dist_tflops": 2.0 * n * 2 * K * M * D_FEAT / (float(np.mean(d32)) * 1e-3) / 1e12}
+                 "fp32_dist_ms": float(np.mean(d32))}
+        if wl["metric"] == S.COSINE:
+            extra["fp32_dist_tflops"] = 2.0 * n * 2 * K * M * D_FEAT / (float(np.mean(d32)) * 1e-3) / 1e12
         if not same:
             from sklearn.metrics import adjusted_rand_score
             extra["ari_vs_fp32"] = float(adjusted_rand_score(lab32, lab16))
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(ctx, bits)
+        cpu = cpu_baseline(ctx, bits, wl, 20000 if wl["metric"] == S.COSINE else 60000)
     last = phase[-1]
     out = {
-        "metric": "points/sec, 8M x 784 cosine: BF16+FP64-recheck vs FP32 plain, ARI=1.0",
+        "metric": wl["name"],
         "value": value,
         "unit": "points/sec",
         "n_gpus": 1,
@@ -203,13 +234,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16",
-        "data": "synthetic: 10-component sparse nonnegative mixture generated on the device (seed 1002)",
-        "config": {"workload": f"sDbscan cosine n={n} d={D_FEAT} n_proj={P} topK={K} topM={M} "
-                               f"eps={EPS} minPts={MINPTS}, BF16 distance tiles + FP32/FP64 recheck",
+        "data": wl["data"],
+        "config": {"workload": f"sDbscan {args.workload} n={n} d={D_FEAT} n_proj={P} topK={K} topM={M} "
+                               f"ker_n_features={wl['ker']} eps={wl['eps']} minPts={wl['minPts']}, {wl['tiles']}",
                    "n_points": n, "n_features": D_FEAT},
         "roofline": {"bound": "hbm", "kernel": "k_dist_pipe<bf16>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(n), "algorithmic_bytes": B, "avg_ms": dist_avg},
+                     "traffic": measured_traffic(n, args.workload), "algorithmic_bytes": B, "avg_ms": dist_avg},
         "cpu_baseline": cpu,
         "phases_ms": {k: last[k] for k in ("t_transform_ms", "t_index_ms", "t_core_ms",
                                            "t_dist_ms", "t_cluster_ms")},

@@ -149,8 +149,10 @@ int sdb_get_index(sdb_ctx* ctx, int* topK_out, int* topM_out);
 int sdb_get_transformed(sdb_ctx* ctx, float* X_out);
 
 /* Their BF16 copy (n x n_features, round to nearest even) read by the BF16
- * distance tiles; SDB_ESTATE unless the precision is SDB_BF16.  No
- * counterpart in the reference (it has no BF16 path). */
+ * distance tiles; for Chi2 the BF16 of their square roots (the tiles bound
+ * chi2 from below by 1 - sum sqrt(x_i y_i) and queue the rest for the exact
+ * recheck).  SDB_ESTATE unless the precision is SDB_BF16.  No counterpart in
+ * the reference (it has no BF16 path). */
 int sdb_get_transformed_bf16(sdb_ctx* ctx, uint16_t* X_out);
 
 /* Deduplicated neighbourhoods after sdb_find_core as CSR sorted by index:
@@ -170,7 +172,9 @@ int sdb_get_stats(const sdb_ctx* ctx, sdb_stats* out);
 
 /* Synthetic workloads generated on the device into the context's input
  * buffer (no host copy): kind 0 = mixture of 10 sparse nonnegative centres
- * plus N(0, sigma^2) noise, clipped to [0,1] and L2-normalised. */
+ * plus N(0, sigma^2) noise, clipped to [0,1] and L2-normalised; kind 1 =
+ * Dirichlet histograms over n_features bins from 10 components, each with 20
+ * bins at alpha 2 and the rest at alpha 0.05 (sigma unused). */
 int sdb_generate(sdb_ctx* ctx, int kind, long long n, int d, unsigned long long seed, float sigma);
 
 /* Copy rows [row0, row0 + rows) of the current input to the host. */

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <functional>
 #include <queue>
+#include <random>
 #include <string>
 #include <thread>
 #include <vector>
@@ -23,6 +24,8 @@ int launch_recheck_k(const DistArgs& a, const int* topK, int K, int P, int preci
                      double band32, int n_entries, unsigned long long* stats, hipStream_t st);
 int launch_gen_mixture(float* X, long long n, int d, unsigned long long seed, float sigma,
                        hipStream_t st);
+int launch_gen_dirichlet(float* X, long long n, int d, unsigned long long seed, const uint16_t* bins,
+                         hipStream_t st);
 int launch_csr_fill(const ClusterArgs& c, const long long* ptr, int* cursor, int* idx,
                     hipStream_t st);
 int launch_csr_dist(const float* X, int ldx, int d, int metric, long long n, const long long* ptr,
@@ -272,8 +275,8 @@ int sdb_set_params(sdb_ctx* c, const sdb_params* p) {
     if (q.metric == SDB_COSINE && q.ker_n_features != q.n_features)
         return fail(c, SDB_EINVAL, "Cosine requires ker_n_features == n_features");
     if (q.precision != SDB_FP32 && q.precision != SDB_BF16) return fail(c, SDB_EINVAL, "unknown precision");
-    if (q.precision == SDB_BF16 && q.metric != SDB_COSINE)
-        return fail(c, SDB_EUNSUPPORTED, "BF16 distance tiles are implemented for Cosine only");
+    if (q.precision == SDB_BF16 && q.metric != SDB_COSINE && q.metric != SDB_CHI2)
+        return fail(c, SDB_EUNSUPPORTED, "BF16 distance tiles are implemented for Cosine and Chi2 only");
     if ((long long)q.n_points * 2 * q.topK >= (1ll << 32)) return fail(c, SDB_EUNSUPPORTED, "n_points * 2 * topK must stay below 2^32");
     if (2 * q.n_proj >= (1 << 24)) return fail(c, SDB_EUNSUPPORTED, "n_proj too large");
     c->prm = q;
@@ -319,12 +322,30 @@ int sdb_load_device(sdb_ctx* c, const float* X, long long n, int d) {
 
 int sdb_generate(sdb_ctx* c, int kind, long long n, int d, unsigned long long seed, float sigma) {
     if (!c) return SDB_EINVAL;
-    if (kind != 0) return fail(c, SDB_EINVAL, "unknown generator kind %d", kind);
+    if (kind != 0 && kind != 1) return fail(c, SDB_EINVAL, "unknown generator kind %d", kind);
+    if (kind == 1 && (d < kGenBoost || d > 4096))
+        return fail(c, SDB_EINVAL, "Dirichlet generator needs %d <= n_features <= 4096", kGenBoost);
     HIPCHK(hipSetDevice(c->device));
     int r = set_input_dims(c, n, d);
     if (r) return r;
     ENSURE(c->Xown, (size_t)n * d * 4);
-    LCHK(launch_gen_mixture(P_<float>(c->Xown), n, d, seed, sigma, c->st));
+    if (kind == 0) {
+        LCHK(launch_gen_mixture(P_<float>(c->Xown), n, d, seed, sigma, c->st));
+    } else {
+        // boosted bins of each component: a partial Fisher-Yates shuffle
+        std::mt19937_64 g(seed);
+        std::vector<int> perm(d);
+        std::vector<uint16_t> bins(10 * kGenBoost);
+        for (int k = 0; k < 10; ++k) {
+            for (int i = 0; i < d; ++i) perm[i] = i;
+            for (int j = 0; j < kGenBoost; ++j) {
+                const int t = j + (int)(g() % (uint64_t)(d - j));
+                std::swap(perm[j], perm[t]);
+                bins[k * kGenBoost + j] = (uint16_t)perm[j];
+            }
+        }
+        LCHK(launch_gen_dirichlet(P_<float>(c->Xown), n, d, seed, bins.data(), c->st));
+    }
     HIPCHK(hipStreamSynchronize(c->st));
     c->X = P_<float>(c->Xown);
     return SDB_OK;
@@ -356,14 +377,15 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     hipStream_t st = c->st;
 
     HIPCHK(hipEventRecord(c->ev[0], st));
-    const bool cos = q.metric == SDB_COSINE;
-    // ---- transform (src/Utilities.h:289-335); fused into the index passes for Cosine
+    // ---- transform (src/Utilities.h:289-335); fused into the index passes for
+    // Cosine and for the Chi2 / JS features without sampled components
+    const bool fused = index_embedding(q.metric, d, q.ker_n_features) != 2;
     ENSURE(c->Xn, (size_t)n * ldx * 4);
     if (q.precision == SDB_BF16) ENSURE(c->Xb, (size_t)n * ldx * 2);
     ENSURE(c->err_flag, 4);
     HIPCHK(hipMemsetAsync(c->err_flag.p, 0, 4, st));
     int errf = 0;
-    if (!cos) {
+    if (!fused) {
         LCHK(launch_transform(c->X, n, d, ldx, q.metric, P_<float>(c->Xn),
                               q.precision == SDB_BF16 ? P_<uint16_t>(c->Xb) : nullptr, P_<int>(c->err_flag), st));
         HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
@@ -449,6 +471,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     std::vector<int> hflags(2 * P);
     HIPCHK(hipMemcpyAsync(hflags.data(), c->flags.p, (size_t)2 * P * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
     if (errf & 2) return fail(c, SDB_EZERO, "Error: There is an zero point !");
 
     // ---- host replay of tied top-K points (reference heap semantics)
@@ -649,7 +672,13 @@ static void cosine_cuts(double lo, double hi, float* s_hit, float* s_near) {
     *s_near = ft;
 }
 
-static void compute_bands(const sdb_params& q, float eps, double* band, double* band32) {
+// Chi2 on BF16 rows (prec BF16): the tiles compute BC~, the dot of the BF16
+// square roots of the sum-normalised rows (their Bhattacharyya coefficient).
+// Term by term 2xy/(x+y) <= sqrt(xy), so chi2(x, y) = 1 - 2 sum xy/(x+y) >=
+// 1 - BC: a pair with 1 - BC~ > eps + band has chi2 > eps.  The tiles declare
+// no hits; every other pair goes to the recheck, which decides with the FP32
+// chi2 outside band32 and with the FP64 chi2 inside it.
+static void compute_bands(const sdb_params& q, int prec, float eps, double* band, double* band32) {
     const double u = std::ldexp(1.0, -23);
     const double d = q.n_features;
     const double Smax = 1.0 + 1e-5;  // sum |x_i y_i| <= ||x|| ||y|| for unit rows
@@ -657,7 +686,7 @@ static void compute_bands(const sdb_params& q, float eps, double* band, double* 
     *band32 = b32;
     switch (q.metric) {
         case SDB_COSINE:
-            if (q.precision == SDB_BF16) {
+            if (prec == SDB_BF16) {
                 const double ub = std::ldexp(1.0, -8);
                 *band = 2.0 * ((2.0 * ub + ub * ub) * Smax + (d + 2.0) * u * Smax * (1.0 + 2.0 * ub) + 2.0 * u);
             } else {
@@ -666,7 +695,16 @@ static void compute_bands(const sdb_params& q, float eps, double* band, double* 
             break;
         case SDB_L2: *band = 2.0 * (double)eps * (d + 8.0) * u + 1e-30; break;
         case SDB_L1: *band = 2.0 * (double)eps * (d + 4.0) * u + 1e-30; break;
-        case SDB_CHI2: *band = 2.0 * (d + 8.0) * u; break;
+        case SDB_CHI2:
+            if (prec == SDB_BF16) {
+                // per factor: sqrtf (<= 1 ulp) then BF16 rounding, relative e
+                const double e = std::ldexp(1.0, -8) + 3.0 * u;
+                *band = 2.0 * ((2.0 * e + e * e) * Smax + (d + 2.0) * u * Smax * (1.0 + 2.0 * e) + 2.0 * u) + 1e-12;
+                *band32 = 2.0 * (d + 8.0) * u;
+            } else {
+                *band = 2.0 * (d + 8.0) * u;
+            }
+            break;
         default: *band = 2.0 * (2.0 * d + 200.0) * u; break;
     }
 }
@@ -679,8 +717,12 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
     const long long n = c->n;
     const int M = c->Mt, W = c->W, S = c->S, L2 = c->L2;
     hipStream_t st = c->st;
-    const bool mfma = q.metric == SDB_COSINE && M <= 128;
+    // Cosine on MFMA tiles at either precision; Chi2 with BF16 on the BF16
+    // tiles as a lower-bound prefilter (compute_bands); everything else on the
+    // VALU tiles in FP32
+    const bool mfma = (q.metric == SDB_COSINE || (q.metric == SDB_CHI2 && q.precision == SDB_BF16)) && M <= 128;
     const int prec = mfma ? q.precision : SDB_FP32;
+    const bool prefilter = q.metric == SDB_CHI2 && prec == SDB_BF16;
 
     // tiles: contiguous ranges of one list's group
     int QB;
@@ -730,7 +772,7 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
     ENSURE(c->queue, (size_t)c->qcap * 8);
 
     double band = 0, band32 = 0;
-    compute_bands(q, eps, &band, &band32);
+    compute_bands(q, prec, eps, &band, &band32);
     da.X = prec == SDB_BF16 ? (const void*)c->Xb.p : (const void*)c->Xn.p;
     da.X32 = P_<float>(c->Xn);
     da.grp = P_<uint32_t>(c->grp);
@@ -740,8 +782,9 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
     da.E = P_<uint32_t>(c->E);
     da.colsum = P_<int>(c->colsum);
     da.eps = eps;
-    da.band = band;
-    cosine_cuts((double)eps - band, (double)eps + band, &da.s_hit, &da.s_near);
+    da.lo = prefilter ? -INFINITY : (double)eps - band;
+    da.hi = (double)eps + band;
+    cosine_cuts(da.lo, da.hi, &da.s_hit, &da.s_near);
     da.magicS = ((1ull << 40) + (unsigned long long)S - 1) / (unsigned long long)S;
 
     HIPCHK(hipEventRecord(c->ev[3], st));

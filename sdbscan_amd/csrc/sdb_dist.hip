@@ -45,7 +45,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     const int beg = a.tile_beg[blockIdx.x], end = a.tile_end[blockIdx.x];
     const int* Lm = a.lists + (size_t)l * M;
     const char* Xc = (const char*)a.X;
-    const double lo = (double)a.eps - a.band, hi = (double)a.eps + a.band;
+    const double lo = a.lo, hi = a.hi;
 
     int* pts_lds = col_lds + M;
     if (tid < M) {  // M <= 256 < WAVES * 64
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu(DistArgs a) {
     const int beg = a.tile_beg[blockIdx.x], end = a.tile_end[blockIdx.x];
     const int* Lm = a.lists + (size_t)l * M;
     const int metric = a.metric;
-    const double lo = (double)a.eps - a.band, hi = (double)a.eps + a.band;
+    const double lo = a.lo, hi = a.hi;
     float* qb = qbuf + (size_t)w * QV * a.kc;
 
     for (int i = tid; i < M; i += WAVES * 64) col_lds[i] = 0;
@@ -644,8 +644,9 @@ int launch_dist_valu(const DistArgs& a, int ntiles, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Boundary recheck, one wavefront per queued pair.  Sums run per lane over
 // k = lane, lane + 64, ... and combine through an xor butterfly (offsets
-// 32..1).  The BF16 path first re-decides with the FP32 value when it lies
-// outside the FP32 band; everything else is decided with the FP64 value.
+// 32..1).  The BF16 paths (Cosine, and the Chi2 prefilter) first re-decide
+// with the FP32 distance when it lies outside the FP32 band; everything else is
+// decided with the FP64 value.
 // stats[0]: FP32 rechecks, stats[1]: FP64 rechecks, stats[2]: hits added.
 __device__ __forceinline__ double dist64_wave(const float* x, const float* y, int d, int metric, int lane) {
     double s = 0.0;
@@ -676,6 +677,18 @@ __device__ __forceinline__ float dot32_wave(const float* x, const float* y, int 
     return s;
 }
 
+// sum of x y / (x + y) over the entries positive in both rows, FP32
+__device__ __forceinline__ float chi2sum32_wave(const float* x, const float* y, int d, int lane) {
+    float s = 0.0f;
+    for (int i = lane; i < d; i += 64) {
+        const float xv = x[i], yv = y[i];
+        if (xv > 0.0f && yv > 0.0f) s = s + xv * yv / (xv + yv);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s = s + __shfl_xor(s, m, kWave);
+    return s;
+}
+
 __global__ __launch_bounds__(256) void k_recheck(DistArgs a, const int* topK, int K, int P,
                                                  int precision, double band32, int n_entries,
                                                  unsigned long long* stats) {
@@ -693,8 +706,9 @@ __global__ __launch_bounds__(256) void k_recheck(DistArgs a, const int* topK, in
         const float* y = a.X32 + (size_t)c * a.ldx;
         bool hit;
         bool decided = false;
-        if (precision == kBF16 && a.metric == kCosine) {
-            const float d32 = 1.0f - dot32_wave(x, y, a.d, lane);
+        if (precision == kBF16 && (a.metric == kCosine || a.metric == kChi2)) {
+            const float d32 = a.metric == kCosine ? 1.0f - dot32_wave(x, y, a.d, lane)
+                                                  : 1.0f - 2.0f * chi2sum32_wave(x, y, a.d, lane);
             const double dv = (double)d32;
             if (lane == 0) atomicAdd(&stats[0], 1ull);
             if (dv < (double)a.eps - band32) { hit = true; decided = true; }

@@ -15,7 +15,8 @@ namespace sdb {
 // x / ||x|| with ||x||^2 accumulated in FP64 per lane (stride 64) and combined
 // by an xor butterfly; Chi2/JS: nonnegativity check and x / sum(x).  Writes
 // the FP32 rows (row stride ldx, zero padded) and, when requested, the BF16
-// copy used by the BF16 distance tiles.
+// rows the BF16 distance tiles read (Chi2: BF16 of the square roots, whose
+// dot product bounds chi2 from below).
 // err bit 1: negative entry (Chi2/JS); err bit 2: zero point.
 __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, long long n, int d,
                                                    int ldx, int metric, float* __restrict__ Xn,
@@ -50,7 +51,7 @@ __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, 
             float v = 0.0f;
             if (i < d) v = scaled ? x[i] / scale : x[i];
             o[i] = v;
-            if (ob) ob[i] = (uint16_t)f32_to_bf16_rne(v);
+            if (ob) ob[i] = (uint16_t)f32_to_bf16_rne(metric == kChi2 ? sqrtf(v) : v);
         }
     }
 }
@@ -537,7 +538,16 @@ __device__ __forceinline__ void load_row(const float* row, int d, int lg, float 
         out[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, 4 * lg + 256 * j, 0, 0));
 }
 
-template <int LOGD, int MODE, bool COS>
+// Embedding kinds of k_index.  kEmbCos: the transformed row itself;
+// kEmbSqrt: Chi2 / JS with no sampled components (ker_n_features < 3 d, so
+// (ker / d - 1) / 2 = 0 in src/Utilities.h:134-135 and :176-177): element e
+// is sqrt(x_e L) (JS: sqrt(2 x_e L / ln 4)) for e < min(d, ker) and x_e > 0,
+// else 0 -- elementwise, in registers; kEmbLds: the general embeddings through
+// the wave's LDS slice (embed_lds).  kEmbCos and kEmbSqrt fuse transformData
+// into the passes that read untransformed rows.
+enum EmbKind { kEmbCos = 0, kEmbSqrt = 1, kEmbLds = 2 };
+
+template <int LOGD, int MODE, int EMB>
 // Occupancy: 3 workgroups per CU up to D = 1024 (at most 168 registers),
 // fewer for the longer rows so that their registers do not spill.
 __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_index(IndexArgs a) {
@@ -552,7 +562,7 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
     constexpr int NW = (VPL + 31) / 32;
 
     extern __shared__ __align__(16) unsigned char smem[];
-    // LDS: [top-K scratch 4 x 2 KB (kFull)] [2P keys] [embedding 4 x D x 4 B (!COS)]
+    // LDS: [top-K scratch 4 x 2 KB (kFull)] [2P keys] [embedding 4 x D x 4 B (kEmbLds)]
     const size_t off_red = MODE == kFull ? (size_t)4 * 2048 : 0;
     // threshold area: kSample row max / min accumulators [P] + [P]; kFull
     // per-row (hi, lo) threshold pairs for all D elements (past P: never hit)
@@ -584,7 +594,7 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
     }
     const long long stride = MODE == kSample ? a.sample_stride : 1;
     const long long npts = MODE == kPoints ? a.n_ids : (a.n + stride - 1) / stride;
-    constexpr bool RAW = COS && (MODE == kSample || MODE == kFull);  // reads untransformed rows
+    constexpr bool RAW = EMB != kEmbLds && (MODE == kSample || MODE == kFull);  // reads untransformed rows
     for (long long q = gw; q < npts; q += nw) {
         const long long p = MODE == kPoints ? (long long)a.pt_ids[q] : q * stride;
         float v[VPL];
@@ -594,21 +604,29 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int w = 0; w < NW; ++w) asm volatile("" : "+v"(sg[r][w]));
-        if constexpr (COS) {
+        if constexpr (EMB != kEmbLds) {
             if constexpr (!RAW) {
                 load_row<VPL>(a.Xn + p * a.ldx, a.d, lg, v);
             } else {
-                // transformData for Cosine (src/Utilities.h:295-299): x / ||x||,
+                // transformData (src/Utilities.h:295-310): Cosine x / ||x||, with
                 // ||x||^2 summed in FP64 per lane (elements e = lg + 64 j) then by
-                // the xor butterfly over lg -- the order of k_transform.
+                // the xor butterfly over lg -- the order of k_transform; Chi2 / JS
+                // x / sum(x) with the sum in the same order, and the
+                // nonnegativity check.
                 load_row<VPL>(a.X + p * a.d, a.d, lg, v);
                 double acc = 0.0;
+                bool neg = false;
 #pragma unroll
                 for (int j = 0; j < VPL; ++j) {
                     const double t = v[j];
-                    acc = acc + t * t;
+                    if constexpr (EMB == kEmbCos) acc = acc + t * t;
+                    else { acc = acc + t; neg |= v[j] < 0.0f; }
                 }
-                const float nrm = (float)sqrt(wave_sum_f64(acc, lg));
+                const double tot = wave_sum_f64(acc, lg);
+                const float nrm = EMB == kEmbCos ? (float)sqrt(tot) : (float)tot;
+                if constexpr (EMB != kEmbCos) {
+                    if (__any(neg) && lane == 0) atomicOr(a.err, 1);
+                }
                 if (!(nrm > 0.0f)) {
                     if (lane == 0) atomicOr(a.err, 2);
                 }
@@ -628,18 +646,34 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
                     for (int j = 0; j < VPL; ++j)
                         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v[j]), ro, 4 * lg + 256 * j, 0, 0);
                     if (a.Xb_out) {
-                        // BF16 copy, round to nearest even (hardware pair conversion)
+                        // BF16 rows, round to nearest even (hardware pair
+                        // conversion); Chi2: of the square roots (the rows whose
+                        // dot product bounds chi2 from below)
                         const __amdgpu_buffer_rsrc_t rb = row_rsrc(a.Xb_out + p * a.ldx, a.ldx * 2);
+                        auto bsrc = [&](int j) { return EMB == kEmbCos ? v[j] : sqrtf(v[j]); };
 #pragma unroll
                         for (int j = 0; j + 1 < VPL; j += 2) {
-                            const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){v[j], v[j + 1]}, bf16x2));
+                            const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){bsrc(j), bsrc(j + 1)}, bf16x2));
                             __builtin_amdgcn_raw_buffer_store_b16((uint16_t)u, rb, 2 * lg + 128 * j, 0, 0);
                             __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(u >> 16), rb, 2 * lg + 128 * (j + 1), 0, 0);
                         }
                         if constexpr (VPL % 2 == 1)
-                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)f32_to_bf16_rne(v[VPL - 1]), rb,
+                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)f32_to_bf16_rne(bsrc(VPL - 1)), rb,
                                                                   2 * lg + 128 * (VPL - 1), 0, 0);
                     }
+                }
+            }
+            if constexpr (EMB == kEmbSqrt) {
+                // the features of src/Utilities.h:140-145 (Chi2) / :180-185 (JS)
+                const float L = a.kerL;
+                const int lim = a.d < a.ker ? a.d : a.ker;
+#pragma unroll
+                for (int j = 0; j < VPL; ++j) {
+                    const float xv = v[j];
+                    float f = 0.0f;
+                    if (xv > 0.0f && lg + 64 * j < lim)
+                        f = a.metric == kChi2 ? sqrtf(xv * L) : (float)sqrt((double)(xv * L) * 2.0 / log(4.0));
+                    v[j] = f;
                 }
             }
         } else {
@@ -1076,26 +1110,40 @@ int launch_transform(const float* X, long long n, int d, int ldx, int metric, fl
     return (int)hipGetLastError();
 }
 
-template <int LOGD, int MODE, bool COS>
+template <int LOGD, int MODE, int EMB>
 static int launch_index_t(const IndexArgs& a, int grid, hipStream_t st) {
     constexpr int D = 1 << LOGD;
-    size_t sm = (MODE == kFull ? (size_t)4 * 2048 + (size_t)8 * D : (size_t)8 * a.P) + (COS ? 0 : (size_t)4 * D * 4);
+    size_t sm = (MODE == kFull ? (size_t)4 * 2048 + (size_t)8 * D : (size_t)8 * a.P) + (EMB == kEmbLds ? (size_t)4 * D * 4 : 0);
     if (sm > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE, COS>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE, EMB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_index<LOGD, MODE, COS>), dim3(grid), dim3(256), sm, st, a);
+    hipLaunchKernelGGL((k_index<LOGD, MODE, EMB>), dim3(grid), dim3(256), sm, st, a);
     return (int)hipGetLastError();
+}
+
+template <int LOGD, int MODE>
+static int launch_index_m(const IndexArgs& a, int grid, hipStream_t st) {
+    switch (index_embedding(a.metric, a.d, a.ker)) {
+        case kEmbCos: return launch_index_t<LOGD, MODE, kEmbCos>(a, grid, st);
+        case kEmbSqrt: return launch_index_t<LOGD, MODE, kEmbSqrt>(a, grid, st);
+        default: return launch_index_t<LOGD, MODE, kEmbLds>(a, grid, st);
+    }
 }
 
 template <int LOGD>
 static int launch_index_d(const IndexArgs& a, int mode, int grid, hipStream_t st) {
-    const bool cos = a.metric == kCosine;
-    if (mode == kSample) return cos ? launch_index_t<LOGD, kSample, true>(a, grid, st) : launch_index_t<LOGD, kSample, false>(a, grid, st);
-    if (mode == kFull) return cos ? launch_index_t<LOGD, kFull, true>(a, grid, st) : launch_index_t<LOGD, kFull, false>(a, grid, st);
-    if (mode == kRows) return cos ? launch_index_t<LOGD, kRows, true>(a, grid, st) : launch_index_t<LOGD, kRows, false>(a, grid, st);
-    return cos ? launch_index_t<LOGD, kPoints, true>(a, grid, st) : launch_index_t<LOGD, kPoints, false>(a, grid, st);
+    if (mode == kSample) return launch_index_m<LOGD, kSample>(a, grid, st);
+    if (mode == kFull) return launch_index_m<LOGD, kFull>(a, grid, st);
+    if (mode == kRows) return launch_index_m<LOGD, kRows>(a, grid, st);
+    return launch_index_m<LOGD, kPoints>(a, grid, st);
+}
+
+int index_embedding(int metric, int d, int ker) {
+    if (metric == kCosine) return kEmbCos;
+    if ((metric == kChi2 || metric == kJS) && ker < 3 * d) return kEmbSqrt;
+    return kEmbLds;
 }
 
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st) {

@@ -6,6 +6,7 @@ namespace sdb {
 
 constexpr int kMaxW = 8;          // top-M lists up to 256 points (8 x 32-bit words)
 constexpr int kLdsBytes = 163840;  // LDS per workgroup on gfx950
+constexpr int kGenBoost = 20;      // boosted bins per component of the Dirichlet generator
 
 struct IndexArgs {
     const float* X;   // n x d input rows (cosine passes read and transform them)
@@ -58,7 +59,9 @@ struct DistArgs {
     int* qcount;
     int qcap;
     float eps;
-    double band;          // |d_hat - eps| < band -> recheck
+    // fast distance d_hat: hit iff d_hat < lo, queued for the recheck iff
+    // lo <= d_hat <= hi (lo = -inf: the tiles only prefilter, never decide)
+    double lo, hi;
     int kc;               // dims per LDS chunk
     int nchunk;
     int dk;               // dims covered by the tiles (multiple of the k-step)
@@ -70,6 +73,9 @@ struct DistArgs {
 };
 
 // index
+// embedding kind of the index kernels for (metric, n_features, ker_n_features):
+// 0 and 1 fuse the transform into the index passes, 2 needs launch_transform
+int index_embedding(int metric, int d, int ker);
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
                      uint16_t* Xb, int* err, hipStream_t st);
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st);

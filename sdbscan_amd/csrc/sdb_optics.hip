@@ -156,7 +156,78 @@ __global__ __launch_bounds__(256) void k_gen_mixture(float* X, long long n, int 
     }
 }
 
+// Dirichlet histograms: 10 components, each with kGenBoost bins (drawn on the
+// host) at alpha = 2 and every other bin at alpha = 0.05; x_i = G_i / sum G
+// with G_i ~ Gamma(alpha_i).  Gamma by Marsaglia and Tsang ("A simple method
+// for generating gamma variables", ACM TOMS 26(3), 2000) with a bounded
+// rejection loop, and Gamma(a) = Gamma(a + 1) U^(1/a) for a < 1.
+struct GenBins {
+    uint16_t bin[10][kGenBoost];
+};
+
+__device__ __forceinline__ float u01_open(uint64_t h) { return ((float)(h >> 40) + 1.0f) * (1.0f / 16777216.0f); }
+
+__device__ float gamma_draw(float alpha, uint64_t h) {
+    const bool small = alpha < 1.0f;
+    const float a = small ? alpha + 1.0f : alpha;
+    const float dd = a - 1.0f / 3.0f, cc = 1.0f / sqrtf(9.0f * dd);
+    float v = 1.0f;
+    for (int t = 0; t < 16; ++t) {
+        const uint64_t h1 = smix(h + 0x632BE59BD9B4E019ull * (uint64_t)(t + 1)), h2 = smix(h1), h3 = smix(h2);
+        const float z = sqrtf(-2.0f * logf(u01_open(h1))) * cosf(6.283185307179586f * u24(h2));
+        const float x1 = 1.0f + cc * z;
+        if (x1 <= 0.0f) continue;
+        v = x1 * x1 * x1;
+        if (logf(u01_open(h3)) < 0.5f * z * z + dd - dd * v + dd * logf(v)) break;
+    }
+    float g = dd * v;
+    if (small) g *= powf(u01_open(smix(h ^ 0xD1B54A32D192ED03ull)), 1.0f / alpha);
+    return g;
+}
+
+__global__ __launch_bounds__(256) void k_gen_dirichlet(float* X, long long n, int d, uint64_t seed, GenBins tb) {
+    extern __shared__ unsigned char boosted[];  // 10 x d
+    for (int i = threadIdx.x; i < 10 * d; i += blockDim.x) boosted[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < 10 * kGenBoost; i += blockDim.x)
+        boosted[(i / kGenBoost) * d + tb.bin[i / kGenBoost][i % kGenBoost]] = 1;
+    __syncthreads();
+    const int lane = lane_id();
+    const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long p = wv; p < n; p += nwv) {
+        const int comp = (int)(smix(seed * 0x5851F42D4C957F2Dull + (uint64_t)p) >> 33) % 10;
+        float* x = X + (size_t)p * d;
+        float s = 0.0f;
+        for (int i = lane; i < d; i += 64) {
+            const float alpha = boosted[comp * d + i] ? 2.0f : 0.05f;
+            const float g = gamma_draw(alpha, smix(seed ^ smix(((uint64_t)p * (uint64_t)d + (uint64_t)i) * 2ull + 1ull)));
+            x[i] = g;
+            s = s + g;
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) s = s + __shfl_xor(s, m, kWave);
+        if (s > 0.0f)
+            for (int i = lane; i < d; i += 64) x[i] = x[i] / s;
+    }
+}
+
 static inline unsigned nb(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int launch_gen_dirichlet(float* X, long long n, int d, unsigned long long seed, const uint16_t* bins,
+                         hipStream_t st) {
+    GenBins tb;
+    for (int c = 0; c < 10; ++c)
+        for (int j = 0; j < kGenBoost; ++j) {
+            if (bins[c * kGenBoost + j] >= d) return -1;
+            tb.bin[c][j] = bins[c * kGenBoost + j];
+        }
+    long long blocks = (n + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_gen_dirichlet, dim3((unsigned)blocks), dim3(256), (size_t)10 * d, st, X, n, d,
+                       (uint64_t)seed, tb);
+    return (int)hipGetLastError();
+}
 
 int launch_gen_mixture(float* X, long long n, int d, unsigned long long seed, float sigma,
                        hipStream_t st) {

@@ -46,13 +46,21 @@ def index_equal(tk, tko, tm, tmo, K, M):
             and sets_equal(tm[:, :M], tmo[:, :M]) and sets_equal(tm[:, M:], tmo[:, M:]))
 
 
-def test_transform_bit_exact(ctx):
-    X, _ = datagen.mixture(3000, 784, seed=7)
+@pytest.mark.parametrize("metric,d,ker", [
+    (S.COSINE, 784, None),   # fused into the index passes
+    (S.CHI2, 784, 1024),     # fused (square-root features)
+    (S.CHI2, 100, 512),      # transform kernel, then the LDS embedding
+])
+def test_transform_bit_exact(ctx, metric, d, ker):
+    if metric == S.COSINE:
+        X, _ = datagen.mixture(3000, d, seed=7)
+    else:
+        X, _ = datagen.dirichlet(3000, d, seed=7)
     X = X * np.float32(3.7)
-    print("transform cosine n=3000 d=784", flush=True)
-    run_index(ctx, X)
+    print(f"transform metric {metric} n=3000 d={d} ker={ker}", flush=True)
+    run_index(ctx, X, metric=metric, ker=ker, n_proj=256)
     Xg = ctx.get_transformed()
-    Xo, rc = O.transform(X, S.COSINE)
+    Xo, rc = O.transform(X, metric)
     assert rc == 0
     assert np.array_equal(Xg.view(np.uint32), Xo.view(np.uint32))
 
@@ -216,10 +224,64 @@ def test_l1_l2_random_features(ctx, metric, eps):
 
 
 @pytest.mark.parametrize("metric,eps", [(S.CHI2, 0.35), (S.JS, 0.3)])
-def test_chi2_js(ctx, metric, eps):
+@pytest.mark.parametrize("ker", [512, 256])
+def test_chi2_js(ctx, metric, eps, ker):
+    # ker 512 >= 3 d: sampled components (LDS embedding after the transform
+    # kernel); ker 256 < 3 d: square-root features with the transform fused
     X, _ = datagen.dirichlet(5000, 100, k=5, seed=41)
-    print(f"metric {metric} n=5000 d=100 ker=100", flush=True)
-    full_check(ctx, X, metric, eps, 20, K=8, M=50, ker=512, n_proj=256)
+    print(f"metric {metric} n=5000 d=100 ker={ker}", flush=True)
+    full_check(ctx, X, metric, eps, 20, K=8, M=50, ker=ker, n_proj=256)
+
+
+@pytest.mark.parametrize("n,d,K,M,eps,minPts", [
+    (6000, 784, 10, 100, 0.47, 10),   # single-chunk BF16 tiles (the 8M x 784 Chi2 shape)
+    (5000, 100, 8, 50, 0.25, 20),     # 4 k-steps padded to 8
+    (3000, 1000, 5, 50, 0.5, 5),      # rows longer than one LDS chunk: k-chunked tiles
+])
+def test_chi2_bf16_prefilter_matches_oracle(ctx, n, d, K, M, eps, minPts):
+    """Chi2 with BF16 rows: the MFMA tiles only prune (1 - sum sqrt(x y) bounds
+    chi2 from below) and every surviving pair is decided by the FP32/FP64
+    recheck, so degrees, cores and labels equal the oracle's and the FP32
+    path's."""
+    X, _ = datagen.dirichlet(n, d, k=10 if d >= 784 else 5, seed=61 + d)
+    print(f"chi2 bf16 prefilter n={n} d={d} K={K} M={M} eps={eps}", flush=True)
+    ker = 1024 if d > 512 else 512
+    full_check(ctx, X, S.CHI2, eps, minPts, K=K, M=M, ker=ker, n_proj=256, precision=S.FP32)
+    c32, d32 = ctx.find_core(eps, minPts)
+    l32, n32 = ctx.form_clusters()
+    p, bits, R, tk, tm, Xt, st = full_check(ctx, X, S.CHI2, eps, minPts, K=K, M=M, ker=ker, n_proj=256,
+                                            precision=S.BF16)
+    print("queue", st["n_queue"], "fp32", st["n_recheck32"], "fp64", st["n_recheck64"], "band", st["band"],
+          flush=True)
+    assert st["n_queue"] > 0 and st["n_recheck32"] == st["n_queue"]
+    c16, d16 = ctx.find_core(eps, minPts)
+    l16, n16 = ctx.form_clusters()
+    assert np.array_equal(d32, d16) and np.array_equal(c32, c16)
+    assert np.array_equal(l32, l16) and n32 == n16
+    # the BF16 rows are the square roots of the sum-normalised rows, rounded
+    # to BF16 (within one BF16 ulp of numpy's correctly rounded result)
+    xb = ctx.get_transformed_bf16().astype(np.int32)
+    r = np.sqrt(Xt).view(np.uint32).astype(np.uint64)
+    rne = ((r + 0x7FFF + ((r >> 16) & 1)) >> 16).astype(np.int32)
+    assert np.abs(xb - rne).max() <= 1
+
+
+def test_device_dirichlet_generator(ctx):
+    """Generator kind 1: nonnegative rows summing to one, reproducible from the
+    seed, with the mass of each row concentrated on a few bins."""
+    n, d = 4000, 784
+    ctx.set_params(n, d, n_proj=256, topK=5, topM=20, metric=S.CHI2, ker_n_features=1024)
+    ctx.generate(n, d, 1004, 0.1, kind=1)
+    X = ctx.copy_input(0, n)
+    ctx.generate(n, d, 1004, 0.1, kind=1)
+    assert np.array_equal(X, ctx.copy_input(0, n))
+    assert (X >= 0).all() and np.allclose(X.sum(1), 1.0, atol=1e-5)
+    # 20 boosted bins (alpha 2) against 764 at alpha 0.05: expected mass 40/78.2
+    top = np.sort(X, axis=1)[:, -20:].sum(1)
+    print("mean top-20 mass", top.mean(), flush=True)
+    assert top.mean() > 0.5
+    ctx.generate(n, d, 1005, 0.1, kind=1)
+    assert not np.array_equal(X, ctx.copy_input(0, n))
 
 
 def test_sdbscan_eps_sweep_reuses_index(ctx):
@@ -511,10 +573,17 @@ def test_error_paths(ctx):
     with pytest.raises(S.SdbError) as e:
         run_index(ctx, Xn, metric=S.CHI2, K=4, M=20, n_proj=64, ker=64)
     assert e.value.code == 4
+    # the same checks where the transform is fused into the index passes
+    with pytest.raises(S.SdbError) as e:
+        run_index(ctx, Xn, metric=S.CHI2, K=4, M=20, n_proj=64, ker=32)
+    assert e.value.code == 4
     Xz = X.copy()
     Xz[5] = 0.0
     with pytest.raises(S.SdbError) as e:
         run_index(ctx, Xz, metric=S.COSINE, K=4, M=20, n_proj=64)
+    assert e.value.code == 5
+    with pytest.raises(S.SdbError) as e:
+        run_index(ctx, Xz, metric=S.JS, K=4, M=20, n_proj=64, ker=32)
     assert e.value.code == 5
     with pytest.raises(S.SdbError) as e:
         ctx.set_params(500, 20, n_proj=64, topK=100, topM=20)

@@ -123,6 +123,46 @@ def test_transform_cosine_and_chi2():
     assert O.transform(Xz, CHI2)[1] == 2
 
 
+def _bf16_rne(a):
+    r = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return (((r + 0x7FFF + ((r >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+
+
+def test_chi2_bounded_below_by_sqrt_dot():
+    """chi2(x, y) = 1 - 2 sum xy/(x+y) >= 1 - sum sqrt(x) sqrt(y), with
+    equality for x == y and value 1 for disjoint supports: the bound the BF16
+    Chi2 tiles prune with.  Their BF16 dot of the square roots stays within
+    the pruning band of the exact coefficient."""
+    X, _ = datagen.dirichlet(400, 784, k=10, seed=5)
+    Xs, rc = O.transform(X, CHI2)
+    assert rc == 0
+    R = np.sqrt(Xs.astype(np.float64))
+    BC = R[:200] @ R[200:].T
+    d = Xs.shape[1]
+    u = 2.0 ** -23
+    e = 2.0 ** -8 + 3 * u
+    Smax = 1.0 + 1e-5
+    band = 2.0 * ((2 * e + e * e) * Smax + (d + 2.0) * u * Smax * (1 + 2 * e) + 2 * u) + 1e-12
+    Rb = _bf16_rne(np.sqrt(Xs)).astype(np.float32)
+    worst = 0.0
+    for i in range(200):
+        for j in range(0, 200, 7):
+            c = O.dist64(Xs[i], Xs[200 + j], CHI2)
+            assert c >= 1.0 - BC[i, j] - 1e-12
+            acc = np.float32(0.0)
+            for a, b in zip(Rb[i], Rb[200 + j]):
+                acc = np.float32(acc + np.float32(a * b))
+            worst = max(worst, abs(float(acc) - BC[i, j]))
+        same = O.dist64(Xs[i], Xs[i], CHI2)
+        assert abs(same - (1.0 - float((R[i] * R[i]).sum()))) < 1e-12
+    assert worst < band
+    x = np.zeros(d, np.float32)
+    y = np.zeros(d, np.float32)
+    x[:10] = 0.1
+    y[10:30] = 0.05
+    assert O.dist64(x, y, CHI2) == 1.0
+
+
 # ---------------------------------------------------------- selections
 def test_topk_topm_are_extreme_sets():
     rng = np.random.default_rng(5)
