@@ -712,7 +712,10 @@ static void compute_bands(const sdb_params& q, int prec, float eps, double* band
 // Distance tiles over the ctx's list groups (c->h_grp_ptr over nblk x L2
 // groups of entries, lists of c->Mt points, masks F -> hits E, colsum) and the
 // FP32/FP64 recheck of the pairs inside the error band.
-static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, double* band32_out) {
+static constexpr int kQueueFull = -100;  // internal: more queued pairs than the queue may hold
+
+static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, double* band_out,
+                           double* band32_out) {
     const sdb_params& q = c->prm;
     const long long n = c->n;
     const int M = c->Mt, W = c->W, S = c->S, L2 = c->L2;
@@ -720,8 +723,8 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
     // Cosine on MFMA tiles at either precision; Chi2 with BF16 on the BF16
     // tiles as a lower-bound prefilter (compute_bands); everything else on the
     // VALU tiles in FP32
-    const bool mfma = (q.metric == SDB_COSINE || (q.metric == SDB_CHI2 && q.precision == SDB_BF16)) && M <= 128;
-    const int prec = mfma ? q.precision : SDB_FP32;
+    const bool mfma = (q.metric == SDB_COSINE || (q.metric == SDB_CHI2 && precision == SDB_BF16)) && M <= 128;
+    const int prec = mfma ? precision : SDB_FP32;
     const bool prefilter = q.metric == SDB_CHI2 && prec == SDB_BF16;
 
     // tiles: contiguous ranges of one list's group
@@ -766,7 +769,7 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
 
     ENSURE(c->E, (size_t)n * S * W * 4);
     ENSURE(c->colsum, (size_t)L2 * M * 4);
-    ENSURE(c->qcount, 4);
+    ENSURE(c->qcount, 8);
     ENSURE(c->stats64, 64);
     if (c->qcap == 0) c->qcap = 1 << 22;
     ENSURE(c->queue, (size_t)c->qcap * 8);
@@ -788,25 +791,35 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
     da.magicS = ((1ull << 40) + (unsigned long long)S - 1) / (unsigned long long)S;
 
     HIPCHK(hipEventRecord(c->ev[3], st));
-    int nq = 0;
+    // queued pairs are counted in 64 bits (a pass may meet more than 2^31)
+    // and stored while they fit; a pass that overflows is run again with a
+    // queue of twice the count, up to kQueueMax entries (SDB_QUEUE_MAX lowers
+    // the limit)
+    long long kQueueMax = 1ll << 30;
+    if (const char* e = std::getenv("SDB_QUEUE_MAX")) kQueueMax = std::max(1ll, std::min(kQueueMax, std::atoll(e)));
+    unsigned long long nq64 = 0;
     for (int attempt = 0; attempt < 4; ++attempt) {
         da.queue = (unsigned long long*)c->queue.p;
-        da.qcount = P_<int>(c->qcount);
+        da.qcount = (unsigned long long*)c->qcount.p;
         da.qcap = (int)c->qcap;
         HIPCHK(hipMemsetAsync(c->colsum.p, 0, (size_t)L2 * M * 4, st));
-        HIPCHK(hipMemsetAsync(c->qcount.p, 0, 4, st));
+        HIPCHK(hipMemsetAsync(c->qcount.p, 0, 8, st));
         HIPCHK(hipEventRecord(c->ev[4], st));
         if (mfma) LCHK(launch_dist_mfma(da, prec, ntiles, st));
         else LCHK(launch_dist_valu(da, ntiles, st));
         HIPCHK(hipEventRecord(c->ev[5], st));
-        HIPCHK(hipMemcpyAsync(&nq, c->qcount.p, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&nq64, c->qcount.p, 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (nq <= c->qcap) break;
-        c->qcap = (long long)nq * 2;
-        if (c->qcap > (1ll << 30)) return fail(c, SDB_ENOMEM, "recheck queue of %d pairs is too large", nq);
+        if (nq64 > (unsigned long long)kQueueMax) {
+            c->stats.n_queue = (long long)nq64;
+            return kQueueFull;
+        }
+        if (nq64 <= (unsigned long long)c->qcap) break;
+        c->qcap = std::min<long long>(kQueueMax, (long long)nq64 * 2);
         ENSURE(c->queue, (size_t)c->qcap * 8);
     }
-    if (nq > c->qcap) return fail(c, SDB_ENOMEM, "recheck queue overflow");
+    if (nq64 > (unsigned long long)c->qcap) return fail(c, SDB_ENOMEM, "recheck queue overflow");
+    const int nq = (int)nq64;
     c->stats.t_dist_ms = elapsed(c->ev[4], c->ev[5]);
     HIPCHK(hipMemsetAsync(c->stats64.p, 0, 64, st));
     LCHK(launch_recheck_k(da, P_<int>(c->topK), q.topK, q.n_proj, prec, band32, nq, (unsigned long long*)c->stats64.p, st));
@@ -815,6 +828,18 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
     *band_out = band;
     *band32_out = band32;
     return SDB_OK;
+}
+
+// The tiles of the context's precision; when they leave more pairs near eps
+// than the recheck queue holds (BF16 Chi2 lower bounds at a large eps), the
+// FP32 tiles, which decide all pairs outside a band of 2(d+8)u themselves.
+static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, double* band32_out) {
+    int r = run_distances_p(c, eps, c->prm.precision, nq_out, band_out, band32_out);
+    if (r == kQueueFull && c->prm.precision != SDB_FP32)
+        r = run_distances_p(c, eps, SDB_FP32, nq_out, band_out, band32_out);
+    if (r == kQueueFull)
+        return fail(c, SDB_ENOMEM, "recheck queue of %lld pairs is too large", c->stats.n_queue);
+    return r;
 }
 
 int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* degree_out) {

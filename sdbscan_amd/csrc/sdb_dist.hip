@@ -179,8 +179,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
                     colacc[t] += hit ? 1 : 0;
                     const uint32_t er = __shfl(ent[q], row, kWave);
                     if (unc) {
-                        int pos = atomicAdd(a.qcount, 1);
-                        if (pos < a.qcap) a.queue[pos] = queue_entry(er, l, col);
+                        const unsigned long long pos = atomicAdd(a.qcount, 1ull);
+                        if (pos < (unsigned long long)a.qcap) a.queue[pos] = queue_entry(er, l, col);
                     }
                 }
             }
@@ -400,8 +400,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
                     const int bp = __ffs(unc) - 1;
                     unc &= unc - 1;
                     const int col = (bp >> 2) * 16 + 4 * g + (bp & 3);
-                    const int pos = atomicAdd(a.qcount, 1);
-                    if (pos < a.qcap) a.queue[pos] = queue_entry(ent, l, col);
+                    const unsigned long long pos = atomicAdd(a.qcount, 1ull);
+                    if (pos < (unsigned long long)a.qcap) a.queue[pos] = queue_entry(ent, l, col);
                 }
             }
             if (!has_next) break;
@@ -604,8 +604,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu(DistArgs a) {
                 }
                 if (hit) atomicAdd(&col_lds[ci], 1);
                 if (unc) {
-                    int pos = atomicAdd(a.qcount, 1);
-                    if (pos < a.qcap) a.queue[pos] = queue_entry(ent[q], l, ci);
+                    const unsigned long long pos = atomicAdd(a.qcount, 1ull);
+                    if (pos < (unsigned long long)a.qcap) a.queue[pos] = queue_entry(ent[q], l, ci);
                 }
             }
         }

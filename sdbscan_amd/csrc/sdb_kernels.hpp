@@ -56,7 +56,7 @@ struct DistArgs {
     uint32_t* E;          // hit masks (n*S x W)
     int* colsum;          // L2 x M
     unsigned long long* queue;  // (entry << 32) | column
-    int* qcount;
+    unsigned long long* qcount;  // 64-bit count of queued pairs (stored while < qcap)
     int qcap;
     float eps;
     // fast distance d_hat: hit iff d_hat < lo, queued for the recheck iff

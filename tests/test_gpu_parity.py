@@ -266,6 +266,20 @@ def test_chi2_bf16_prefilter_matches_oracle(ctx, n, d, K, M, eps, minPts):
     assert np.abs(xb - rne).max() <= 1
 
 
+def test_chi2_bf16_queue_overflow_falls_back_to_fp32_tiles(ctx, monkeypatch):
+    """More pairs near eps than the recheck queue may hold: the pass is
+    redone on the FP32 tiles, with the same decisions."""
+    X, _ = datagen.dirichlet(5000, 100, k=5, seed=161)
+    full_check(ctx, X, S.CHI2, 0.25, 20, K=8, M=50, ker=256, n_proj=256, precision=S.BF16)
+    nq = ctx.stats()["n_queue"]
+    assert nq > 1000
+    monkeypatch.setenv("SDB_QUEUE_MAX", "1000")
+    p, bits, R, tk, tm, Xt, st = full_check(ctx, X, S.CHI2, 0.25, 20, K=8, M=50, ker=256, n_proj=256,
+                                            precision=S.BF16)
+    print("queue limit 1000: queued", st["n_queue"], "of", nq, "with BF16 tiles", flush=True)
+    assert st["n_queue"] < nq
+
+
 def test_device_dirichlet_generator(ctx):
     """Generator kind 1: nonnegative rows summing to one, reproducible from the
     seed, with the mass of each row concentrated on a few bins."""
