@@ -675,9 +675,13 @@ static void cosine_cuts(double lo, double hi, float* s_hit, float* s_near) {
 // Chi2 on BF16 rows (prec BF16): the tiles compute BC~, the dot of the BF16
 // square roots of the sum-normalised rows (their Bhattacharyya coefficient).
 // Term by term 2xy/(x+y) <= sqrt(xy), so chi2(x, y) = 1 - 2 sum xy/(x+y) >=
-// 1 - BC: a pair with 1 - BC~ > eps + band has chi2 > eps.  The tiles declare
-// no hits; every other pair goes to the recheck, which decides with the FP32
-// chi2 outside band32 and with the FP64 chi2 inside it.
+// 1 - BC: a pair with 1 - BC~ > eps + band has chi2 > eps.  And term by term
+// 2 sqrt(xy) - 2xy/(x+y) <= (x+y)/2 (the difference is (x+y-2 sqrt(xy))^2 /
+// (2(x+y))), so with sum x = sum y = 1 (within 4u after the transform)
+// chi2 <= 2(1 - BC) + 4u: a pair with 2(1 - BC~ + band) + 1e-6 < eps has
+// chi2 < eps and is a hit on the tiles.  Every other pair goes to the recheck,
+// which decides with the FP32 chi2 outside band32 and with the FP64 chi2
+// inside it.
 static void compute_bands(const sdb_params& q, int prec, float eps, double* band, double* band32) {
     const double u = std::ldexp(1.0, -23);
     const double d = q.n_features;
@@ -721,7 +725,7 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     const int M = c->Mt, W = c->W, S = c->S, L2 = c->L2;
     hipStream_t st = c->st;
     // Cosine on MFMA tiles at either precision; Chi2 with BF16 on the BF16
-    // tiles as a lower-bound prefilter (compute_bands); everything else on the
+    // tiles through bounds on chi2 (compute_bands); everything else on the
     // VALU tiles in FP32
     const bool mfma = (q.metric == SDB_COSINE || (q.metric == SDB_CHI2 && precision == SDB_BF16)) && M <= 128;
     const int prec = mfma ? precision : SDB_FP32;
@@ -785,7 +789,7 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     da.E = P_<uint32_t>(c->E);
     da.colsum = P_<int>(c->colsum);
     da.eps = eps;
-    da.lo = prefilter ? -INFINITY : (double)eps - band;
+    da.lo = prefilter ? ((double)eps - 1e-6) / 2.0 - band : (double)eps - band;
     da.hi = (double)eps + band;
     cosine_cuts(da.lo, da.hi, &da.s_hit, &da.s_near);
     da.magicS = ((1ull << 40) + (unsigned long long)S - 1) / (unsigned long long)S;

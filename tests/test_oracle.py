@@ -128,11 +128,12 @@ def _bf16_rne(a):
     return (((r + 0x7FFF + ((r >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
 
 
-def test_chi2_bounded_below_by_sqrt_dot():
-    """chi2(x, y) = 1 - 2 sum xy/(x+y) >= 1 - sum sqrt(x) sqrt(y), with
-    equality for x == y and value 1 for disjoint supports: the bound the BF16
-    Chi2 tiles prune with.  Their BF16 dot of the square roots stays within
-    the pruning band of the exact coefficient."""
+def test_chi2_bounded_by_sqrt_dot():
+    """1 - BC <= chi2(x, y) = 1 - 2 sum xy/(x+y) <= 2 (1 - BC) (+4u for the
+    rounded row sums) with BC = sum sqrt(x) sqrt(y), equality on the left for
+    x == y, and value 1 for disjoint supports: the bounds the BF16 Chi2 tiles
+    prune and accept with.  Their BF16 dot of the square roots stays within
+    the band of the exact coefficient."""
     X, _ = datagen.dirichlet(400, 784, k=10, seed=5)
     Xs, rc = O.transform(X, CHI2)
     assert rc == 0
@@ -148,7 +149,7 @@ def test_chi2_bounded_below_by_sqrt_dot():
     for i in range(200):
         for j in range(0, 200, 7):
             c = O.dist64(Xs[i], Xs[200 + j], CHI2)
-            assert c >= 1.0 - BC[i, j] - 1e-12
+            assert 1.0 - BC[i, j] - 1e-12 <= c <= 2.0 * (1.0 - BC[i, j]) + 4 * u
             acc = np.float32(0.0)
             for a, b in zip(Rb[i], Rb[200 + j]):
                 acc = np.float32(acc + np.float32(a * b))
