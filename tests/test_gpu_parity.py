@@ -183,6 +183,38 @@ def test_c0_parameters_k10_m100(ctx):
     assert index_equal(tk, tko, tm, tmo, 10, 100)
 
 
+@pytest.mark.parametrize("precision", [S.FP32, S.BF16])
+@pytest.mark.parametrize("case", ["c0", "chi2"])
+def test_end_to_end_matches_reference_arithmetic_fit(ctx, case, precision):
+    """The whole path on raw rows against the oracle's full fit, which decides
+    with the reference's FP32 arithmetic (sequential sums): c0 exactly
+    (20,000 x 784 cosine, topK 5, topM 50, eps 0.25, minPts 50), and 10,000
+    Chi2 histograms at the 8M x 784 Chi2 parameters with eps 0.47, minPts 10
+    (at eps 0.2 every point is noise).  On these inputs no pair's FP32 and
+    FP64 decisions differ, so labels, cluster counts and core flags must be
+    equal."""
+    if case == "c0":
+        X, _ = datagen.mixture(20000, 784, seed=1000)
+        metric, K, M, ker, eps, mp = S.COSINE, 5, 50, 784, 0.25, 50
+    else:
+        X, _ = datagen.dirichlet(10000, 784, seed=1004)
+        metric, K, M, ker, eps, mp = S.CHI2, 10, 100, 1024, 0.47, 10
+    n, d = X.shape
+    print(f"end to end {case} n={n} precision={precision}", flush=True)
+    p = ctx.set_params(n, d, n_proj=1024, topK=K, topM=M, metric=metric, ker_n_features=ker,
+                       precision=precision)
+    bits = S.hd3_bits(3 * p.fhtDim, 42)
+    ctx.load(X)
+    ctx.build_index(bits)
+    core, _ = ctx.find_core(eps, mp)
+    lab, nc = ctx.form_clusters()
+    olab, onc, ocore, _ = O.fit_sdbscan(X, metric, 1024, K, M, ker, 0.4, None, bits, p.fhtDim, eps, mp,
+                                        O.max_threads())
+    print("clusters", nc, "oracle", onc, "cores", int(core.sum()), flush=True)
+    assert np.array_equal(core, ocore)
+    assert np.array_equal(lab, olab) and nc == onc
+
+
 def test_duplicate_points_replay_tied_rows(ctx):
     X, _ = datagen.mixture(3000, 64, k=4, seed=21)
     X[1000:1500] = X[:500]          # exact duplicates: tied projections in every row
