@@ -535,20 +535,27 @@ def test_sng_exact_matches_oracle(ctx):
 
 
 @pytest.mark.parametrize("precision", [S.FP32, S.BF16])
-def test_sng_sampled_matches_oracle(ctx, precision):
+@pytest.mark.parametrize("metric", [S.COSINE, S.CHI2])
+def test_sng_sampled_matches_oracle(ctx, precision, metric):
     n, S_ = 5000, 500
-    X, _ = datagen.mixture(n, 128, k=5, sigma=0.3, seed=103)
+    if metric == S.COSINE:
+        X, _ = datagen.mixture(n, 128, k=5, sigma=0.3, seed=103)
+        d, eps = 128, 0.3
+    else:
+        X, _ = datagen.dirichlet(n, 100, k=5, seed=103)
+        d, eps = 100, 0.25
     seq = std_uniform_ints(11, n, S_ + 64)
     seq[10] = seq[20] = 7            # repeated draws
     seq[30] = 30                     # a point drawing itself
     win = sng_windows(seq, n, S_)
-    print("sng sampled n=5000 d=128 samples=500 precision", precision, flush=True)
-    ctx.set_params(n, 128, n_proj=128, topK=4, topM=20, metric=S.COSINE, precision=precision)
+    print(f"sng sampled n=5000 d={d} samples=500 metric {metric} precision {precision}", flush=True)
+    ctx.set_params(n, d, n_proj=128, topK=4, topM=20, metric=metric, ker_n_features=max(d, 128),
+                   precision=precision)
     ctx.load(X)
-    core, deg = ctx.sng_find_core(seq, 0.3, 12, win_end=win)
+    core, deg = ctx.sng_find_core(seq, eps, 12, win_end=win)
     lab, nc = ctx.form_clusters()
     Xt = ctx.get_transformed()
-    oc, od, (ptr, idx, dd) = O.sng_find_core(Xt, S.COSINE, seq, 0.3, 12, win_end=win)
+    oc, od, (ptr, idx, dd) = O.sng_find_core(Xt, metric, seq, eps, 12, win_end=win)
     olab, onc = O.form_cluster(oc, ptr, idx)
     print("cores", int(core.sum()), "clusters", nc, "queue", ctx.stats()["n_queue"], flush=True)
     assert np.array_equal(deg, od) and np.array_equal(core, oc)
