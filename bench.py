@@ -93,6 +93,18 @@ def run_path(ctx, bits, steps, warmup, tag, wl):
     return times, dist_ms, phase, lab, nc
 
 
+def tile_accuracy(ctx, wl, n_samples=1 << 20):
+    """Tile arithmetic against FP64 on candidate pairs sampled from the index,
+    and the exact count of computed pairs within 1e-6 of eps."""
+    a, b, fast, exact = ctx.sample_pair_dist(n_samples, seed=12345)
+    err = np.abs(fast.astype(np.float64) - exact)
+    st = ctx.stats()
+    return {"sampled_pairs": n_samples, "max_abs_err_vs_fp64": float(err.max()),
+            "mean_abs_err_vs_fp64": float(err.mean()), "band": st["band"],
+            "quantity": "1 - sum sqrt(x y) (chi2 lower bound)" if wl["metric"] == 3 else "distance",
+            "pairs_within_1e-6_of_eps": st["n_near_eps"]}
+
+
 def cpu_baseline(ctx, bits, wl, n_default):
     """The oracle (C++/OpenMP restatement with the reference's data structures)
     on the first rows of the same data, timed on this host."""
@@ -197,6 +209,10 @@ def main():
     log(f"generated {n} x {D_FEAT} ({args.workload}) on device in {time.perf_counter() - t0:.1f} s")
 
     times, dist_ms, phase, lab16, nc16 = run_path(ctx, bits, args.steps, args.warmup, "bf16", wl)
+    # after the timed steps: core flags (same eps, so only the core test runs)
+    # and the sampled accuracy of the tiles
+    core16, _ = ctx.find_core(wl["eps"], wl["minPts"])
+    accuracy = tile_accuracy(ctx, wl)
     total = float(sum(times))
     value = n * len(times) / total
     dist_avg = float(np.mean(dist_ms))
@@ -208,6 +224,9 @@ def main():
         # the FP32 Chi2 tiles take ~20 s per 8M step: one untimed-warmup-free step
         s32, w32 = (max(1, args.steps), 1) if wl["metric"] == S.COSINE else (1, 0)
         t32, d32, ph32, lab32, nc32 = run_path(ctx, bits, s32, w32, "fp32", wl)
+        core32, _ = ctx.find_core(wl["eps"], wl["minPts"])
+        accuracy["core_disagreements_vs_fp32"] = int(np.count_nonzero(core16 != core32))
+        accuracy["label_disagreements_vs_fp32"] = int(np.count_nonzero(lab16 != lab32))
         v32 = n * len(t32) / float(sum(t32))
         same = bool(np.array_equal(lab16, lab32)) and nc16 == nc32
         extra = {"fp32_points_per_sec": v32, "speedup_vs_fp32": value / v32,
@@ -247,6 +266,7 @@ def main():
         "recheck": {"queued": last["n_queue"], "fp32": last["n_recheck32"],
                     "fp64": last["n_recheck64"], "band": last["band"]},
         "n_clusters": nc16,
+        "accuracy": accuracy,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

@@ -78,6 +78,9 @@ typedef struct sdb_stats {
     int n_components;
     double band;                  /* error band of the fast distance */
     double band32;                /* error band of the FP32 recheck */
+    long long n_near_eps;         /* FP64-rechecked pairs within 1e-6 of eps: all such pairs
+                                     whenever the band exceeds 1e-6 (Cosine, Chi2, JS; L1/L2
+                                     when 2 eps (d + 8) 2^-23 > 1e-6) */
 } sdb_stats;
 
 /* Host random structures, the reference's std:: calls (no device work):
@@ -167,6 +170,16 @@ int sdb_core_dist(sdb_ctx* ctx, int minPts, float* core_dist_out);
 /* Reachability of formOptics_scikit for a given ordering (n ints), computed
  * on the device from the neighbourhoods and core distances. */
 int sdb_optics_reach(sdb_ctx* ctx, const int* ordering, float* reach_out);
+
+/* Accuracy sample (no counterpart in the reference): n_samples candidate
+ * pairs (a, b) drawn from the index after sdb_build_index (a uniform, one of
+ * its 2 topK lists uniform, a position in it uniform; a itself skipped), with
+ * the quantity the distance tiles test computed in the tiles' precision
+ * (fast_out) and in FP64 from the FP32 rows (exact_out).  The quantity is the
+ * distance, except for Chi2 with SDB_BF16, where it is 1 - sum sqrt(x_i y_i).
+ * |fast - exact| is what the error band of sdb_stats.band bounds. */
+int sdb_sample_pair_dist(sdb_ctx* ctx, long long n_samples, unsigned long long seed, int* a_out, int* b_out,
+                         float* fast_out, double* exact_out);
 
 int sdb_get_stats(const sdb_ctx* ctx, sdb_stats* out);
 

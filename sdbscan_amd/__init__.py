@@ -40,7 +40,7 @@ class Stats(ctypes.Structure):
                 ("n_hubs", ctypes.c_longlong), ("n_tie_points", ctypes.c_longlong),
                 ("n_tie_rows", ctypes.c_longlong), ("n_core", ctypes.c_longlong),
                 ("n_components", ctypes.c_int), ("band", ctypes.c_double),
-                ("band32", ctypes.c_double)]
+                ("band32", ctypes.c_double), ("n_near_eps", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -87,6 +87,7 @@ _SIGS = {
     "sdb_batch_run": ([ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                        ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "sdb_get_stats": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
+    "sdb_sample_pair_dist": ([_P, ctypes.c_longlong, ctypes.c_ulonglong, _P, _P, _P, _P], ctypes.c_int),
     "sdb_generate": ([_P, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_ulonglong,
                       ctypes.c_float], ctypes.c_int),
     "sdb_copy_input": ([_P, ctypes.c_longlong, ctypes.c_longlong, _P], ctypes.c_int),
@@ -334,6 +335,18 @@ class Context:
         s = Stats()
         self._chk(self.lib.sdb_get_stats(self.h, ctypes.byref(s)))
         return s.as_dict()
+
+    def sample_pair_dist(self, n_samples, seed=1):
+        """Candidate pairs drawn from the index with the tiles' value of the
+        tested quantity and its FP64 value: (a, b, fast, exact)."""
+        import numpy as np
+        a = np.zeros(n_samples, np.int32)
+        b = np.zeros(n_samples, np.int32)
+        f = np.zeros(n_samples, np.float32)
+        e = np.zeros(n_samples, np.float64)
+        self._chk(self.lib.sdb_sample_pair_dist(self.h, n_samples, seed, a.ctypes.data, b.ctypes.data,
+                                                 f.ctypes.data, e.ctypes.data))
+        return a, b, f, e
 
     def copy_input(self, row0, rows):
         import numpy as np

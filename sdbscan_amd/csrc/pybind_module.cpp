@@ -187,6 +187,7 @@ PYBIND11_MODULE(sDbscan, m) {
             d["n_components"] = t.n_components;
             d["band"] = t.band;
             d["band32"] = t.band32;
+            d["n_near_eps"] = t.n_near_eps;
             return d;
         });
 }

@@ -24,6 +24,9 @@ int launch_recheck_k(const DistArgs& a, const int* topK, int K, int P, int preci
                      double band32, int n_entries, unsigned long long* stats, hipStream_t st);
 int launch_gen_mixture(float* X, long long n, int d, unsigned long long seed, float sigma,
                        hipStream_t st);
+int launch_pair_sample(const DistArgs& a, const int* topK, int K, int P, long long n, int precision,
+                       unsigned long long seed, long long n_samples, int* pa, int* pb, float* fast,
+                       double* exact, hipStream_t st);
 int launch_gen_dirichlet(float* X, long long n, int d, unsigned long long seed, const uint16_t* bins,
                          hipStream_t st);
 int launch_csr_fill(const ClusterArgs& c, const long long* ptr, int* cursor, int* idx,
@@ -846,6 +849,49 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
     return r;
 }
 
+int sdb_sample_pair_dist(sdb_ctx* c, long long n_samples, unsigned long long seed, int* pa_out, int* pb_out,
+                         float* fast_out, double* exact_out) {
+    if (!c || n_samples < 0 || (n_samples > 0 && (!pa_out || !pb_out || !fast_out || !exact_out))) return SDB_EINVAL;
+    if (!c->indexed || c->sng) return fail(c, SDB_ESTATE, "build_index first");
+    if (n_samples > (1ll << 26)) return fail(c, SDB_EINVAL, "at most 2^26 sampled pairs per call");
+    HIPCHK(hipSetDevice(c->device));
+    const sdb_params& q = c->prm;
+    hipStream_t st = c->st;
+    // the precision the distance tiles run at (as in run_distances_p)
+    const bool mfma = (q.metric == SDB_COSINE || (q.metric == SDB_CHI2 && q.precision == SDB_BF16)) && c->Mt <= 128;
+    const int prec = mfma ? q.precision : SDB_FP32;
+    DistArgs da{};
+    da.X = prec == SDB_BF16 ? (const void*)c->Xb.p : (const void*)c->Xn.p;
+    da.X32 = P_<float>(c->Xn);
+    da.ldx = c->ldx; da.d = c->d; da.metric = q.metric;
+    da.M = c->Mt;
+    da.lists = P_<int>(c->lists);
+    // one allocation: FP64 values first (8-byte aligned), then the FP32
+    // values and the two index arrays
+    const size_t ns = (size_t)std::max<long long>(n_samples, 1);
+    void* base = nullptr;
+    if (hipMalloc(&base, ns * 20) != hipSuccess) return fail(c, SDB_ENOMEM, "sample buffer");
+    double* ex = (double*)base;
+    float* fs = (float*)(ex + ns);
+    int* pa = (int*)(fs + ns);
+    int* pb = pa + ns;
+    int r = SDB_OK;
+    if (launch_pair_sample(da, P_<int>(c->topK), q.topK, q.n_proj, c->n, prec, seed, n_samples, pa, pb, fs, ex, st))
+        r = SDB_EHIP;
+    if (!r && n_samples > 0) {
+        if (hipMemcpyAsync(pa_out, pa, (size_t)n_samples * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(pb_out, pb, (size_t)n_samples * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(fast_out, fs, (size_t)n_samples * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(exact_out, ex, (size_t)n_samples * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            r = SDB_EHIP;
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(base);
+    if (r) return fail(c, r, "sampled pair distances failed");
+    return SDB_OK;
+}
+
 int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* degree_out) {
     if (!c) return SDB_EINVAL;
     if (!c->indexed || c->sng) return fail(c, SDB_ESTATE, "build_index first");
@@ -902,6 +948,7 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
     c->stats.n_recheck32 = (long long)hs[0];
     c->stats.n_recheck64 = (long long)hs[1];
     c->stats.n_recheck_hits = (long long)hs[2];
+    c->stats.n_near_eps = (long long)hs[4];
     c->stats.n_core = (long long)hs[3];
     c->stats.band = band;
     c->stats.band32 = band32;
@@ -1473,6 +1520,7 @@ int sdb_sng_find_core(sdb_ctx* c, const int* samples, int n_samples, const int* 
     c->stats.n_recheck32 = (long long)hs[0];
     c->stats.n_recheck64 = (long long)hs[1];
     c->stats.n_recheck_hits = (long long)hs[2];
+    c->stats.n_near_eps = (long long)hs[4];
     c->stats.n_core = (long long)hs[3];
     c->stats.band = band;
     c->stats.band32 = band32;

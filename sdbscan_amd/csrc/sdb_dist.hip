@@ -647,7 +647,8 @@ int launch_dist_valu(const DistArgs& a, int ntiles, hipStream_t st) {
 // 32..1).  The BF16 paths (Cosine, and the Chi2 prefilter) first re-decide
 // with the FP32 distance when it lies outside the FP32 band; everything else is
 // decided with the FP64 value.
-// stats[0]: FP32 rechecks, stats[1]: FP64 rechecks, stats[2]: hits added.
+// stats[0]: FP32 rechecks, stats[1]: FP64 rechecks, stats[2]: hits added,
+// stats[4]: FP64 distances within 1e-6 of eps.
 __device__ __forceinline__ double dist64_wave(const float* x, const float* y, int d, int metric, int lane) {
     double s = 0.0;
     for (int i = lane; i < d; i += 64) {
@@ -717,6 +718,9 @@ __global__ __launch_bounds__(256) void k_recheck(DistArgs a, const int* topK, in
         if (!decided) {
             const double d64 = dist64_wave(x, y, a.d, a.metric, lane);
             if (lane == 0) atomicAdd(&stats[1], 1ull);
+            // pairs decided in FP32 (tiles or recheck) lie at least a band from eps,
+            // which exceeds 1e-6 except for L1/L2 at small eps * d
+            if (lane == 0 && fabs(d64 - (double)a.eps) < 1e-6) atomicAdd(&stats[4], 1ull);
             hit = d64 <= (double)a.eps;
         }
         if (hit && lane == 0) {
@@ -725,6 +729,82 @@ __global__ __launch_bounds__(256) void k_recheck(DistArgs a, const int* topK, in
             atomicAdd(&stats[2], 1ull);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Accuracy sample: candidate pairs drawn from the index (point a uniform,
+// slot s uniform, list position uniform; the point itself skipped), one
+// wavefront per pair.  fast: the quantity the tiles test, in the tiles'
+// precision (BF16 operands or FP32, FP32 sums over lanes k = lane, lane + 64,
+// ...); exact: the same quantity in FP64 from the FP32 rows.  The quantity is
+// the distance, except for Chi2 on BF16 rows, where it is 1 - sum sqrt(x y).
+__device__ __forceinline__ uint64_t acc_mix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_pair_sample(DistArgs a, const int* topK, int K, int P, long long n,
+                                                     int precision, uint64_t seed, long long n_samples,
+                                                     int* pa, int* pb, float* fast, double* exact) {
+    const int lane = lane_id();
+    const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
+    const bool sq = a.metric == kChi2 && precision == kBF16;
+    for (long long e = wv; e < n_samples; e += nwv) {
+        const uint64_t h = acc_mix(seed ^ acc_mix((uint64_t)e));
+        const long long p = (long long)(h % (uint64_t)n);
+        const int s = (int)((h >> 40) % (uint64_t)(2 * K));
+        int i = (int)(acc_mix(h) % (uint64_t)a.M);
+        const int l = (s & 1) ? P + topK[p * 2 * K + K + (s >> 1)] : topK[p * 2 * K + (s >> 1)];
+        int c = a.lists[(size_t)l * a.M + i];
+        if (c == p) c = a.lists[(size_t)l * a.M + (i + 1) % a.M];
+        const float* x = a.X32 + (size_t)p * a.ldx;
+        const float* y = a.X32 + (size_t)c * a.ldx;
+        double ex;
+        float fs;
+        if (sq) {
+            double t = 0.0;
+            for (int k = lane; k < a.d; k += 64) t = t + sqrt((double)x[k]) * sqrt((double)y[k]);
+            ex = 1.0 - wave_sum_f64_tree(t);
+        } else {
+            ex = dist64_wave(x, y, a.d, a.metric, lane);
+        }
+        if (precision == kBF16) {
+            const uint16_t* xb = (const uint16_t*)a.X + (size_t)p * a.ldx;
+            const uint16_t* yb = (const uint16_t*)a.X + (size_t)c * a.ldx;
+            float t = 0.0f;
+            for (int k = lane; k < a.d; k += 64)
+                t = fmaf(__uint_as_float((uint32_t)xb[k] << 16), __uint_as_float((uint32_t)yb[k] << 16), t);
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) t = t + __shfl_xor(t, m, kWave);
+            fs = 1.0f - t;
+        } else if (a.metric == kCosine) {
+            fs = 1.0f - dot32_wave(x, y, a.d, lane);
+        } else if (a.metric == kChi2) {
+            fs = 1.0f - 2.0f * chi2sum32_wave(x, y, a.d, lane);
+        } else {
+            fs = (float)ex;  // the other metrics' FP32 tiles are not sampled here
+        }
+        if (lane == 0) {
+            pa[e] = (int)p;
+            pb[e] = c;
+            fast[e] = fs;
+            exact[e] = ex;
+        }
+    }
+}
+
+int launch_pair_sample(const DistArgs& a, const int* topK, int K, int P, long long n, int precision,
+                       unsigned long long seed, long long n_samples, int* pa, int* pb, float* fast,
+                       double* exact, hipStream_t st) {
+    if (n_samples <= 0) return 0;
+    long long blocks = (n_samples + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_pair_sample, dim3((unsigned)blocks), dim3(256), 0, st, a, topK, K, P, n, precision,
+                       (uint64_t)seed, n_samples, pa, pb, fast, exact);
+    return (int)hipGetLastError();
 }
 
 int launch_recheck_k(const DistArgs& a, const int* topK, int K, int P, int precision,

@@ -63,3 +63,27 @@ def test_host_rng_matches_oracle_kats():
     assert np.array_equal(r, np.asarray(O.gauss(512, 51, 0.0, 0.5, 7)).ravel())
     r = sdbscan_amd.rff_matrix(sdbscan_amd.L1, 64, 13, 1.0, 3)
     assert np.array_equal(r, np.asarray(O.cauchy(64, 13, 0.0, 1.0, 3)).ravel())
+
+
+def test_struct_layouts_match_ctypes(tmp_path):
+    """sdb_params, sdb_stats and sdb_batch_job as the C compiler lays them out
+    (size and every field offset) equal the ctypes mirrors."""
+    import subprocess
+    structs = {"sdb_params": sdbscan_amd.Params, "sdb_stats": sdbscan_amd.Stats,
+               "sdb_batch_job": sdbscan_amd.BatchJob}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sdbscan_amd.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)

@@ -312,6 +312,44 @@ def test_chi2_bf16_queue_overflow_falls_back_to_fp32_tiles(ctx, monkeypatch):
     assert st["n_queue"] < nq
 
 
+@pytest.mark.parametrize("metric,precision", [(S.COSINE, S.BF16), (S.COSINE, S.FP32), (S.CHI2, S.BF16),
+                                              (S.CHI2, S.FP32)])
+def test_sampled_tile_error_within_band(ctx, metric, precision):
+    """Candidate pairs sampled from the index: each pair is a real candidate,
+    the FP64 value matches numpy's, and the tiles' arithmetic stays within the
+    error band the eps decision relies on."""
+    n, d = 6000, 784
+    if metric == S.COSINE:
+        X, _ = datagen.mixture(n, d, sigma=0.3, seed=211)
+        eps, ker = 0.3, None
+    else:
+        X, _ = datagen.dirichlet(n, d, seed=211)
+        eps, ker = 0.47, 1024
+    p, bits, R = run_index(ctx, X, metric=metric, K=10, M=100, n_proj=1024, ker=ker, precision=precision)
+    ctx.find_core(eps, 10)
+    band = ctx.stats()["band"]
+    a, b, fast, exact = ctx.sample_pair_dist(100000, seed=7)
+    err = np.abs(fast.astype(np.float64) - exact)
+    print(f"metric {metric} precision {precision}: max |fast - exact| {err.max():.3e}, band {band:.3e}",
+          flush=True)
+    assert err.max() <= band
+    assert (a != b).all() and a.min() >= 0 and a.max() < n and b.min() >= 0 and b.max() < n
+    tk, tm = ctx.get_index()
+    K, M = 10, 100
+    Xt = ctx.get_transformed().astype(np.float64)
+    for j in range(0, 100000, 997):
+        lists = [tm[tk[a[j], k], :M] for k in range(K)] + [tm[tk[a[j], K + k], M:] for k in range(K)]
+        assert b[j] in np.concatenate(lists)
+        x, y = Xt[a[j]], Xt[b[j]]
+        if metric == S.COSINE:
+            ref = 1.0 - x @ y
+        elif precision == S.BF16:
+            ref = 1.0 - np.sqrt(x) @ np.sqrt(y)
+        else:
+            ref = O.dist64(Xt[a[j]].astype(np.float32), Xt[b[j]].astype(np.float32), S.CHI2)
+        assert abs(exact[j] - ref) < 1e-12
+
+
 def test_device_dirichlet_generator(ctx):
     """Generator kind 1: nonnegative rows summing to one, reproducible from the
     seed, with the mass of each row concentrated on a few bins."""
