@@ -99,7 +99,9 @@ def tile_accuracy(ctx, wl, n_samples=1 << 20):
     a, b, fast, exact = ctx.sample_pair_dist(n_samples, seed=12345)
     err = np.abs(fast.astype(np.float64) - exact)
     st = ctx.stats()
+    rel = err / np.maximum(np.abs(exact), 1e-30)
     return {"sampled_pairs": n_samples, "max_abs_err_vs_fp64": float(err.max()),
+            "max_rel_err_vs_fp64": float(rel.max()),
             "mean_abs_err_vs_fp64": float(err.mean()), "band": st["band"],
             "quantity": "1 - sum sqrt(x y) (chi2 lower bound)" if wl["metric"] == 3 else "distance",
             "pairs_within_1e-6_of_eps": st["n_near_eps"]}
@@ -120,6 +122,78 @@ def cpu_baseline(ctx, bits, wl, n_default):
     return {"value": n_s / dt, "unit": "points/sec", "cores": thr, "kind": "port",
             "sample": f"first {n_s} of the {N} points, same parameters, full fit (transform, index, "
                       f"core points, clusters) with {thr} OpenMP threads; {dt:.2f} s"}
+
+
+def l2_bench(args):
+    """1.8M x 51 blobs (12 anisotropic Gaussian blobs, condition <= 10, plus 5%
+    uniform noise), L2 with random Fourier features to 1024 dims, sigma = the
+    median pairwise L2 distance of a 10,000-point sample, topK 10, topM 100,
+    minPts 50.  The reference tests eps on the input-space L2 distance
+    (src/Utilities.h:218-219); eps = 0.3 on the embedded cosine distance
+    1 - exp(-r^2 / (2 sigma^2)) maps to r = sigma * sqrt(-2 ln 0.7).  FP32
+    VALU distance tiles.  One JSON line."""
+    import sdbscan_amd as S
+    from sdbscan_amd import datagen
+    n = args.n if args.n != N else 1_800_000
+    d = 51
+    t0 = time.perf_counter()
+    X, _ = datagen.blobs(n, d, seed=1001)
+    rng = np.random.default_rng(1001)
+    smp = X[rng.choice(n, 10000, replace=False)].astype(np.float64)
+    sq = (smp * smp).sum(1)
+    D2 = np.maximum(sq[:, None] + sq[None, :] - 2.0 * smp @ smp.T, 0.0)
+    sigma = float(np.sqrt(np.median(D2[np.triu_indices(10000, 1)])))
+    eps = sigma * float(np.sqrt(-2.0 * np.log(0.7)))
+    log(f"[l2] generated {n} x {d} in {time.perf_counter() - t0:.1f} s; sigma {sigma:.3f}, eps {eps:.3f}")
+    ctx = S.Context(0)
+    p = ctx.set_params(n, d, n_proj=P, topK=K, topM=M, metric=S.L2, ker_n_features=1024, ker_sigma=sigma)
+    bits = S.hd3_bits(3 * p.fhtDim, SEED_RNG)
+    R = S.rff_matrix(S.L2, 512, d, sigma, SEED_RNG)
+    ctx.load(X)
+    times, dist_ms = [], []
+    for i in range(args.warmup + args.steps):
+        t1 = time.perf_counter()
+        ctx.build_index(bits, R)
+        ctx.find_core(eps, 50, want=False)
+        lab, nc = ctx.form_clusters(want=(i == args.warmup + args.steps - 1))
+        t2 = time.perf_counter()
+        st = ctx.stats()
+        log(f"[l2] step {i} {1e3 * (t2 - t1):.1f} ms (index {st['t_index_ms']:.1f} dist {st['t_dist_ms']:.1f} "
+            f"core {st['t_core_ms']:.1f} cluster {st['t_cluster_ms']:.1f}) queue {st['n_queue']} "
+            f"cores {st['n_core']} clusters {nc}")
+        if i >= args.warmup:
+            times.append(t2 - t1)
+            dist_ms.append(st["t_dist_ms"])
+    core, _ = ctx.find_core(eps, 50)
+    acc = tile_accuracy(ctx, {"metric": S.L2})
+    value = n * len(times) / float(sum(times))
+    dist_avg = float(np.mean(dist_ms))
+    B = dist_bytes(n, d, 4)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        n_s = int(os.environ.get("SDB_CPU_SAMPLE", "200000"))
+        thr = O.max_threads()
+        t1 = time.perf_counter()
+        _, nc_s, _, tt = O.fit_sdbscan(X[:n_s], S.L2, P, K, M, 1024, 0.4, R, bits, p.fhtDim, eps, 50, thr)
+        dt = time.perf_counter() - t1
+        log(f"[cpu] n={n_s} threads={thr} {dt:.2f} s phases(ms)={tt.tolist()}")
+        cpu = {"value": n_s / dt, "unit": "points/sec", "cores": thr, "kind": "port",
+               "sample": f"first {n_s} of the {n} points, same parameters, full fit with {thr} OpenMP threads; "
+                         f"{dt:.2f} s"}
+    st = ctx.stats()
+    print(json.dumps({
+        "metric": "points/sec, 1.8M x 51 L2 (RFF 1024), FP32", "value": value, "unit": "points/sec",
+        "n_gpus": 1, "steps": len(times), "warmup": args.warmup, "ms_per_step": 1e3 * float(np.mean(times)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic: 12 anisotropic Gaussian blobs + 5% uniform noise (numpy, seed 1001), loaded from the host",
+        "config": {"workload": f"sDbscan L2 n={n} d={d} n_proj={P} topK={K} topM={M} ker_n_features=1024 "
+                               f"sigma={sigma:.4f} eps={eps:.4f} (0.3 embedded cosine) minPts=50, FP32 VALU tiles"},
+        "roofline": {"bound": "hbm", "kernel": "k_dist_valu<l2>", "achieved": B / (dist_avg * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": B / (dist_avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": B, "avg_ms": dist_avg},
+        "cpu_baseline": cpu, "n_clusters": nc, "n_core": int(core.sum()), "accuracy": acc,
+        "recheck": {"queued": st["n_queue"], "fp64": st["n_recheck64"], "band": st["band"]}}), flush=True)
 
 
 def batch_bench(args):
@@ -170,8 +244,9 @@ def main():
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="cosine",
-                    help="cosine: the 8M x 784 cosine step; chi2: the 8M x 784 Chi2 step")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["l2"], default="cosine",
+                    help="cosine: the 8M x 784 cosine step; chi2: the 8M x 784 Chi2 step; "
+                         "l2: the 1.8M x 51 L2 (RFF) step")
     ap.add_argument("--batch", action="store_true",
                     help="run the many-dataset sweep (sdb_batch_run) instead of the 8M x 784 step")
     ap.add_argument("--batch-datasets", type=int, default=64)
@@ -190,6 +265,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.workload == "l2":
+        l2_bench(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "

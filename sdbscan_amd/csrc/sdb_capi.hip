@@ -86,6 +86,7 @@ struct sdb_ctx {
     int ldx = 0;
     // transformed
     DevBuf Xn, Xb;
+    DevBuf emb;  // L1/L2 embeddings (n x fhtDim) of the index passes
     // index
     DevBuf bits, R, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
         cand_idx, topM, flags, lists, is_hub, hub_flags, hub_excl, hub_id, hub_cnt, hub_ptr,
@@ -240,7 +241,7 @@ void sdb_destroy(sdb_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
-    DevBuf* all[] = {&c->Xown, &c->Xn, &c->Xb, &c->bits, &c->R, &c->topK, &c->tie_idx, &c->tie_vals,
+    DevBuf* all[] = {&c->Xown, &c->Xn, &c->Xb, &c->emb, &c->bits, &c->R, &c->topK, &c->tie_idx, &c->tie_vals,
                      &c->tie_count, &c->cmax, &c->cmin, &c->thi, &c->tlo, &c->cand_cnt, &c->cand_key,
                      &c->cand_idx, &c->topM, &c->flags, &c->lists, &c->is_hub, &c->hub_flags,
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
@@ -418,6 +419,12 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     if (l12) {
         ENSURE(c->R, (size_t)(q.ker_n_features / 2) * d * 4);
         HIPCHK(hipMemcpyAsync(c->R.p, Rh, (size_t)(q.ker_n_features / 2) * d * 4, hipMemcpyHostToDevice, st));
+        // the embeddings of all rows at once (a tiled FP32 product with the
+        // same sequential sums), read as rows by the index passes
+        const int half = q.ker_n_features / 2;
+        ENSURE(c->emb, (size_t)n * D * 4);
+        if (2 * half < D) HIPCHK(hipMemsetAsync(c->emb.p, 0, (size_t)n * D * 4, st));
+        LCHK(launch_rff_embed(P_<float>(c->Xn), n, d, ldx, P_<float>(c->R), half, P_<float>(c->emb), D, st));
     }
 
     // ---- sample pass: top-M thresholds from every stride-th point
@@ -454,6 +461,8 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ia.n = n; ia.ldx = ldx; ia.d = d; ia.metric = q.metric; ia.ker = q.ker_n_features;
     ia.kerL = q.ker_intervalSampling;
     ia.R = l12 ? P_<float>(c->R) : nullptr;
+    ia.emb = l12 ? P_<float>(c->emb) : nullptr;
+    ia.ldE = D;
     ia.P = P; ia.K = K;
     ia.topK = P_<int>(c->topK);
     ia.tie_count = P_<int>(c->tie_count); ia.tie_cap = tie_cap;

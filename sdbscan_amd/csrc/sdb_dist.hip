@@ -784,8 +784,17 @@ __global__ __launch_bounds__(256) void k_pair_sample(DistArgs a, const int* topK
             fs = 1.0f - dot32_wave(x, y, a.d, lane);
         } else if (a.metric == kChi2) {
             fs = 1.0f - 2.0f * chi2sum32_wave(x, y, a.d, lane);
+        } else if (a.metric == kL1 || a.metric == kL2) {
+            float t = 0.0f;
+            for (int k = lane; k < a.d; k += 64) {
+                const float df = x[k] - y[k];
+                t = a.metric == kL2 ? fmaf(df, df, t) : t + fabsf(df);
+            }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) t = t + __shfl_xor(t, m, kWave);
+            fs = a.metric == kL2 ? sqrtf(t) : t;
         } else {
-            fs = (float)ex;  // the other metrics' FP32 tiles are not sampled here
+            fs = (float)ex;  // JS: its FP32 tiles are not sampled here
         }
         if (lane == 0) {
             pa[e] = (int)p;

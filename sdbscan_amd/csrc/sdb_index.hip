@@ -543,9 +543,68 @@ __device__ __forceinline__ void load_row(const float* row, int d, int lg, float 
 // (ker / d - 1) / 2 = 0 in src/Utilities.h:134-135 and :176-177): element e
 // is sqrt(x_e L) (JS: sqrt(2 x_e L / ln 4)) for e < min(d, ker) and x_e > 0,
 // else 0 -- elementwise, in registers; kEmbLds: the general embeddings through
-// the wave's LDS slice (embed_lds).  kEmbCos and kEmbSqrt fuse transformData
-// into the passes that read untransformed rows.
-enum EmbKind { kEmbCos = 0, kEmbSqrt = 1, kEmbLds = 2 };
+// the wave's LDS slice (embed_lds); kEmbRows: L1/L2 embeddings precomputed by
+// k_rff_embed (the same values), read as rows.  kEmbCos and kEmbSqrt fuse
+// transformData into the passes that read untransformed rows.
+enum EmbKind { kEmbCos = 0, kEmbSqrt = 1, kEmbLds = 2, kEmbRows = 3 };
+
+// L1/L2 random features of every row (src/sDbscan.cpp:59-64): E[p][r] =
+// cos(s), E[p][half + r] = sin(s) with s = R[0][r] x_0 + ... + R[d-1][r] x_{d-1}
+// as a sequential fmaf chain over the input dims, as in embed_lds, so the
+// values are the same.  A 64-point x 64-feature tile per workgroup (4 x 4 per
+// thread), its X and R slices staged through LDS 32 dims at a time.
+__global__ __launch_bounds__(256) void k_rff_embed(const float* __restrict__ Xn, long long n, int d, int ldx,
+                                                   const float* __restrict__ R, int half, float* __restrict__ E,
+                                                   int ldE) {
+    __shared__ float xs[64][33];
+    __shared__ float rs[32][64];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const long long p0 = (long long)blockIdx.x * 64;
+    const int r0 = blockIdx.y * 64;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int c0 = 0; c0 < d; c0 += 32) {
+        const int cn = min(32, d - c0);
+        for (int t = threadIdx.x; t < 64 * 32; t += 256) {
+            const int pp = t >> 5, cc = t & 31;
+            const long long p = p0 + pp;
+            xs[pp][cc] = (p < n && cc < cn) ? Xn[p * ldx + c0 + cc] : 0.0f;
+        }
+        for (int t = threadIdx.x; t < 32 * 64; t += 256) {
+            const int cc = t >> 6, rr = t & 63;
+            rs[cc][rr] = (cc < cn && r0 + rr < half) ? R[(size_t)(c0 + cc) * half + r0 + rr] : 0.0f;
+        }
+        __syncthreads();
+        for (int cc = 0; cc < cn; ++cc) {
+            float xv[4], rv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xv[i] = xs[4 * ty + i][cc];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rv[j] = rs[cc][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(rv[j], xv[i], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const long long p = p0 + 4 * ty + i;
+        if (p >= n) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = r0 + tx + 16 * j;
+            if (r < half) {
+                E[p * ldE + r] = cosf(acc[i][j]);
+                E[p * ldE + half + r] = sinf(acc[i][j]);
+            }
+        }
+    }
+}
 
 template <int LOGD, int MODE, int EMB>
 // Occupancy: 3 workgroups per CU up to D = 1024 (at most 168 registers),
@@ -594,7 +653,7 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
     }
     const long long stride = MODE == kSample ? a.sample_stride : 1;
     const long long npts = MODE == kPoints ? a.n_ids : (a.n + stride - 1) / stride;
-    constexpr bool RAW = EMB != kEmbLds && (MODE == kSample || MODE == kFull);  // reads untransformed rows
+    constexpr bool RAW = (EMB == kEmbCos || EMB == kEmbSqrt) && (MODE == kSample || MODE == kFull);  // reads untransformed rows
     for (long long q = gw; q < npts; q += nw) {
         const long long p = MODE == kPoints ? (long long)a.pt_ids[q] : q * stride;
         float v[VPL];
@@ -604,7 +663,9 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int w = 0; w < NW; ++w) asm volatile("" : "+v"(sg[r][w]));
-        if constexpr (EMB != kEmbLds) {
+        if constexpr (EMB == kEmbRows) {
+            load_row<VPL>(a.emb + p * a.ldE, a.ldE, lg, v);
+        } else if constexpr (EMB != kEmbLds) {
             if constexpr (!RAW) {
                 load_row<VPL>(a.Xn + p * a.ldx, a.d, lg, v);
             } else {
@@ -1101,6 +1162,17 @@ __global__ void k_group_ptr(const int* __restrict__ wgoff, int NG, int Gx, long 
 
 // ---------------------------------------------------------------------------
 // Host launchers
+int launch_rff_embed(const float* Xn, long long n, int d, int ldx, const float* R, int half, float* E, int ldE,
+                     hipStream_t st) {
+    if (n <= 0 || half <= 0) return 0;
+    if (2 * half > ldE) return -1;
+    const long long gx = (n + 63) / 64;
+    if (gx > 0x7fffffffll) return -3;
+    hipLaunchKernelGGL(k_rff_embed, dim3((unsigned)gx, (unsigned)((half + 63) / 64)), dim3(256), 0, st, Xn, n, d,
+                       ldx, R, half, E, ldE);
+    return (int)hipGetLastError();
+}
+
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
                      uint16_t* Xb, int* err, hipStream_t st) {
     long long blocks = (n + 3) / 4;
@@ -1128,7 +1200,12 @@ static int launch_index_m(const IndexArgs& a, int grid, hipStream_t st) {
     switch (index_embedding(a.metric, a.d, a.ker)) {
         case kEmbCos: return launch_index_t<LOGD, MODE, kEmbCos>(a, grid, st);
         case kEmbSqrt: return launch_index_t<LOGD, MODE, kEmbSqrt>(a, grid, st);
-        default: return launch_index_t<LOGD, MODE, kEmbLds>(a, grid, st);
+        default:
+            // the bulk passes read the precomputed L1/L2 embeddings; the
+            // replays of a few rows or points compute them in place
+            if constexpr (MODE == kSample || MODE == kFull)
+                if (a.emb && (a.metric == kL1 || a.metric == kL2)) return launch_index_t<LOGD, MODE, kEmbRows>(a, grid, st);
+            return launch_index_t<LOGD, MODE, kEmbLds>(a, grid, st);
     }
 }
 

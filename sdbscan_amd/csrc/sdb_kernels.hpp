@@ -40,6 +40,8 @@ struct IndexArgs {
     const int* pt_ids;    // points whose projections are written (mode 3)
     long long n_ids;
     float* pt_buf;        // P x n_ids (mode 3)
+    const float* emb;     // n x ldE precomputed L1/L2 embeddings (sample / full passes), or null
+    int ldE;
 };
 
 struct DistArgs {
@@ -76,6 +78,9 @@ struct DistArgs {
 // embedding kind of the index kernels for (metric, n_features, ker_n_features):
 // 0 and 1 fuse the transform into the index passes, 2 needs launch_transform
 int index_embedding(int metric, int d, int ker);
+// [cos(R x), sin(R x)] of every row (L1/L2 random features) into E (n x ldE)
+int launch_rff_embed(const float* Xn, long long n, int d, int ldx, const float* R, int half, float* E, int ldE,
+                     hipStream_t st);
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
                      uint16_t* Xb, int* err, hipStream_t st);
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st);
