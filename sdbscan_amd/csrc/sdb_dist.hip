@@ -617,6 +617,168 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu(DistArgs a) {
     }
 }
 
+// Rows that fit one LDS chunk (d <= 256): the candidate rows are staged once
+// per tile; every wave then streams its own query sets through a private LDS
+// slice, the next set's rows already loading into registers while the current
+// set is summed, with no workgroup barrier between sets.  The metric is a
+// template parameter; rows are zero padded to d4 = d rounded up to 4 (zeros
+// add nothing for Cosine, L1, L2 and Chi2; JS sums exactly d terms).
+template <int METRIC>
+__device__ __forceinline__ float valu_term(float x, float y) {
+    if constexpr (METRIC == kL2) { const float t = x - y; return t * t; }
+    else if constexpr (METRIC == kL1) return fabsf(x - y);
+    else if constexpr (METRIC == kChi2) return (x > 0.0f && y > 0.0f) ? x * y / (x + y) : 0.0f;
+    else if constexpr (METRIC == kJS) {
+        const float xe = x + 1e-6f, ye = y + 1e-6f, s = xe + ye;
+        return xe * (logf(s / xe) * 1.4426950408889634f) + ye * (logf(s / ye) * 1.4426950408889634f);
+    } else return x * y;
+}
+
+template <int WAVES, int QV, int CM, int METRIC>
+__global__ __launch_bounds__(WAVES * 64) void k_dist_valu1(DistArgs a) {
+    constexpr int KPL = 4;  // query values per lane: d <= 256
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int M = a.M, d = a.d;
+    const int d4 = (d + 3) & ~3;
+    const int kend = METRIC == kJS ? d : d4;
+    const int RS = d4 + 1;
+    float* cand = (float*)smem;
+    float* qbuf = cand + (size_t)M * RS;
+    int* col_lds = (int*)(qbuf + (size_t)WAVES * QV * d4);
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const int l = a.tile_list[blockIdx.x];
+    const int beg = a.tile_beg[blockIdx.x], end = a.tile_end[blockIdx.x];
+    const int* Lm = a.lists + (size_t)l * M;
+    const double lo = a.lo, hi = a.hi;
+    float* qb = qbuf + (size_t)w * QV * d4;
+
+    for (int i = tid; i < M; i += WAVES * 64) col_lds[i] = 0;
+    for (int v = tid; v < M * d4; v += WAVES * 64) {
+        const int row = v / d4, k = v - row * d4;
+        cand[(size_t)row * RS + k] = k < d ? a.X32[(size_t)Lm[row] * a.ldx + k] : 0.0f;
+    }
+    __syncthreads();
+
+    const int per_set = WAVES * QV;
+    const int nsets = (end - beg + per_set - 1) / per_set;
+    float pre[QV][KPL];
+    uint32_t entn[QV];
+    bool validn[QV];
+    auto fetch = [&](int set) {
+#pragma unroll
+        for (int q = 0; q < QV; ++q) {
+            const int rr = beg + set * per_set + w * QV + q;
+            validn[q] = rr < end;
+            entn[q] = a.grp[validn[q] ? rr : beg];
+            const long long pa = (long long)(entn[q] / (uint32_t)a.S);
+            const float* xr = a.X32 + (size_t)pa * a.ldx;
+#pragma unroll
+            for (int t = 0; t < KPL; ++t) {
+                const int k = lane + 64 * t;
+                pre[q][t] = k < d ? xr[k] : 0.0f;
+            }
+        }
+    };
+    if (nsets > 0) fetch(0);
+    for (int set = 0; set < nsets; ++set) {
+        uint32_t ent[QV];
+        bool valid[QV];
+#pragma unroll
+        for (int q = 0; q < QV; ++q) {
+            ent[q] = entn[q];
+            valid[q] = validn[q];
+#pragma unroll
+            for (int t = 0; t < KPL; ++t) {
+                const int k = lane + 64 * t;
+                if (k < d4) qb[q * d4 + k] = pre[q][t];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (set + 1 < nsets) fetch(set + 1);
+        float acc[QV][CM];
+#pragma unroll
+        for (int q = 0; q < QV; ++q)
+#pragma unroll
+            for (int c = 0; c < CM; ++c) acc[q][c] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+            const int ci = lane + 64 * c;
+            const float* cr = cand + (size_t)(ci < M ? ci : M - 1) * RS;
+            for (int k = 0; k < kend; ++k) {
+                const float y = cr[k];
+#pragma unroll
+                for (int q = 0; q < QV; ++q) acc[q][c] = acc[q][c] + valu_term<METRIC>(qb[q * d4 + k], y);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QV; ++q) {
+            uint32_t Fw[2 * CM];
+#pragma unroll
+            for (int x = 0; x < 2 * CM; ++x) Fw[x] = (valid[q] && x < a.W) ? a.F[(size_t)ent[q] * a.W + x] : 0u;
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+                const int ci = lane + 64 * c;
+                const float s = acc[q][c];
+                float dist;
+                if constexpr (METRIC == kL2) dist = sqrtf(s);
+                else if constexpr (METRIC == kL1) dist = s;
+                else if constexpr (METRIC == kChi2) dist = 1.0f - 2.0f * s;
+                else if constexpr (METRIC == kJS) dist = 1.0f - s * 0.5f;
+                else dist = 1.0f - s;
+                const uint32_t fw = Fw[2 * c + (lane >> 5)];
+                const bool act = valid[q] && ci < M && ((fw >> (lane & 31)) & 1u);
+                const double dv = (double)dist;
+                const bool hit = act && dv < lo;
+                const bool unc = act && !(dv < lo) && !(dv > hi);
+                const unsigned long long bh = __ballot(hit);
+                if (valid[q] && lane == 0) {
+                    if (2 * c < a.W) a.E[(size_t)ent[q] * a.W + 2 * c] = (uint32_t)bh;
+                    if (2 * c + 1 < a.W) a.E[(size_t)ent[q] * a.W + 2 * c + 1] = (uint32_t)(bh >> 32);
+                }
+                if (hit) atomicAdd(&col_lds[ci], 1);
+                if (unc) {
+                    const unsigned long long pos = atomicAdd(a.qcount, 1ull);
+                    if (pos < (unsigned long long)a.qcap) a.queue[pos] = queue_entry(ent[q], l, ci);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __syncthreads();
+    for (int i = tid; i < M; i += WAVES * 64) {
+        const int v = col_lds[i];
+        if (v) atomicAdd(&a.colsum[(size_t)l * M + i], v);
+    }
+}
+
+template <int CM, int METRIC>
+static int launch_valu1(const DistArgs& a, int ntiles, hipStream_t st) {
+    constexpr int WAVES = 8, QV = 8;
+    const size_t d4 = (size_t)((a.d + 3) & ~3);
+    const size_t sm = ((size_t)a.M * (d4 + 1) + (size_t)WAVES * QV * d4) * 4 + (size_t)a.M * 4;
+    if (sm > (size_t)kLdsBytes) return -2;
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_valu1<WAVES, QV, CM, METRIC>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL((k_dist_valu1<WAVES, QV, CM, METRIC>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
+    return (int)hipGetLastError();
+}
+
+template <int CM>
+static int launch_valu1_m(const DistArgs& a, int ntiles, hipStream_t st) {
+    switch (a.metric) {
+        case kL2: return launch_valu1<CM, kL2>(a, ntiles, st);
+        case kL1: return launch_valu1<CM, kL1>(a, ntiles, st);
+        case kChi2: return launch_valu1<CM, kChi2>(a, ntiles, st);
+        case kJS: return launch_valu1<CM, kJS>(a, ntiles, st);
+        default: return launch_valu1<CM, kCosine>(a, ntiles, st);
+    }
+}
+
 template <int CM>
 static int launch_valu_cm(const DistArgs& a, int ntiles, hipStream_t st) {
     constexpr int WAVES = 8, QV = 4;
@@ -632,6 +794,16 @@ static int launch_valu_cm(const DistArgs& a, int ntiles, hipStream_t st) {
 int launch_dist_valu(const DistArgs& a, int ntiles, hipStream_t st) {
     if (ntiles <= 0) return 0;
     const int CM = (a.M + 63) / 64;
+    if (a.d <= 256 && CM <= 4) {
+        int r = -2;
+        switch (CM) {
+            case 1: r = launch_valu1_m<1>(a, ntiles, st); break;
+            case 2: r = launch_valu1_m<2>(a, ntiles, st); break;
+            case 3: r = launch_valu1_m<3>(a, ntiles, st); break;
+            case 4: r = launch_valu1_m<4>(a, ntiles, st); break;
+        }
+        if (r != -2) return r;  // -2: the tile does not fit LDS, take the chunked tiles
+    }
     switch (CM) {
         case 1: return launch_valu_cm<1>(a, ntiles, st);
         case 2: return launch_valu_cm<2>(a, ntiles, st);
