@@ -1039,9 +1039,44 @@ __global__ void k_pair_fill(const int* __restrict__ pos_hub, const int* __restri
 // entry: the slot lists go to LDS for the duplicate checks, and the W mask
 // words of consecutive entries are written by consecutive threads (one
 // 16-byte store each when W = 4).
+// Each list's pair segment sorted by (other list, position) in LDS, one
+// workgroup per list, so that k_first_mask can look up one other list by
+// binary search; segments longer than kPairSortCap stay unsorted (flag 0) and
+// are scanned.
+constexpr int kPairSortCap = 4096;
+__global__ __launch_bounds__(256) void k_pair_sort(const int* __restrict__ poff, int M, uint32_t* __restrict__ pairs,
+                                                   uint8_t* __restrict__ sorted) {
+    __shared__ uint32_t keys[kPairSortCap];
+    const int l = blockIdx.x;
+    const int q0 = poff[(size_t)l * M], q1 = poff[(size_t)(l + 1) * M];
+    const int len = q1 - q0;
+    if (len > kPairSortCap) {
+        if (threadIdx.x == 0) sorted[l] = 0;
+        return;
+    }
+    int np = 1;
+    while (np < len) np <<= 1;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) keys[i] = i < len ? pairs[q0 + i] : 0xffffffffu;
+    __syncthreads();
+    for (int k = 2; k <= np; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const uint32_t x = keys[i], y = keys[ixj];
+                    if ((x > y) == ((i & k) == 0)) { keys[i] = y; keys[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < len; i += blockDim.x) pairs[q0 + i] = keys[i];
+    if (threadIdx.x == 0) sorted[l] = 1;
+}
+
 __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK, long long n, int P, int K,
                                                     int M, int W, int ppb, const int* __restrict__ poff,
                                                     const uint32_t* __restrict__ pairs,
+                                                    const uint8_t* __restrict__ psorted,
                                                     const int* __restrict__ hub_id,
                                                     const int* __restrict__ hub_ptr,
                                                     const int* __restrict__ hub_mem,
@@ -1066,16 +1101,38 @@ __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK
     if (s > 0) {
         // positions of l whose point is also in another list L: a duplicate
         // when L is one of the earlier slots' lists
-        const int q1 = poff[(size_t)(l + 1) * M];
-        for (int q = poff[(size_t)l * M]; q < q1; ++q) {
-            const uint32_t pr = pairs[q];
-            const int L = (int)(pr >> 8), i = (int)(pr & 255u);
-            bool dup = false;
-            for (int s2 = 0; s2 < s; ++s2) dup |= mine[s2] == L;
-            if (dup) {
+        const int q0 = poff[(size_t)l * M], q1 = poff[(size_t)(l + 1) * M];
+        if (q1 - q0 > 32 && psorted[l]) {
+            // long segment, sorted by (L, position): one lower bound per
+            // earlier slot's list
+            for (int s2 = 0; s2 < s; ++s2) {
+                const uint32_t key = (uint32_t)mine[s2] << 8;
+                int lo = q0, hi = q1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (pairs[mid] < key) lo = mid + 1;
+                    else hi = mid;
+                }
+                for (int q = lo; q < q1; ++q) {
+                    const uint32_t pr = pairs[q];
+                    if ((pr >> 8) != (uint32_t)mine[s2]) break;
+                    const int i = (int)(pr & 255u);
 #pragma unroll
-                for (int x = 0; x < kMaxW; ++x)
-                    if (x == (i >> 5)) m[x] &= ~(1u << (i & 31));
+                    for (int x = 0; x < kMaxW; ++x)
+                        if (x == (i >> 5)) m[x] &= ~(1u << (i & 31));
+                }
+            }
+        } else {
+            for (int q = q0; q < q1; ++q) {
+                const uint32_t pr = pairs[q];
+                const int L = (int)(pr >> 8), i = (int)(pr & 255u);
+                bool dup = false;
+                for (int s2 = 0; s2 < s; ++s2) dup |= mine[s2] == L;
+                if (dup) {
+#pragma unroll
+                    for (int x = 0; x < kMaxW; ++x)
+                        if (x == (i >> 5)) m[x] &= ~(1u << (i & 31));
+                }
             }
         }
     }
@@ -1299,13 +1356,14 @@ int launch_pair_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem,
 }
 
 int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const int* poff,
-                      const uint32_t* pairs, const int* hub_id, const int* hub_ptr, const int* hub_mem,
+                      uint32_t* pairs, uint8_t* psorted, const int* hub_id, const int* hub_ptr, const int* hub_mem,
                       uint32_t* F, hipStream_t st) {
     const int S = 2 * K;
+    hipLaunchKernelGGL(k_pair_sort, dim3(2 * P), dim3(256), 0, st, poff, M, pairs, psorted);
     const int ppb = std::max(1, 512 / S);
     const long long blocks = (n + ppb - 1) / ppb;
     hipLaunchKernelGGL(k_first_mask, dim3((unsigned)blocks), dim3(ppb * S), (size_t)ppb * S * 4, st, topK, n, P, K, M,
-                       W, ppb, poff, pairs, hub_id, hub_ptr, hub_mem, F);
+                       W, ppb, poff, pairs, psorted, hub_id, hub_ptr, hub_mem, F);
     return (int)hipGetLastError();
 }
 
