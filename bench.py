@@ -172,7 +172,7 @@ def l2_bench(args):
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
-        n_s = int(os.environ.get("SDB_CPU_SAMPLE", "200000"))
+        n_s = int(os.environ.get("SDB_CPU_SAMPLE", "50000"))
         thr = O.max_threads()
         t1 = time.perf_counter()
         _, nc_s, _, tt = O.fit_sdbscan(X[:n_s], S.L2, P, K, M, 1024, 0.4, R, bits, p.fhtDim, eps, 50, thr)
@@ -191,7 +191,8 @@ def l2_bench(args):
                                f"sigma={sigma:.4f} eps={eps:.4f} (0.3 embedded cosine) minPts=50, FP32 VALU tiles"},
         "roofline": {"bound": "hbm", "kernel": "k_dist_valu<l2>", "achieved": B / (dist_avg * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": B / (dist_avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": B, "avg_ms": dist_avg},
+                     "traffic": None, "algorithmic_bytes": B, "avg_ms": dist_avg,
+                     "valu_fp32_tflops": 2.0 * n * 2 * K * M * d / (dist_avg * 1e-3) / 1e12},
         "cpu_baseline": cpu, "n_clusters": nc, "n_core": int(core.sum()), "accuracy": acc,
         "recheck": {"queued": st["n_queue"], "fp64": st["n_recheck64"], "band": st["band"]}}), flush=True)
 
