@@ -255,6 +255,23 @@ def test_l1_l2_random_features(ctx, metric, eps):
     assert agree > 0.99
 
 
+@pytest.mark.parametrize("metric,d,M,eps", [
+    (S.COSINE, 64, 200, 0.3),    # Cosine with topM > 128: VALU tiles, 4 candidate columns per lane
+    (S.L1, 40, 150, 25.0),       # 3 columns per lane, rows padded to 4
+    (S.L2, 300, 120, 10.6),      # rows longer than 256: k-chunked VALU tiles
+])
+def test_valu_tile_shapes(ctx, metric, d, M, eps):
+    if metric == S.COSINE:
+        X, _ = datagen.mixture(4000, d, k=5, sigma=0.3, seed=301)
+        kw = {}
+    else:
+        X, _ = datagen.blobs(4000, d, k=5, seed=301)
+        kw = dict(ker=1024, sigma=4.0 if metric == S.L1 else 6.0)
+    print(f"valu tiles metric {metric} d={d} M={M} eps={eps}", flush=True)
+    p, bits, R, tk, tm, Xt, st = full_check(ctx, X, metric, eps, 10, K=6, M=M, n_proj=256, **kw)
+    print("cores", st["n_core"], "queue", st["n_queue"], flush=True)
+
+
 @pytest.mark.parametrize("metric,eps", [(S.CHI2, 0.35), (S.JS, 0.3)])
 @pytest.mark.parametrize("ker", [512, 256])
 def test_chi2_js(ctx, metric, eps, ker):
