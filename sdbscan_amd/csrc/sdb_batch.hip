@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -32,7 +33,8 @@ typedef std::tuple<int, int, int> TuneKey;  // n_features, metric, precision
 
 struct TuneTable {
     std::mutex mu;
-    std::map<TuneKey, int> rows;  // > 0 tuned, 0 being tuned
+    std::condition_variable cv;
+    std::map<TuneKey, int> rows;  // > 0 tuned, 0 being tuned, -1 tuning failed (default tiles)
 };
 
 double now_ms() {
@@ -81,20 +83,28 @@ void run_job(sdb_ctx* c, sdb_batch_job& j, int autotune, TuneTable& tune) {
     const TuneKey key(p.n_features, p.metric, p.precision);
     bool tune_here = false;
     {
-        std::lock_guard<std::mutex> lk(tune.mu);
+        // a job of a class being tuned waits for the tuner's choice
+        std::unique_lock<std::mutex> lk(tune.mu);
         auto it = tune.rows.find(key);
-        if (it != tune.rows.end()) rows = it->second;
-        else if (autotune) { tune.rows[key] = 0; tune_here = true; }
+        if (it == tune.rows.end()) {
+            if (autotune) { tune.rows[key] = 0; tune_here = true; }
+        } else {
+            tune.cv.wait(lk, [&] { return tune.rows[key] != 0; });
+            rows = std::max(0, tune.rows[key]);
+        }
     }
     if (tune_here) {
         static const int cand[] = {256, 1024, 4096, 16384};
         int best = 0;
         double ms = 0.0;
         r = sdb_tune_tile_rows(c, eps_max, cand, 4, &best, &ms);
+        {
+            std::lock_guard<std::mutex> lk(tune.mu);
+            tune.rows[key] = r ? -1 : best;
+        }
+        tune.cv.notify_all();
         if (r) { job_fail(j, r, "tune_tile_rows", c); return; }
         rows = best;
-        std::lock_guard<std::mutex> lk(tune.mu);
-        tune.rows[key] = best;
     } else {
         r = sdb_set_tile_rows(c, rows);
         if (r) { job_fail(j, r, "set_tile_rows", c); return; }

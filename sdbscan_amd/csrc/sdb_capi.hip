@@ -1235,24 +1235,27 @@ typedef std::priority_queue<ReachPair, std::vector<ReachPair>, std::greater<Reac
 static void optics_walk(long long n, const long long* ptr, const int* idx, const float* dd, const uint8_t* core,
                         const float* cdist, int* ordering_out) {
     const float NEG = -2147483648.0f;  // src/Header.h:20
+    // one array for the reach and the processed flag: a processed point holds
+    // -inf, which no update test (reach == NEG or reach > rr) accepts
+    const float DONE = -INFINITY;
     std::vector<float> reach(n, NEG);
-    std::vector<uint8_t> done(n, 0);
     long long k = 0;
     auto expand = [&](int xi, ReachHeap& seeds) {
         const float cd = cdist[xi];
         for (long long j = ptr[xi]; j < ptr[xi + 1]; ++j) {
             const int xj = idx[j];
-            if (done[xj]) continue;
+            const float rj = reach[xj];
+            if (rj == DONE) continue;
             const float rr = std::max(cd, dd[j]);
-            if (reach[xj] == NEG || reach[xj] > rr) {
+            if (rj == NEG || rj > rr) {
                 reach[xj] = rr;
-                seeds.emplace(xj, reach[xj]);
+                seeds.emplace(xj, rr);
             }
         }
     };
     for (long long xi = 0; xi < n; ++xi) {
-        if (done[xi]) continue;
-        done[xi] = 1;
+        if (reach[xi] == DONE) continue;
+        reach[xi] = DONE;
         ordering_out[k++] = (int)xi;
         if (!core[xi]) continue;
         ReachHeap seeds;
@@ -1260,8 +1263,8 @@ static void optics_walk(long long n, const long long* ptr, const int* idx, const
         while (!seeds.empty()) {
             const int xj = seeds.top().idx;
             seeds.pop();
-            if (done[xj]) continue;
-            done[xj] = 1;
+            if (reach[xj] == DONE) continue;
+            reach[xj] = DONE;
             ordering_out[k++] = xj;
             if (core[xj]) expand(xj, seeds);
         }
