@@ -1,6 +1,7 @@
 // C ABI of the sDbscan HIP path: context, device buffers, pipeline order and
 // the host replay of top-K / top-M selections whose boundary is tied.
 #include <hipcub/hipcub.hpp>
+#include <immintrin.h>
 
 #include <algorithm>
 #include <cfloat>
@@ -1232,26 +1233,64 @@ struct ReachPair {
 typedef std::priority_queue<ReachPair, std::vector<ReachPair>, std::greater<ReachPair>> ReachHeap;
 }  // namespace
 
+// The neighbour scan of a core point: a neighbour xj with reach NEG (unseen)
+// or above rr = max(core_dist, d) takes rr and is pushed; processed points
+// hold -inf, which neither test accepts.
+static inline void expand_scalar(long long j0, long long j1, float cd, const int* idx, const float* dd, float* reach,
+                                 ReachHeap& seeds) {
+    const float NEG = -2147483648.0f;  // src/Header.h:20
+    for (long long j = j0; j < j1; ++j) {
+        const int xj = idx[j];
+        const float rj = reach[xj];
+        const float rr = std::max(cd, dd[j]);
+        if (rj == NEG || rj > rr) {
+            reach[xj] = rr;
+            seeds.emplace(xj, rr);
+        }
+    }
+}
+
+// The same scan 16 neighbours at a time: gathered reach values and the two
+// tests in AVX-512, then the flagged neighbours in order through the scalar
+// test against the current reach (so repeated ids behave as in the scalar
+// loop) -- the pushes, and so the heap, are those of expand_scalar.
+__attribute__((target("avx512f"))) static void expand_avx512(long long j0, long long j1, float cd, const int* idx,
+                                                             const float* dd, float* reach, ReachHeap& seeds) {
+    const float NEG = -2147483648.0f;
+    const __m512 vcd = _mm512_set1_ps(cd), vneg = _mm512_set1_ps(NEG);
+    long long j = j0;
+    for (; j + 16 <= j1; j += 16) {
+        const __m512i vi = _mm512_loadu_si512((const void*)(idx + j));
+        const __m512 rj = _mm512_i32gather_ps(vi, reach, 4);
+        const __m512 vd = _mm512_loadu_ps(dd + j);
+        const __m512 rr = _mm512_mask_blend_ps(_mm512_cmp_ps_mask(vcd, vd, _CMP_LT_OQ), vcd, vd);  // std::max(cd, d)
+        unsigned m = (unsigned)(_mm512_cmp_ps_mask(rj, vneg, _CMP_EQ_OQ) | _mm512_cmp_ps_mask(rj, rr, _CMP_GT_OQ));
+        while (m) {
+            const int b = __builtin_ctz(m);
+            m &= m - 1;
+            const int xj = idx[j + b];
+            const float r = std::max(cd, dd[j + b]);
+            if (reach[xj] == NEG || reach[xj] > r) {
+                reach[xj] = r;
+                seeds.emplace(xj, r);
+            }
+        }
+    }
+    expand_scalar(j, j1, cd, idx, dd, reach, seeds);
+}
+
 static void optics_walk(long long n, const long long* ptr, const int* idx, const float* dd, const uint8_t* core,
                         const float* cdist, int* ordering_out) {
     const float NEG = -2147483648.0f;  // src/Header.h:20
     // one array for the reach and the processed flag: a processed point holds
     // -inf, which no update test (reach == NEG or reach > rr) accepts
     const float DONE = -INFINITY;
+    static const bool wide = __builtin_cpu_supports("avx512f");
     std::vector<float> reach(n, NEG);
     long long k = 0;
     auto expand = [&](int xi, ReachHeap& seeds) {
-        const float cd = cdist[xi];
-        for (long long j = ptr[xi]; j < ptr[xi + 1]; ++j) {
-            const int xj = idx[j];
-            const float rj = reach[xj];
-            if (rj == DONE) continue;
-            const float rr = std::max(cd, dd[j]);
-            if (rj == NEG || rj > rr) {
-                reach[xj] = rr;
-                seeds.emplace(xj, rr);
-            }
-        }
+        if (wide) expand_avx512(ptr[xi], ptr[xi + 1], cdist[xi], idx, dd, reach.data(), seeds);
+        else expand_scalar(ptr[xi], ptr[xi + 1], cdist[xi], idx, dd, reach.data(), seeds);
     };
     for (long long xi = 0; xi < n; ++xi) {
         if (reach[xi] == DONE) continue;
