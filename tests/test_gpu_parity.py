@@ -215,6 +215,28 @@ def test_end_to_end_matches_reference_arithmetic_fit(ctx, case, precision):
     assert np.array_equal(lab, olab) and nc == onc
 
 
+@pytest.mark.parametrize("n,d,K,M,n_proj,precision", [
+    (200, 16, 3, 200, 64, S.FP32),   # topM = n: every list holds every point (D = 64)
+    (200, 16, 3, 200, 64, S.BF16),
+    (64, 40, 1, 64, 128, S.FP32),    # one slot pair per point, two mask words
+])
+def test_all_points_in_every_list_is_exact_dbscan(ctx, n, d, K, M, n_proj, precision):
+    """topM = n makes every point a candidate of every other: the labels are
+    exact DBSCAN's (sklearn pins the core points), the pair tables are
+    longer than the sorted segments (the scan path), and every point is a
+    member of every list."""
+    from sklearn.cluster import DBSCAN
+    X, _ = datagen.mixture(n, d, k=3, sigma=0.3, seed=401 + n)
+    eps, mp = 0.3, 5
+    print(f"all points candidates n={n} d={d} K={K} M={M} n_proj={n_proj} precision={precision}", flush=True)
+    p, bits, R, tk, tm, Xt, st = full_check(ctx, X, S.COSINE, eps, mp, K=K, M=M, n_proj=n_proj, precision=precision)
+    core, _ = ctx.find_core(eps, mp)
+    sk = DBSCAN(eps=eps, min_samples=mp, metric="cosine").fit(Xt.astype(np.float64))
+    sk_core = np.zeros(n, np.uint8)
+    sk_core[sk.core_sample_indices_] = 1
+    assert np.array_equal(core, sk_core)
+
+
 def test_duplicate_points_replay_tied_rows(ctx):
     X, _ = datagen.mixture(3000, 64, k=4, seed=21)
     X[1000:1500] = X[:500]          # exact duplicates: tied projections in every row
