@@ -807,6 +807,7 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     da.hi = (double)eps + band;
     cosine_cuts(da.lo, da.hi, &da.s_hit, &da.s_near);
     da.magicS = ((1ull << 40) + (unsigned long long)S - 1) / (unsigned long long)S;
+    da.n_ent = n * S;
 
     HIPCHK(hipEventRecord(c->ev[3], st));
     // queued pairs are counted in 64 bits (a pass may meet more than 2^31)

@@ -766,6 +766,153 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu1(DistArgs a) {
     }
 }
 
+// Rows of 257 to 1024 dims: the candidates of a tile are split into parts of
+// at most 64 (grid y), one candidate per lane, whose rows fill the LDS; each
+// wave keeps QV query rows in registers (element k in lane k % 64, register
+// k / 64, the next set already loading) and broadcasts element k with a
+// readlane.  Parts share the hit words of an entry, so E is cleared first and
+// the hits are OR-ed in.
+template <int WAVES, int QV, int KPL, int METRIC>
+__global__ __launch_bounds__(WAVES * 64) void k_dist_valu2(DistArgs a, int Mc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int M = a.M, d = a.d;
+    const int RS = d | 1;
+    const int c0 = blockIdx.y * Mc;
+    const int mc = min(Mc, M - c0);
+    float* cand = (float*)smem;
+    int* col_lds = (int*)(cand + (size_t)Mc * RS);
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const int l = a.tile_list[blockIdx.x];
+    const int beg = a.tile_beg[blockIdx.x], end = a.tile_end[blockIdx.x];
+    const int* Lm = a.lists + (size_t)l * M + c0;
+    const double lo = a.lo, hi = a.hi;
+
+    for (int i = tid; i < mc; i += WAVES * 64) col_lds[i] = 0;
+    for (int v = tid; v < mc * d; v += WAVES * 64) {
+        const int row = v / d, k = v - row * d;
+        cand[(size_t)row * RS + k] = a.X32[(size_t)Lm[row] * a.ldx + k];
+    }
+    __syncthreads();
+    const bool colok = lane < mc;
+    const int ci = c0 + lane;
+    const float* cr = cand + (size_t)(colok ? lane : 0) * RS;
+
+    const int per_set = WAVES * QV;
+    const int nsets = (end - beg + per_set - 1) / per_set;
+    float pre[QV][KPL];
+    uint32_t entn[QV];
+    bool validn[QV];
+    auto fetch = [&](int set) {
+#pragma unroll
+        for (int q = 0; q < QV; ++q) {
+            const int rr = beg + set * per_set + w * QV + q;
+            validn[q] = rr < end;
+            entn[q] = a.grp[validn[q] ? rr : beg];
+            const long long pa = (long long)(entn[q] / (uint32_t)a.S);
+            const float* xr = a.X32 + (size_t)pa * a.ldx;
+#pragma unroll
+            for (int t = 0; t < KPL; ++t) {
+                const int k = lane + 64 * t;
+                pre[q][t] = k < d ? xr[k] : 0.0f;
+            }
+        }
+    };
+    if (nsets > 0) fetch(0);
+    for (int set = 0; set < nsets; ++set) {
+        float cur[QV][KPL];
+        uint32_t ent[QV];
+        bool valid[QV];
+#pragma unroll
+        for (int q = 0; q < QV; ++q) {
+            ent[q] = entn[q];
+            valid[q] = validn[q];
+#pragma unroll
+            for (int t = 0; t < KPL; ++t) cur[q][t] = pre[q][t];
+        }
+        if (set + 1 < nsets) fetch(set + 1);
+        float acc[QV];
+#pragma unroll
+        for (int q = 0; q < QV; ++q) acc[q] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < KPL; ++t) {
+            const int kn = min(64, d - 64 * t);
+            for (int kk = 0; kk < kn; ++kk) {
+                const float y = cr[64 * t + kk];
+#pragma unroll
+                for (int q = 0; q < QV; ++q) {
+                    const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur[q][t]), kk));
+                    acc[q] = acc[q] + valu_term<METRIC>(x, y);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QV; ++q) {
+            const float s = acc[q];
+            float dist;
+            if constexpr (METRIC == kL2) dist = sqrtf(s);
+            else if constexpr (METRIC == kL1) dist = s;
+            else if constexpr (METRIC == kChi2) dist = 1.0f - 2.0f * s;
+            else if constexpr (METRIC == kJS) dist = 1.0f - s * 0.5f;
+            else dist = 1.0f - s;
+            const uint32_t fw = (valid[q] && colok) ? a.F[(size_t)ent[q] * a.W + (ci >> 5)] : 0u;
+            const bool act = valid[q] && colok && ((fw >> (ci & 31)) & 1u);
+            const double dv = (double)dist;
+            const bool hit = act && dv < lo;
+            const bool unc = act && !(dv < lo) && !(dv > hi);
+            const unsigned long long bh = __ballot(hit);
+            if (bh && lane == 0) {
+                // bit (c - c0) of bh is candidate c; OR it into word c >> 5
+                for (int wi = c0 >> 5; wi <= (c0 + mc - 1) >> 5; ++wi) {
+                    const int sh = 32 * wi - c0;
+                    const uint32_t bits = sh >= 0 ? (uint32_t)(bh >> sh) : (uint32_t)(bh << -sh);
+                    if (bits) atomicOr(&a.E[(size_t)ent[q] * a.W + wi], bits);
+                }
+            }
+            if (hit) atomicAdd(&col_lds[lane], 1);
+            if (unc) {
+                const unsigned long long pos = atomicAdd(a.qcount, 1ull);
+                if (pos < (unsigned long long)a.qcap) a.queue[pos] = queue_entry(ent[q], l, ci);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < mc; i += WAVES * 64) {
+        const int v = col_lds[i];
+        if (v) atomicAdd(&a.colsum[(size_t)l * M + c0 + i], v);
+    }
+}
+
+template <int KPL, int METRIC>
+static int launch_valu2(const DistArgs& a, int ntiles, hipStream_t st) {
+    constexpr int WAVES = 8, QV = 4;
+    const int RS = a.d | 1;
+    const int cap = std::min(64, (int)((kLdsBytes - 64 * 4) / (4 * RS)));
+    if (cap < 1) return -2;
+    const int parts = (a.M + cap - 1) / cap;
+    const int Mc = (a.M + parts - 1) / parts;
+    const size_t sm = (size_t)Mc * RS * 4 + (size_t)Mc * 4;
+    if (sm > (size_t)kLdsBytes) return -2;
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_valu2<WAVES, QV, KPL, METRIC>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return (int)e;
+    // the parts OR their hits into cleared words
+    e = hipMemsetAsync(a.E, 0, (size_t)a.n_ent * a.W * 4, st);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL((k_dist_valu2<WAVES, QV, KPL, METRIC>), dim3(ntiles, parts), dim3(WAVES * 64), sm, st, a, Mc);
+    return (int)hipGetLastError();
+}
+
+template <int KPL>
+static int launch_valu2_m(const DistArgs& a, int ntiles, hipStream_t st) {
+    switch (a.metric) {
+        case kL2: return launch_valu2<KPL, kL2>(a, ntiles, st);
+        case kL1: return launch_valu2<KPL, kL1>(a, ntiles, st);
+        case kChi2: return launch_valu2<KPL, kChi2>(a, ntiles, st);
+        case kJS: return launch_valu2<KPL, kJS>(a, ntiles, st);
+        default: return launch_valu2<KPL, kCosine>(a, ntiles, st);
+    }
+}
+
 template <int CM, int METRIC>
 static int launch_valu1(const DistArgs& a, int ntiles, hipStream_t st) {
     constexpr int WAVES = 8, QV = 8;
@@ -814,6 +961,11 @@ int launch_dist_valu(const DistArgs& a, int ntiles, hipStream_t st) {
             case 4: r = launch_valu1_m<4>(a, ntiles, st); break;
         }
         if (r != -2) return r;  // -2: the tile does not fit LDS, take the chunked tiles
+    } else if (a.d <= 1024 && a.n_ent > 0) {
+        int r = a.d <= 512 ? launch_valu2_m<8>(a, ntiles, st)
+              : a.d <= 768 ? launch_valu2_m<12>(a, ntiles, st)
+                           : launch_valu2_m<16>(a, ntiles, st);
+        if (r != -2) return r;
     }
     switch (CM) {
         case 1: return launch_valu_cm<1>(a, ntiles, st);

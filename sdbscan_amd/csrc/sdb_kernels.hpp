@@ -72,6 +72,7 @@ struct DistArgs {
     // fl(1 - s), see cosine_cuts)
     float s_hit, s_near;
     unsigned long long magicS;  // ceil(2^40 / S): a = (ent * magicS) >> 40
+    long long n_ent;            // entries (rows of E and F)
 };
 
 // index
