@@ -62,7 +62,9 @@ typedef struct sdb_params {
     float ker_sigma;
     float ker_intervalSampling;
     int fhtDim;            /* 2^ceil(log2(max(n_proj, ker_n_features))) */
-    int precision;         /* enum sdb_precision: distance tiles */
+    int precision;         /* enum sdb_precision: distance tiles.  SDB_BF16 for Cosine and
+                              Chi2 (Chi2: bounds on the BF16 square-root rows, the rest
+                              rechecked); the decisions equal SDB_FP32's */
 } sdb_params;
 
 typedef struct sdb_stats {
@@ -113,7 +115,10 @@ int sdb_load_device(sdb_ctx* ctx, const float* X_dev, long long n, int d);
 int sdb_build_index(sdb_ctx* ctx, const uint8_t* hd3_bits, const float* R);
 
 /* Neighbourhood counts and core flags for (eps, minPts); core_out (n bytes)
- * and degree_out (n ints) are optional. */
+ * and degree_out (n ints) are optional.  At an unchanged eps only the core
+ * test runs again.  Pairs inside the error band of eps are re-decided in
+ * FP32/FP64; when more than 2^30 are queued (SDB_QUEUE_MAX lowers the limit)
+ * the pass is redone on the FP32 tiles. */
 int sdb_find_core(sdb_ctx* ctx, float eps, int minPts, uint8_t* core_out, int* degree_out);
 
 /* Cluster labels after sdb_find_core; n_clusters as the reference reports
