@@ -152,15 +152,17 @@ __global__ void k_uf_init(ClusterArgs c) {
 // connected through the point.  The point is joined to the first core hit;
 // the later ones are recorded as star edges from the first (adj[l][first])
 // for the list-local unions of k_uf_lists.
-__global__ void k_uf_points(ClusterArgs c) {
-    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= c.n || !c.core[a]) return;
+// The star edges of core point a: per slot, the first core hit (after
+// masking by corehub) is joined to a, and the other core hits of the slot are
+// recorded as adjacent to that first one (adj, joined by k_uf_lists).
+// words(s, m) yields the E words of slot s.
+template <typename Words>
+__device__ __forceinline__ void uf_point(const ClusterArgs& c, long long a, Words words) {
     int ra = uf_find(c.parent, (int)a);
     for (int s = 0; s < c.S; ++s) {
         const int l = slot_list_c(c.topK, a, s, c.K, c.P);
-        const size_t t = (size_t)a * c.S + s;
         uint32_t m[kMaxW];
-        entry_words(c.E, t, c.W, m);
+        words(s, m);
         int first = -1;
 #pragma unroll
         for (int w = 0; w < kMaxW; ++w) {
@@ -181,6 +183,34 @@ __global__ void k_uf_points(ClusterArgs c) {
             ra = uf_find(c.parent, (int)a);
         }
     }
+}
+
+__global__ void k_uf_points(ClusterArgs c) {
+    const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= c.n || !c.core[a]) return;
+    uf_point(c, a, [&](int s, uint32_t (&m)[kMaxW]) { entry_words(c.E, (size_t)a * c.S + s, c.W, m); });
+}
+
+// W = 4: the E rows of the workgroup's 128 points are staged through LDS with
+// coalesced 16-byte loads, instead of each thread striding through its own
+// 16 S bytes in global memory across the slot loop.
+__global__ __launch_bounds__(128) void k_uf_points_staged(ClusterArgs c) {
+    extern __shared__ uint4 erow[];
+    const int S = c.S;
+    const long long a0 = (long long)blockIdx.x * 128;
+    const int npt = (int)min((long long)128, c.n - a0);
+    const uint4* Eb = (const uint4*)c.E + (size_t)a0 * S;
+    for (int v = threadIdx.x; v < npt * S; v += 128) erow[v] = Eb[v];
+    __syncthreads();
+    const long long a = a0 + threadIdx.x;
+    if ((int)threadIdx.x >= npt || !c.core[a]) return;
+    const uint4* mine = erow + (size_t)threadIdx.x * S;
+    uf_point(c, a, [&](int s, uint32_t (&m)[kMaxW]) {
+        const uint4 v = mine[s];
+        m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
+#pragma unroll
+        for (int w = 4; w < kMaxW; ++w) m[w] = 0u;
+    });
 }
 
 __global__ void k_uf_lists(ClusterArgs c) {
@@ -283,7 +313,12 @@ int launch_uf_init(const ClusterArgs& c, hipStream_t st) {
     return (int)hipGetLastError();
 }
 int launch_uf_points(const ClusterArgs& c, hipStream_t st) {
-    hipLaunchKernelGGL(k_uf_points, dim3(nblk(c.n, 256)), dim3(256), 0, st, c);
+    const size_t sm = (size_t)128 * c.S * 16;
+    if (c.W == 4 && sm <= 64 * 1024) {
+        hipLaunchKernelGGL(k_uf_points_staged, dim3(nblk(c.n, 128)), dim3(128), sm, st, c);
+    } else {
+        hipLaunchKernelGGL(k_uf_points, dim3(nblk(c.n, 256)), dim3(256), 0, st, c);
+    }
     return (int)hipGetLastError();
 }
 int launch_uf_lists(const ClusterArgs& c, hipStream_t st) {
