@@ -87,3 +87,23 @@ def test_struct_layouts_match_ctypes(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for f, _ in cls._fields_:
             assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
+
+
+def test_cpp_dropin_class_builds_and_configures(tmp_path):
+    """A C++ caller of the drop-in class (INTEGRATION: one changed line per
+    fit call) compiles against sDbscan.h, links sDbscan.cpp and the C ABI
+    library, and runs the host-side configuration (no device call before a
+    fit)."""
+    import subprocess
+    src = tmp_path / "caller.cpp"
+    src.write_text('#include <cstdio>\n#include "sDbscan.h"\n'
+                   'int main() { sDbscan dbs(100, 8); dbs.set_params(64, 5, 20, "Cosine");\n'
+                   '  dbs.set_precision("BF16"); std::printf("configured\\n"); return 0; }\n')
+    exe = tmp_path / "caller"
+    lib_dir = os.path.dirname(sdbscan_amd.LIB_PATH)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(ROOT, "sdbscan_amd", "csrc"), str(src),
+                    os.path.join(ROOT, "sdbscan_amd", "csrc", "sDbscan.cpp"), "-L", lib_dir, "-lsdbscan_amd",
+                    "-Wl,-rpath," + lib_dir, "-fopenmp", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert out.strip() == "configured"
