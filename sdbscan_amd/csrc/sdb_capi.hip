@@ -91,7 +91,7 @@ struct sdb_ctx {
     // index
     DevBuf bits, R, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
         cand_idx, topM, flags, lists, is_hub, hub_flags, hub_excl, hub_id, hub_cnt, hub_ptr,
-        hub_cur, hub_mem, inv, pos_hub, pcnt, poff, pairs, psorted, F, wgcnt, wgoff, grp_ptr, grp, scan_tmp, total, err_flag,
+        hub_cur, hub_mem, inv, pos_hub, pcnt, poff, pairs, pseg, F, wgcnt, wgoff, grp_ptr, grp, scan_tmp, total, err_flag,
         row_slot, row_buf;
     bool indexed = false;
     int H = 0, L2 = 0, W = 0, S = 0;
@@ -246,7 +246,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->tie_count, &c->cmax, &c->cmin, &c->thi, &c->tlo, &c->cand_cnt, &c->cand_key,
                      &c->cand_idx, &c->topM, &c->flags, &c->lists, &c->is_hub, &c->hub_flags,
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
-                     &c->inv, &c->pos_hub, &c->pcnt, &c->poff, &c->pairs, &c->psorted, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
+                     &c->inv, &c->pos_hub, &c->pcnt, &c->poff, &c->pairs, &c->pseg, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
                      &c->scan_tmp, &c->total, &c->err_flag, &c->row_slot, &c->row_buf, &c->E,
                      &c->colsum, &c->queue, &c->qcount, &c->tiles, &c->degree, &c->core,
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
@@ -578,8 +578,8 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     LCHK(launch_pair_fill(P_<int>(c->pos_hub), P_<int>(c->hub_ptr), P_<int>(c->hub_mem), L2, M, P_<int>(c->poff),
                           P_<uint32_t>(c->pairs), st));
     ENSURE(c->F, (size_t)n * S * W * 4);
-    ENSURE(c->psorted, (size_t)L2);
-    LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, W, P_<int>(c->poff), P_<uint32_t>(c->pairs), P_<uint8_t>(c->psorted),
+    ENSURE(c->pseg, (size_t)L2 * 8);
+    LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, W, P_<int>(c->poff), P_<uint32_t>(c->pairs), P_<int2>(c->pseg),
                            P_<int>(c->hub_id), P_<int>(c->hub_ptr), P_<int>(c->hub_mem), P_<uint32_t>(c->F), st));
     // group the (point, slot) entries by (query block, list): the distance
     // tiles then sweep the points block by block, and a 192 MB block of query

@@ -1041,17 +1041,17 @@ __global__ void k_pair_fill(const int* __restrict__ pos_hub, const int* __restri
 // 16-byte store each when W = 4).
 // Each list's pair segment sorted by (other list, position) in LDS, one
 // workgroup per list, so that k_first_mask can look up one other list by
-// binary search; segments longer than kPairSortCap stay unsorted (flag 0) and
-// are scanned.
+// binary search; segments longer than kPairSortCap stay unsorted and are
+// scanned.  seg[l] = (first pair, one past the last | 2^31 when sorted).
 constexpr int kPairSortCap = 4096;
 __global__ __launch_bounds__(256) void k_pair_sort(const int* __restrict__ poff, int M, uint32_t* __restrict__ pairs,
-                                                   uint8_t* __restrict__ sorted) {
+                                                   int2* __restrict__ seg) {
     __shared__ uint32_t keys[kPairSortCap];
     const int l = blockIdx.x;
     const int q0 = poff[(size_t)l * M], q1 = poff[(size_t)(l + 1) * M];
     const int len = q1 - q0;
     if (len > kPairSortCap) {
-        if (threadIdx.x == 0) sorted[l] = 0;
+        if (threadIdx.x == 0) seg[l] = make_int2(q0, q1);
         return;
     }
     int np = 1;
@@ -1070,13 +1070,13 @@ __global__ __launch_bounds__(256) void k_pair_sort(const int* __restrict__ poff,
             __syncthreads();
         }
     for (int i = threadIdx.x; i < len; i += blockDim.x) pairs[q0 + i] = keys[i];
-    if (threadIdx.x == 0) sorted[l] = 1;
+    if (threadIdx.x == 0) seg[l] = make_int2(q0, (int)((uint32_t)q1 | 0x80000000u));
 }
 
 __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK, long long n, int P, int K,
                                                     int M, int W, int ppb, const int* __restrict__ poff,
                                                     const uint32_t* __restrict__ pairs,
-                                                    const uint8_t* __restrict__ psorted,
+                                                    const int2* __restrict__ seg,
                                                     const int* __restrict__ hub_id,
                                                     const int* __restrict__ hub_ptr,
                                                     const int* __restrict__ hub_mem,
@@ -1101,8 +1101,9 @@ __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK
     if (s > 0) {
         // positions of l whose point is also in another list L: a duplicate
         // when L is one of the earlier slots' lists
-        const int q0 = poff[(size_t)l * M], q1 = poff[(size_t)(l + 1) * M];
-        if (q1 - q0 > 32 && psorted[l]) {
+        const int2 sg = seg[l];
+        const int q0 = sg.x, q1 = sg.y & 0x7fffffff;
+        if (q1 - q0 > 32 && sg.y < 0) {
             // long segment, sorted by (L, position): one lower bound per
             // earlier slot's list
             for (int s2 = 0; s2 < s; ++s2) {
@@ -1356,14 +1357,14 @@ int launch_pair_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem,
 }
 
 int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const int* poff,
-                      uint32_t* pairs, uint8_t* psorted, const int* hub_id, const int* hub_ptr, const int* hub_mem,
+                      uint32_t* pairs, int2* seg, const int* hub_id, const int* hub_ptr, const int* hub_mem,
                       uint32_t* F, hipStream_t st) {
     const int S = 2 * K;
-    hipLaunchKernelGGL(k_pair_sort, dim3(2 * P), dim3(256), 0, st, poff, M, pairs, psorted);
+    hipLaunchKernelGGL(k_pair_sort, dim3(2 * P), dim3(256), 0, st, poff, M, pairs, seg);
     const int ppb = std::max(1, 512 / S);
     const long long blocks = (n + ppb - 1) / ppb;
     hipLaunchKernelGGL(k_first_mask, dim3((unsigned)blocks), dim3(ppb * S), (size_t)ppb * S * 4, st, topK, n, P, K, M,
-                       W, ppb, poff, pairs, psorted, hub_id, hub_ptr, hub_mem, F);
+                       W, ppb, poff, pairs, seg, hub_id, hub_ptr, hub_mem, F);
     return (int)hipGetLastError();
 }
 

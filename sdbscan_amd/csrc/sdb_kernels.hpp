@@ -100,7 +100,7 @@ int launch_pair_count(const int* lists, const int* hub_id, const int* hub_ptr, i
 int launch_pair_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem, int L2, int M, const int* poff,
                      uint32_t* pairs, hipStream_t st);
 int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const int* poff,
-                      uint32_t* pairs, uint8_t* psorted, const int* hub_id, const int* hub_ptr, const int* hub_mem,
+                      uint32_t* pairs, int2* seg, const int* hub_id, const int* hub_ptr, const int* hub_mem,
                       uint32_t* F, hipStream_t st);
 int launch_group_count(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
                        int Gx, int* wgcnt, hipStream_t st);
