@@ -79,7 +79,9 @@ def run_path(ctx, bits, steps, warmup, tag, wl):
         t0 = time.perf_counter()
         ctx.build_index(bits)
         ctx.find_core(wl["eps"], wl["minPts"], want=False)
-        lab, nc = ctx.form_clusters(want=(i == warmup + steps - 1))
+        # labels to the host on the last step, and on the untimed warmup steps
+        # so that the first device-to-host copy of that size is not timed
+        lab, nc = ctx.form_clusters(want=(i == warmup + steps - 1 or i < warmup))
         t1 = time.perf_counter()
         st = ctx.stats()
         log(f"[{tag}] step {i} {'warmup' if i < warmup else 'timed'} {1e3 * (t1 - t0):.1f} ms "
@@ -155,7 +157,7 @@ def l2_bench(args):
         t1 = time.perf_counter()
         ctx.build_index(bits, R)
         ctx.find_core(eps, 50, want=False)
-        lab, nc = ctx.form_clusters(want=(i == args.warmup + args.steps - 1))
+        lab, nc = ctx.form_clusters(want=(i == args.warmup + args.steps - 1 or i < args.warmup))
         t2 = time.perf_counter()
         st = ctx.stats()
         log(f"[l2] step {i} {1e3 * (t2 - t1):.1f} ms (index {st['t_index_ms']:.1f} dist {st['t_dist_ms']:.1f} "
