@@ -123,7 +123,18 @@ def cpu_baseline(ctx, bits, wl, n_default):
     log(f"[cpu] n={n_s} threads={thr} {dt:.2f} s phases(ms)={t.tolist()} clusters={nc}")
     return {"value": n_s / dt, "unit": "points/sec", "cores": thr, "kind": "port",
             "sample": f"first {n_s} of the {N} points, same parameters, full fit (transform, index, "
-                      f"core points, clusters) with {thr} OpenMP threads; {dt:.2f} s"}
+                      f"core points, clusters) with {thr} OpenMP threads; {dt:.2f} s",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def l2_bench(args):
@@ -182,7 +193,7 @@ def l2_bench(args):
         log(f"[cpu] n={n_s} threads={thr} {dt:.2f} s phases(ms)={tt.tolist()}")
         cpu = {"value": n_s / dt, "unit": "points/sec", "cores": thr, "kind": "port",
                "sample": f"first {n_s} of the {n} points, same parameters, full fit with {thr} OpenMP threads; "
-                         f"{dt:.2f} s"}
+                         f"{dt:.2f} s", "cpu_model": cpu_model(), "nproc": os.cpu_count()}
     st = ctx.stats()
     print(json.dumps({
         "metric": "points/sec, 1.8M x 51 L2 (RFF 1024), FP32", "value": value, "unit": "points/sec",
