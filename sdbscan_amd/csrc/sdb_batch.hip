@@ -94,10 +94,10 @@ void run_job(sdb_ctx* c, sdb_batch_job& j, int autotune, TuneTable& tune) {
         }
     }
     if (tune_here) {
-        static const int cand[] = {256, 1024, 4096, 16384};
+        static const int cand[] = {256, 1024, 2048, 4096, 16384};
         int best = 0;
         double ms = 0.0;
-        r = sdb_tune_tile_rows(c, eps_max, cand, 4, &best, &ms);
+        r = sdb_tune_tile_rows(c, eps_max, cand, (int)(sizeof(cand) / sizeof(cand[0])), &best, &ms);
         {
             std::lock_guard<std::mutex> lk(tune.mu);
             tune.rows[key] = r ? -1 : best;

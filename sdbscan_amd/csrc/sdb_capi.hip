@@ -757,7 +757,7 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
         if (steps <= 0) return fail(c, SDB_EUNSUPPORTED, "topM too large for the LDS tile");
         da.kc = std::min(da.dk, steps * SE);
         da.nchunk = (da.dk + da.kc - 1) / da.kc;
-        QB = da.nchunk == 1 ? 4096 : 256;
+        QB = da.nchunk == 1 ? 2048 : 256;  // single-chunk tiles: 1024-3072 rows measured best (8M x 784 BF16)
         if (const char* e = std::getenv("SDB_TILE_ROWS")) QB = std::max(256, std::atoi(e));
         if (c->tile_rows > 0) QB = c->tile_rows;
         QB = std::min(QB, 16 * 8 * 255);  // per-lane byte counters of the single-chunk tiles
