@@ -1019,8 +1019,10 @@ int sdb_form_clusters(sdb_ctx* c, int* labels_out, int* n_clusters_out) {
         HIPCHK(hipMemsetAsync(c->adj.p, 0, (size_t)L2 * M * W * 4, st));
         LCHK(launch_corehub(ca, st));
         LCHK(launch_uf_init(ca, st));
-        LCHK(launch_uf_points(ca, st));
-        LCHK(launch_uf_lists(ca, st));
+        if (c->stats.n_core > 0) {  // without core points there is nothing to join
+            LCHK(launch_uf_points(ca, st));
+            LCHK(launch_uf_lists(ca, st));
+        }
         LCHK(launch_uf_compress(ca, st));
         LCHK(launch_root_flags(ca, P_<int>(c->rootf), st));
         LCHK(scan_exclusive_i32(P_<int>(c->rootf), P_<int>(c->rank), n, P_<int>(c->total), c->scan_tmp.p, c->scan_tmp.bytes, st));
