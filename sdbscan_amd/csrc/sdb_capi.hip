@@ -284,6 +284,12 @@ int sdb_set_params(sdb_ctx* c, const sdb_params* p) {
         return fail(c, SDB_EUNSUPPORTED, "BF16 distance tiles are implemented for Cosine and Chi2 only");
     if ((long long)q.n_points * 2 * q.topK >= (1ll << 32)) return fail(c, SDB_EUNSUPPORTED, "n_points * 2 * topK must stay below 2^32");
     if (2 * q.n_proj >= (1 << 24)) return fail(c, SDB_EUNSUPPORTED, "n_proj too large");
+    // an input loaded under other dimensions does not fit these parameters:
+    // it has to be loaded again (its buffer is kept for reuse)
+    if (c->X && (q.n_points != c->n || q.n_features != c->d)) {
+        c->X = nullptr;
+        c->transformed = c->sng = c->clustered = false;
+    }
     c->prm = q;
     c->have_params = true;
     c->indexed = false;

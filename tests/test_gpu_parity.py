@@ -718,3 +718,17 @@ def test_error_paths(ctx):
     with pytest.raises(S.SdbError) as e:
         ctx.set_params(500, 20, n_proj=64, topK=100, topM=20)
     assert e.value.code == 1
+    # parameters of other dimensions drop the loaded input (SDB_ESTATE until
+    # the next load); the same dimensions keep it
+    run_index(ctx, X, metric=S.COSINE, K=4, M=20, n_proj=64)
+    p = ctx.set_params(400, 20, n_proj=64, topK=4, topM=20)
+    bits = O.hd3_bits(3 * p.fhtDim, 42)
+    with pytest.raises(S.SdbError) as e:
+        ctx.build_index(bits)
+    assert e.value.code == 6
+    ctx.load(X[:400])
+    ctx.build_index(bits)
+    ctx.set_params(400, 20, n_proj=64, topK=4, topM=20)
+    ctx.build_index(bits)
+    tk, _ = ctx.get_index()
+    assert tk.shape[0] == 400
