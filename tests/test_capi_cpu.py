@@ -107,3 +107,34 @@ def test_cpp_dropin_class_builds_and_configures(tmp_path):
                     "-Wl,-rpath," + lib_dir, "-fopenmp", "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     assert out.strip() == "configured"
+
+
+def test_product_path_fails_loudly_without_device_or_library():
+    """No CPU fallback: without a visible HIP device the context refuses to
+    start (SDB_EHIP), and without the built library the package raises."""
+    import subprocess
+    import sys
+    code = (
+        "import sdbscan_amd as S\n"
+        "try:\n"
+        "    S.Context()\n"
+        "    print('created')\n"
+        "except S.SdbError as e:\n"
+        "    print('refused', e.code)\n"
+    )
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True,
+                         text=True, timeout=120)
+    assert out.stdout.strip() == "refused 2", out.stdout + out.stderr
+    code = (
+        "import sdbscan_amd as S\n"
+        "S.LIB_PATH = '/nonexistent/libsdbscan_amd.so'\n"
+        "try:\n"
+        "    S.capi()\n"
+        "    print('loaded')\n"
+        "except RuntimeError as e:\n"
+        "    print('missing' if 'not built' in str(e) else str(e))\n"
+    )
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True,
+                         timeout=120)
+    assert out.stdout.strip() == "missing", out.stdout + out.stderr
