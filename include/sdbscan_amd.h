@@ -101,6 +101,9 @@ int sdb_create(int device, sdb_ctx** out);
 void sdb_destroy(sdb_ctx* ctx);
 const char* sdb_last_error(const sdb_ctx* ctx);
 
+/* Validates and stores the parameters; invalidates the index.  Parameters of
+ * other n_points / n_features than the loaded input also drop that input
+ * (SDB_ESTATE until the next load); the same dimensions keep it. */
 int sdb_set_params(sdb_ctx* ctx, const sdb_params* p);
 
 /* Input points: host buffer (copied) or device buffer (borrowed; must stay
