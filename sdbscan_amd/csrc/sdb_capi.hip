@@ -1121,6 +1121,8 @@ static int build_neighbors(sdb_ctx* c) {
     std::vector<long long> ptr(n + 1, 0);
     for (long long a = 0; a < n; ++a) ptr[a + 1] = ptr[a] + deg[a];
     const long long tot = ptr[n];
+    // the segmented sorts count entries in 32 bits
+    if (tot >= (1ll << 31)) return fail(c, SDB_EUNSUPPORTED, "%lld neighbourhood entries", tot);
     HIPCHK(hipMemcpyAsync(c->nb_ptr.p, ptr.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(c->nb_cur.p, 0, (size_t)n * 4, st));
     ENSURE(c->nb_idx, (size_t)std::max<long long>(tot, 1) * 4);
