@@ -258,6 +258,11 @@ def main():
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eps", type=float, default=None,
+                    help="eps of the step instead of the workload's (e.g. inside the within-cluster bulk, "
+                         "where the recheck band is populated)")
+    ap.add_argument("--sigma", type=float, default=None,
+                    help="noise of the cosine mixture generator instead of 0.1")
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["l2"], default="cosine",
                     help="cosine: the 8M x 784 cosine step; chi2: the 8M x 784 Chi2 step; "
                          "l2: the 1.8M x 51 L2 (RFF) step")
@@ -290,14 +295,19 @@ def main():
 
     import sdbscan_amd as S
 
-    wl = WORKLOADS[args.workload]
+    wl = dict(WORKLOADS[args.workload])
+    if args.eps is not None:
+        wl["eps"] = args.eps
+    sigma = 0.1 if args.sigma is None else args.sigma
+    if args.sigma is not None:
+        wl["data"] = wl["data"].replace("mixture generated", f"mixture (noise sigma {sigma}) generated")
     n = args.n
     ctx = S.Context(0)
     prm = dict(n_proj=P, topK=K, topM=M, metric=wl["metric"], ker_n_features=wl["ker"])
     p = ctx.set_params(n, D_FEAT, precision=S.BF16, **prm)
     bits = S.hd3_bits(3 * p.fhtDim, SEED_RNG)
     t0 = time.perf_counter()
-    ctx.generate(n, D_FEAT, wl["seed"], 0.1, kind=wl["gen"])
+    ctx.generate(n, D_FEAT, wl["seed"], sigma, kind=wl["gen"])
     log(f"generated {n} x {D_FEAT} ({args.workload}) on device in {time.perf_counter() - t0:.1f} s")
 
     times, dist_ms, phase, lab16, nc16 = run_path(ctx, bits, args.steps, args.warmup, "bf16", wl)

@@ -140,6 +140,12 @@ def heap_select(vals, k):
     return c, f
 
 
+def dist32(x, y, metric):
+    x = _c(x, np.float32)
+    y = _c(y, np.float32)
+    return lib().orc_dist32(x.ctypes.data, y.ctypes.data, len(x), metric)
+
+
 def dist64(x, y, metric):
     x = _c(x, np.float32)
     y = _c(y, np.float32)

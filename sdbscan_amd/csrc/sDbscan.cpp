@@ -313,7 +313,7 @@ void sDbscan::test_sDbscan(const float* X, float eps, float rangeEps, int n_test
     sdb_get_stats(ctx, &s);
     std::cout << "Build index time = " << (long long)(s.t_transform_ms + s.t_index_ms) << "[ms]" << std::endl;
     for (int i = 0; i < n_tests; ++i) {
-        const float e = eps + 1.0f * i * rangeEps;
+        const float e = sweep_eps(eps, rangeEps, i);
         std::cout << "Eps: " << e << std::endl;
         std::vector<uint8_t> core(clusterNoise ? n_points : 0);
         check(sdb_find_core(ctx, e, minPts, clusterNoise ? core.data() : nullptr, nullptr), "find_core");
@@ -452,7 +452,7 @@ void sDbscan::test_sngDbscan(const float* X, float eps, float rangeEps, int n_te
     std::cout << "n_tests: " << n_tests << std::endl;
     const bool v = verbose;
     for (int i = 0; i < n_tests; ++i) {
-        const float e = eps + 1.0f * i * rangeEps;
+        const float e = sweep_eps(eps, rangeEps, i);
         std::cout << "Eps: " << e << std::endl;
         verbose = true;
         fit_sngDbscan(X, e, minPts);

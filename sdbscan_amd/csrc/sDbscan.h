@@ -112,6 +112,10 @@ public:
     void load_test_sngDbscan(const std::string& dataset, float eps, float rangeEps, int n_tests, int minPts);
     void load_fit_sngOptics(const std::string& dataset, float eps, int minPts);
 
+    // eps of test i of an eps sweep: eps + 1.0 * i * rangeEps in double,
+    // rounded once to float (src/sDbscan.cpp:1016, :1221)
+    static float sweep_eps(float eps, float rangeEps, int i) { return (float)(eps + 1.0 * i * rangeEps); }
+
     // Extensions of this build (no counterpart in the reference)
     void set_precision(const std::string& p);  // "FP32" (default) or "BF16" distance tiles
     void set_device(int dev);

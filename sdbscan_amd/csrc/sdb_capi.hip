@@ -282,14 +282,19 @@ int sdb_set_params(sdb_ctx* c, const sdb_params* p) {
     if (q.precision != SDB_FP32 && q.precision != SDB_BF16) return fail(c, SDB_EINVAL, "unknown precision");
     if (q.precision == SDB_BF16 && q.metric != SDB_COSINE && q.metric != SDB_CHI2)
         return fail(c, SDB_EUNSUPPORTED, "BF16 distance tiles are implemented for Cosine and Chi2 only");
-    if ((long long)q.n_points * 2 * q.topK >= (1ll << 32)) return fail(c, SDB_EUNSUPPORTED, "n_points * 2 * topK must stay below 2^32");
+    // the (point, slot) entries are counted, scanned and grouped in signed
+    // 32-bit offsets
+    if ((long long)q.n_points * 2 * q.topK >= (1ll << 31)) return fail(c, SDB_EUNSUPPORTED, "n_points * 2 * topK must stay below 2^31");
     if (2 * q.n_proj >= (1 << 24)) return fail(c, SDB_EUNSUPPORTED, "n_proj too large");
     // an input loaded under other dimensions does not fit these parameters:
     // it has to be loaded again (its buffer is kept for reuse)
     if (c->X && (q.n_points != c->n || q.n_features != c->d)) {
         c->X = nullptr;
-        c->transformed = c->sng = c->clustered = false;
+        c->sng = c->clustered = false;
     }
+    // the transformed rows depend on the metric, the precision and the kernel
+    // parameters: made again by the next pass that needs them
+    c->transformed = false;
     c->prm = q;
     c->have_params = true;
     c->indexed = false;

@@ -1184,7 +1184,7 @@ __global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ top
     for (int i = threadIdx.x; i < L2; i += blockDim.x) hist[i] = 0;
     __syncthreads();
     for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-        const uint32_t a = (uint32_t)t / (uint32_t)(2 * K);  // n * 2K < 2^32 (sdb_set_params)
+        const uint32_t a = (uint32_t)t / (uint32_t)(2 * K);  // n * 2K < 2^31 (sdb_set_params)
         const int s = (int)((uint32_t)t - a * (uint32_t)(2 * K));
         atomicAdd(&hist[slot_list(topK, a, s, K, P)], 1);
     }
@@ -1204,7 +1204,7 @@ __global__ __launch_bounds__(256) void k_group_scatter(const int* __restrict__ t
         cur[i] = wgoff[((size_t)blockIdx.y * L2 + i) * Gx + blockIdx.x];
     __syncthreads();
     for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-        const uint32_t a = (uint32_t)t / (uint32_t)(2 * K);  // n * 2K < 2^32 (sdb_set_params)
+        const uint32_t a = (uint32_t)t / (uint32_t)(2 * K);  // n * 2K < 2^31 (sdb_set_params)
         const int s = (int)((uint32_t)t - a * (uint32_t)(2 * K));
         const int pos = atomicAdd(&cur[slot_list(topK, a, s, K, P)], 1);
         grp[pos] = (uint32_t)t;

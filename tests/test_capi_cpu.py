@@ -138,3 +138,35 @@ def test_product_path_fails_loudly_without_device_or_library():
     out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True,
                          timeout=120)
     assert out.stdout.strip() == "missing", out.stdout + out.stderr
+
+
+def test_eps_sweep_rounds_like_the_reference(tmp_path):
+    """test_sDbscan / test_sngDbscan sweep eps + 1.0 * i * rangeEps in double
+    and round once to float (src/sDbscan.cpp:1016, :1221).  A float-only
+    expression differs by one ulp for a few inputs in a hundred, e.g.
+    0.7f + 3 * 0.05f -> 0.849999964 (double, then float) against 0.850000024
+    (float throughout)."""
+    import subprocess
+    src = tmp_path / "sweep.cpp"
+    src.write_text(
+        '#include <cstdio>\n#include <random>\n#include "sDbscan.h"\n'
+        'int main() {\n'
+        '  int bad = 0, float_differs = 0;\n'
+        '  std::mt19937 g(7); std::uniform_real_distribution<float> u(0.0f, 1.0f);\n'
+        '  for (int t = 0; t < 20000; ++t) {\n'
+        '    const float eps = u(g), range = 0.1f * u(g); const int i = t % 16;\n'
+        '    const float ref = eps + 1.0 * i * range;  /* the reference statement */\n'
+        '    bad += sDbscan::sweep_eps(eps, range, i) != ref;\n'
+        '    float_differs += (eps + 1.0f * i * range) != ref;\n'
+        '  }\n'
+        '  std::printf("%d %d %.9g\\n", bad, float_differs, sDbscan::sweep_eps(0.7f, 0.05f, 3));\n'
+        '  return 0; }\n')
+    exe = tmp_path / "sweep"
+    lib_dir = os.path.dirname(sdbscan_amd.LIB_PATH)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(ROOT, "sdbscan_amd", "csrc"), str(src), "-L", lib_dir, "-lsdbscan_amd",
+                    "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
+    bad, differs, e = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert int(bad) == 0
+    assert int(differs) > 100  # the float-only form would not pass
+    assert e == "0.849999964"

@@ -359,3 +359,227 @@ def test_sng_sampled_counts_duplicates():
         assert deg[p] == own + rev
         nb = {int(q) for q in seq if q != p and hit[p, q]} | {m for m in range(60) if m != p and p in seq and hit[m, p]}
         assert degd[p] == len(nb)
+
+
+# ------------------------------------------------------------ embeddings
+U32 = 2.0 ** -23
+
+
+def _rff_inputs(metric, seed=11, n=40, d=51, half=256, sigma=3.0):
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((n, d)) * 2.0).astype(np.float32)
+    if metric == L2:
+        R = O.gauss(half, d, 0.0, 1.0 / sigma, seed)
+    else:
+        R = O.cauchy(half, d, 0.0, 1.0 / sigma, seed)
+    return X, R.reshape(d, half).T.copy(), R  # R as (half x d), and the flat column-major buffer
+
+
+@pytest.mark.parametrize("metric", [L2, L1])
+def test_rff_embedding_values_and_layout(metric):
+    """[cos(Rx); sin(Rx)] (src/sDbscan.cpp:59-64) with R the (ker/2) x d
+    column-major matrix: each feature pair lies on the unit circle, each value
+    is within the FP32 rounding of the product R x (sequential sum, bound
+    (d+1) u sum |R_i x_i|) of the FP64 cos / sin, and the layout puts all
+    cosines first (ker = 2 * (ker / 2): R has ker / 2 rows)."""
+    X, Rm, Rflat = _rff_inputs(metric)
+    half, d = Rm.shape
+    E = O.embed(X, metric, 2 * half, 0.4, Rflat).astype(np.float64)
+    c, s = E[:, :half], E[:, half:2 * half]
+    assert np.abs(c * c + s * s - 1.0).max() < 4 * U32
+    prod = X.astype(np.float64) @ Rm.T.astype(np.float64)          # n x half
+    bound = (d + 1) * U32 * (np.abs(X.astype(np.float64)) @ np.abs(Rm.T.astype(np.float64))) + 2 * U32
+    assert np.all(np.abs(c - np.cos(prod)) <= bound)
+    assert np.all(np.abs(s - np.sin(prod)) <= bound)
+    # the transposed matrix (a plausible indexing mistake) is far off
+    wrong = X.astype(np.float64) @ Rflat.reshape(half, d).T.astype(np.float64)
+    assert np.abs(c - np.cos(wrong)).max() > 0.1
+
+
+@pytest.mark.parametrize("metric", [L2, L1])
+def test_rff_inner_product_approximates_the_kernel(metric):
+    """sum_r cos(w_r . (x - y)) = <e(x), e(y)> exactly in FP64; its mean over
+    the draws approximates exp(-||x-y||^2 / (2 sigma^2)) for w ~ N(0, 1/sigma)
+    and exp(-||x-y||_1 / sigma) for w ~ Cauchy(0, 1/sigma)."""
+    sigma, half = 6.0, 4096
+    rng = np.random.default_rng(3)
+    d = 8
+    X = (rng.standard_normal((30, d)) * 1.5).astype(np.float32)
+    R = O.gauss(half, d, 0.0, 1.0 / sigma, 5) if metric == L2 else O.cauchy(half, d, 0.0, 1.0 / sigma, 5)
+    Rm = R.reshape(d, half).T.astype(np.float64)
+    E = O.embed(X, metric, 2 * half, 0.4, R).astype(np.float64)
+    G = E @ E.T
+    X64 = X.astype(np.float64)
+    # per feature: the FP32 product R x is within (d+1) u sum |R_i x_i| of R x
+    bnd = (d + 1) * U32 * (np.abs(X64) @ np.abs(Rm.T)) + 2 * U32
+    for i in range(0, 30, 3):
+        for j in range(1, 30, 4):
+            delta = X64[i] - X64[j]
+            exact = np.cos(Rm @ delta).sum()
+            assert abs(G[i, j] - exact) <= 2.0 * (bnd[i] + bnd[j]).sum() + 1e-9
+            if metric == L2:
+                ker = np.exp(-(delta @ delta) / (2 * sigma * sigma))
+            else:
+                ker = np.exp(-np.abs(delta).sum() / sigma)
+            assert abs(G[i, j] / half - ker) < 5.0 / np.sqrt(half)
+
+
+def _kappa(metric, lam):
+    sech = 1.0 / np.cosh(np.pi * lam)
+    if metric == CHI2:
+        return sech
+    return 2.0 * sech / (np.log(4.0) * (1.0 + 4.0 * lam * lam))
+
+
+@pytest.mark.parametrize("metric", [CHI2, JS])
+def test_sqrt_features_without_sampled_components(metric):
+    """ker < 3d: only e_j = sqrt(L kappa(0) x_j) for x_j > 0
+    (src/Utilities.h:139-145, 182-188), zero elsewhere and past d."""
+    X, _ = datagen.dirichlet(50, 100, k=3, seed=2)
+    X[:, 7] = 0.0
+    L = 0.4
+    E = O.embed(X, metric, 250, L).astype(np.float64)
+    want = np.sqrt(L * _kappa(metric, 0.0) * X.astype(np.float64))
+    assert np.all(E[:, 100:] == 0.0) and np.all(E[:, 7] == 0.0)
+    # x L rounds in FP32 (Chi2): a subnormal product keeps an absolute error
+    # of up to 2^-149, so sqrt of it up to 2^-74.5
+    tol = 2 * U32 * want + np.where(X.astype(np.float64) * L < 2.0 ** -126, 2.0 ** -74, 0.0)
+    assert np.all(np.abs(E[:, :100] - want) <= tol)
+
+
+@pytest.mark.parametrize("metric", [CHI2, JS])
+def test_sampled_feature_map_closed_form(metric):
+    """ker >= 3d adds n = (ker/d - 1)/2 components of sqrt(2 x L kappa(iL))
+    times cos / sin of iL ln x (src/Utilities.h:147-163, 190-207).  Their
+    inner product has the closed form
+      <e(x), e(y)> = sum_j L sqrt(x_j y_j) [kappa(0) + 2 sum_i kappa(iL) cos(iL ln(x_j / y_j))]
+    over coordinates positive in both (kappa = sech(pi lambda) for Chi2,
+    2 sech(pi lambda) / (ln 4 (1 + 4 lambda^2)) for JS), which tends to the
+    kernel 2xy/(x+y) (Chi2) or x/2 log2((x+y)/x) + y/2 log2((x+y)/y) (JS)
+    as n grows.  Each value also sits in its slot: cosines at d + 2(i-1)d + j,
+    sines d further."""
+    d, L, comps = 60, 0.4, 3
+    ker = d * (2 * comps + 1)
+    X, _ = datagen.dirichlet(40, d, k=4, base=0.3, seed=7)
+    X[:, 5] = 0.0
+    X = X / X.sum(1, keepdims=True)
+    E = O.embed(X, metric, ker + 13, L).astype(np.float64)
+    assert np.all(E[:, ker:] == 0.0)
+    X64 = X.astype(np.float64)
+    lx = np.log(np.where(X64 > 0, X64, 1.0))
+    for i in range(1, comps + 1):
+        f = np.sqrt(2.0 * X64 * L * _kappa(metric, i * L))
+        base = d + (i - 1) * 2 * d
+        tol = 8e-6 * (1 + np.abs(i * L * lx)) * f + 1e-9
+        assert np.all(np.abs(E[:, base:base + d] - f * np.cos(i * L * lx)) <= tol)
+        assert np.all(np.abs(E[:, base + d:base + 2 * d] - f * np.sin(i * L * lx)) <= tol)
+    G = E @ E.T
+    for a in range(0, 40, 3):
+        for b in range(1, 40, 5):
+            x, y = X64[a], X64[b]
+            m = (x > 0) & (y > 0)
+            r = np.log(x[m] / y[m])
+            s = _kappa(metric, 0.0) + 2.0 * sum(_kappa(metric, i * L) * np.cos(i * L * r) for i in range(1, comps + 1))
+            closed = (L * np.sqrt(x[m] * y[m]) * s).sum()
+            assert abs(G[a, b] - closed) < 2e-5 * (1 + abs(closed))
+            if metric == CHI2:
+                kern = (2 * x[m] * y[m] / (x[m] + y[m])).sum()
+            else:
+                xm, ym = x[m], y[m]
+                kern = (xm / 2 * np.log2((xm + ym) / xm) + ym / 2 * np.log2((xm + ym) / ym)).sum()
+            assert abs(G[a, b] - kern) < 0.02 * kern
+
+
+# ------------------------------------------------------------- distances
+def _js_via_kl(x, y):
+    """Jensen-Shannon form of src/Utilities.h:245-266 on the smoothed rows
+    x' = x + 1e-6: sum x' log2((x'+y')/x') = sum x' - KL2(x' || m) with
+    m = (x' + y') / 2, so the reference's value is
+    1 - (sum x' + sum y') / 2 + (KL2(x' || m) + KL2(y' || m)) / 2."""
+    from scipy.special import rel_entr
+    xe = x.astype(np.float64) + 1e-6
+    ye = y.astype(np.float64) + 1e-6
+    m = (xe + ye) / 2.0
+    kl = (rel_entr(xe, m).sum() + rel_entr(ye, m).sum()) / np.log(2.0)
+    return 1.0 - (xe.sum() + ye.sum()) / 2.0 + kl / 2.0
+
+
+def _brute64(x, y, metric):
+    x64, y64 = x.astype(np.float64), y.astype(np.float64)
+    if metric == COSINE:
+        return 1.0 - float(np.dot(x64, y64)), float(np.abs(x64 * y64).sum())
+    if metric == L1:
+        v = float(np.abs(x64 - y64).sum())
+        return v, v
+    if metric == L2:
+        v = float(np.linalg.norm(x64 - y64))
+        return v, v
+    if metric == CHI2:
+        m = (x64 > 0) & (y64 > 0)
+        t = (x64[m] * y64[m] / (x64[m] + y64[m])).sum()
+        return 1.0 - 2.0 * t, 2.0 * t
+    v = _js_via_kl(x, y)
+    return v, 1.0 + abs(v)
+
+
+@pytest.mark.parametrize("metric", [COSINE, L1, L2, CHI2, JS])
+def test_distances_match_fp64_brute_force(metric):
+    """orc_dist64 (the FP64 decision value) equals an FP64 brute force in
+    numpy (JS through a Kullback-Leibler form of the same quantity), and
+    orc_dist32 (sequential FP32 sums, the reference's arithmetic) lies within
+    the FP32 rounding bound (d + 8) u of the summed magnitudes."""
+    rng = np.random.default_rng(metric)
+    d = 300
+    if metric in (CHI2, JS):
+        X, _ = datagen.dirichlet(30, d, k=3, seed=metric)
+        X, _ = O.transform(X, metric)
+        X[3, :40] = 0.0  # disjoint supports for a few coordinates
+    elif metric == COSINE:
+        X, _ = O.transform(rng.standard_normal((30, d)).astype(np.float32), COSINE)
+    else:
+        X = (rng.standard_normal((30, d)) * 0.7).astype(np.float32)
+    for i in range(30):
+        for j in range(0, 30, 4):
+            want, mag = _brute64(X[i], X[j], metric)
+            got64 = O.dist64(X[i], X[j], metric)
+            assert abs(got64 - want) <= 1e-12 * (1 + mag), (i, j)
+            got32 = O.dist32(X[i], X[j], metric)
+            tol = (d + 8) * U32 * mag + 2 * U32 * (1 + abs(want))
+            if metric == JS:
+                tol = 2 * (2 * d + 200) * U32 * (1 + abs(want))
+            assert abs(got32 - want) <= tol, (i, j, got32, want)
+
+
+def test_distance_special_values():
+    x = np.zeros(64, np.float32)
+    x[3] = 1.0
+    y = np.zeros(64, np.float32)
+    y[5] = 1.0
+    assert O.dist64(x, x, COSINE) == 0.0 and O.dist64(x, y, COSINE) == 1.0
+    assert O.dist64(x, y, L1) == 2.0 and O.dist64(x, y, L2) == np.sqrt(2.0)
+    assert O.dist64(x, x, CHI2) == 0.0 and O.dist64(x, y, CHI2) == 1.0
+    # JS of identical histograms: (x'+y')/x' = 2 -> sum x' log2 2 = sum x'
+    want = 1.0 - (1.0 + 64e-6)
+    assert abs(O.dist64(x, x, JS) - want) < 1e-12
+
+
+# --------------------------------------------------------- Cauchy draws
+def test_cauchy_quantiles():
+    """cauchy_distribution(x0, gamma) (src/Utilities.h:105-122): median x0,
+    quartiles x0 -+ gamma (the inverse CDF x0 + gamma tan(pi (p - 1/2)))."""
+    for x0, g in ((0.0, 0.5), (1.5, 2.0)):
+        v = O.cauchy(1000, 100, x0, g, 17).astype(np.float64)
+        q1, med, q3 = np.quantile(v, [0.25, 0.5, 0.75])
+        assert abs(med - x0) < 0.03 * g
+        assert abs(q1 - (x0 - g)) < 0.05 * g and abs(q3 - (x0 + g)) < 0.05 * g
+        # tails: P(|x - x0| > 10 gamma) = 1 - 2 atan(10) / pi
+        tail = np.mean(np.abs(v - x0) > 10 * g)
+        assert abs(tail - (1 - 2 * np.arctan(10.0) / np.pi)) < 0.006
+
+
+def test_gauss_moments():
+    """normal_distribution(mean, stddev) fills column-major in draw order
+    (src/Utilities.h:76-93): moments of 10^5 draws."""
+    v = O.gauss(1000, 100, 0.25, 1.5, 23).astype(np.float64)
+    assert abs(v.mean() - 0.25) < 0.02 and abs(v.std() - 1.5) < 0.02
+    assert abs(np.mean(np.abs(v - 0.25) < 1.5) - 0.6827) < 0.006
