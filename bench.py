@@ -151,12 +151,8 @@ def l2_bench(args):
     d = 51
     t0 = time.perf_counter()
     X, _ = datagen.blobs(n, d, seed=1001)
-    rng = np.random.default_rng(1001)
-    smp = X[rng.choice(n, 10000, replace=False)].astype(np.float64)
-    sq = (smp * smp).sum(1)
-    D2 = np.maximum(sq[:, None] + sq[None, :] - 2.0 * smp @ smp.T, 0.0)
-    sigma = float(np.sqrt(np.median(D2[np.triu_indices(10000, 1)])))
-    eps = sigma * float(np.sqrt(-2.0 * np.log(0.7)))
+    sigma = datagen.median_pairwise_l2(X, 10000, seed=1001)
+    eps = datagen.l2_eps_for_embedded_cosine(sigma, 0.3)
     log(f"[l2] generated {n} x {d} in {time.perf_counter() - t0:.1f} s; sigma {sigma:.3f}, eps {eps:.3f}")
     ctx = S.Context(0)
     p = ctx.set_params(n, d, n_proj=P, topK=K, topM=M, metric=S.L2, ker_n_features=1024, ker_sigma=sigma)

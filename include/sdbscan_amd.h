@@ -20,6 +20,8 @@
  *   sdb_batch_run      <- test_sDbscan's eps sweep (src/sDbscan.cpp:980-1039) over
  *                         many datasets at once (no batch entry in the reference)
  *   sdb_get_index      <- the protected matrix_topK / matrix_topM (src/sDbscan.h:54-55)
+ *   sdb_get_projections <- columns of the local MATRIX_FHT of rp_parIndex
+ *                         (src/sDbscan.cpp:31, 74-91)
  *
  * All entry points return 0 on success and a nonzero SDB_E* code otherwise;
  * sdb_last_error() describes the last failure.  No exception or exit() crosses
@@ -155,6 +157,14 @@ int sdb_sng_find_core(sdb_ctx* ctx, const int* samples, int n_samples, const int
 
 /* Index matrices in the reference layouts (either pointer may be NULL). */
 int sdb_get_index(sdb_ctx* ctx, int* topK_out, int* topM_out);
+
+/* HD3 projections of listed points after sdb_build_index (the columns of
+ * MATRIX_FHT, src/sDbscan.cpp:74-91, recomputed on the device from the
+ * transformed rows and the index's sign bits): out is n_proj x n_ids, column
+ * q (n_proj consecutive floats) for point ids[q].  The values equal the
+ * reference's fht_float chain as IEEE values (the sign of an exact zero may
+ * differ). */
+int sdb_get_projections(sdb_ctx* ctx, const int* ids, int n_ids, float* out);
 
 /* Transformed points (n x n_features) as used by the distance tiles. */
 int sdb_get_transformed(sdb_ctx* ctx, float* X_out);

@@ -191,6 +191,41 @@ float dist32(const float* x, const float* y, int d, int dist) {
     }
 }
 
+// FP32 distance in the order of the device's neighbourhood distances (the
+// sOPTICS lists): per lane l = 0..63 a sequential sum over i = l, l + 64, ...
+// (fmaf for the Cosine and L2 terms), then an xor butterfly over the lanes
+// with offsets 32, 16, ..., 1.  Chi2 divides correctly rounded; JS uses the
+// host log, so its values are not the device's bit for bit.
+float dist32_lanes(const float* x, const float* y, int d, int dist) {
+    float part[64];
+    for (int l = 0; l < 64; ++l) {
+        float s = 0.0f;
+        for (int i = l; i < d; i += 64) {
+            const float xv = x[i], yv = y[i];
+            if (dist == D_COSINE) s = std::fmaf(xv, yv, s);
+            else if (dist == D_L1) s = s + std::fabs(xv - yv);
+            else if (dist == D_L2) { const float t = xv - yv; s = std::fmaf(t, t, s); }
+            else if (dist == D_CHI2) { if (xv > 0.0f && yv > 0.0f) s = s + xv * yv / (xv + yv); }
+            else {
+                const float xe = xv + 1e-6f, ye = yv + 1e-6f, u = xe + ye;
+                s = s + xe * (std::log(u / xe) * 1.4426950408889634f) + ye * (std::log(u / ye) * 1.4426950408889634f);
+            }
+        }
+        part[l] = s;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        float nxt[64];
+        for (int l = 0; l < 64; ++l) nxt[l] = part[l] + part[l ^ off];
+        std::memcpy(part, nxt, sizeof(nxt));
+    }
+    const float S = part[0];
+    if (dist == D_COSINE) return 1.0f - S;
+    if (dist == D_L1) return S;
+    if (dist == D_L2) return std::sqrt(S);
+    if (dist == D_CHI2) return 1.0f - 2.0f * S;
+    return 1.0f - S * 0.5f;
+}
+
 }  // namespace
 
 extern "C" {
@@ -333,6 +368,7 @@ void orc_topm(const float* proj, int n, int P, int M, int* topM) {
 
 // ----------------------------------------------------------- distances
 float orc_dist32(const float* x, const float* y, int d, int dist) { return dist32(x, y, d, dist); }
+float orc_dist32_lanes(const float* x, const float* y, int d, int dist) { return dist32_lanes(x, y, d, dist); }
 
 // FP64 distance in the fixed order of the HIP boundary recheck: per lane
 // l = 0..63 a sequential sum over i = l, l + 64, ..., then the xor butterfly
@@ -364,9 +400,11 @@ double orc_dist64(const float* x, const float* y, int d, int dist) {
 }
 
 // ------------------------------------------------------------ find core
-// src/sDbscan.cpp:217-408.  mode 0: decide with the FP32 value (reference
-// arithmetic, sequential order); mode 1: decide with the FP64 value (the
-// decision rule of the HIP path).  Outputs the deduplicated neighbour count of
+// src/sDbscan.cpp:217-408.  mode bit 0 clear: decide with the FP32 value
+// (reference arithmetic, sequential order); set: decide with the FP64 value
+// (the decision rule of the HIP path).  mode bit 1: the CSR distances are
+// dist32_lanes (the device's neighbourhood distances) instead of the
+// sequential FP32 values.  Outputs the deduplicated neighbour count of
 // every point, the core flags, the neighbour CSR (sorted by index; kept for
 // every point, core or not) and the number of computed pairs whose FP64
 // distance lies within 1e-6 of eps.  Optionally the FP32 distance per CSR
@@ -401,16 +439,17 @@ void* orc_find_core(const float* X, int n, int d, int dist, const int* topK, int
                     ++ndist;
                     const float d32 = dist32(xp, xq, d, dist);
                     bool hit = d32 <= eps;
-                    if (mode == 1 || near_pairs) {
+                    if ((mode & 1) || near_pairs) {
                         const double d64 = orc_dist64(xp, xq, d, dist);
                         if (std::fabs(d64 - (double)eps) < 1e-6) ++near;
-                        if (mode == 1) hit = d64 <= (double)eps;
+                        if (mode & 1) hit = d64 <= (double)eps;
                     }
                     if (hit) {
+                        const float dk = (mode & 2) ? dist32_lanes(xp, xq, d, dist) : d32;
 #pragma omp critical
                         {
-                            nbr[p].emplace_back(q, d32);
-                            nbr[q].emplace_back(p, d32);
+                            nbr[p].emplace_back(q, dk);
+                            nbr[q].emplace_back(p, dk);
                         }
                     }
                 }

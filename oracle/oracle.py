@@ -40,6 +40,7 @@ def lib():
             "orc_heap_select": ([_P, ctypes.c_int, ctypes.c_int, _P, _P], None),
             "orc_topm": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P], None),
             "orc_dist32": ([_P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_float),
+            "orc_dist32_lanes": ([_P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_float),
             "orc_dist64": ([_P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_double),
             "orc_find_core": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, _P,
                                ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P],
@@ -146,6 +147,12 @@ def dist32(x, y, metric):
     return lib().orc_dist32(x.ctypes.data, y.ctypes.data, len(x), metric)
 
 
+def dist32_lanes(x, y, metric):
+    x = _c(x, np.float32)
+    y = _c(y, np.float32)
+    return lib().orc_dist32_lanes(x.ctypes.data, y.ctypes.data, len(x), metric)
+
+
 def dist64(x, y, metric):
     x = _c(x, np.float32)
     y = _c(y, np.float32)
@@ -153,7 +160,9 @@ def dist64(x, y, metric):
 
 
 def find_core(X, metric, topK, topM, eps, minPts, mode=1):
-    """Returns core (n,), degree (n,), CSR (ptr, idx, dist32), near_pairs, n_dist."""
+    """Returns core (n,), degree (n,), CSR (ptr, idx, dist32), near_pairs, n_dist.
+    mode bit 0: decide in FP64 (else sequential FP32); bit 1: CSR distances in
+    the device's lane order (dist32_lanes)."""
     X = _c(X, np.float32)
     n, d = X.shape
     topK = _c(topK, np.int32)

@@ -74,6 +74,7 @@ _SIGS = {
     "sdb_sng_find_core": ([_P, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int, _P, _P],
                           ctypes.c_int),
     "sdb_get_index": ([_P, _P, _P], ctypes.c_int),
+    "sdb_get_projections": ([_P, _P, ctypes.c_int, _P], ctypes.c_int),
     "sdb_get_transformed": ([_P, _P], ctypes.c_int),
     "sdb_get_transformed_bf16": ([_P, _P], ctypes.c_int),
     "sdb_get_neighbors": ([_P, _P, _P, _P], ctypes.c_int),
@@ -258,6 +259,15 @@ class Context:
         tm = np.zeros((p.n_proj, 2 * p.topM), np.int32)
         self._chk(self.lib.sdb_get_index(self.h, tk.ctypes.data, tm.ctypes.data))
         return tk, tm
+
+    def get_projections(self, ids):
+        """HD3 projections of the listed points: (len(ids), n_proj) float32,
+        row q = the MATRIX_FHT column of point ids[q]."""
+        import numpy as np
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        out = np.zeros((len(ids), self.params.n_proj), np.float32)
+        self._chk(self.lib.sdb_get_projections(self.h, ids.ctypes.data, len(ids), out.ctypes.data))
+        return out
 
     def get_transformed(self):
         import numpy as np

@@ -117,6 +117,8 @@ struct sdb_ctx {
     bool clustered = false;
     // noise labelling
     DevBuf ns_ids, cproj, noise, nbest;
+    // projections read back for listed points
+    DevBuf pj_ids, pj_buf;
     // sample-sequence neighbourhoods
     DevBuf occ_ptr, occ_pos, win, rev, csr_cnt, csr_ptr, csr_cur, csr_idx;
     bool transformed = false;
@@ -252,7 +254,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
                      &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
                      &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
-                     &c->cproj, &c->noise, &c->nbest, &c->occ_ptr, &c->occ_pos, &c->win, &c->rev,
+                     &c->cproj, &c->noise, &c->nbest, &c->pj_ids, &c->pj_buf, &c->occ_ptr, &c->occ_pos, &c->win, &c->rev,
                      &c->csr_cnt, &c->csr_ptr, &c->csr_cur, &c->csr_idx};
     for (DevBuf* b : all)
         if (b->p) hipFree(b->p);
@@ -635,6 +637,47 @@ int sdb_get_index(sdb_ctx* c, int* topK_out, int* topM_out) {
     HIPCHK(hipSetDevice(c->device));
     if (topK_out) HIPCHK(hipMemcpy(topK_out, c->topK.p, (size_t)c->n * 2 * c->prm.topK * 4, hipMemcpyDeviceToHost));
     if (topM_out) HIPCHK(hipMemcpy(topM_out, c->topM.p, (size_t)c->prm.n_proj * 2 * c->prm.topM * 4, hipMemcpyDeviceToHost));
+    return SDB_OK;
+}
+
+// HD3 projections (src/sDbscan.cpp:74-91) of listed points, recomputed from
+// the transformed rows and the index's sign bits by the index kernel's
+// point mode: column q of out (n_proj floats) is the MATRIX_FHT column of
+// point ids[q].
+int sdb_get_projections(sdb_ctx* c, const int* ids, int n_ids, float* out) {
+    if (!c || n_ids < 0 || (n_ids > 0 && (!ids || !out))) return SDB_EINVAL;
+    if (!c->indexed) return fail(c, SDB_ESTATE, "build_index first");
+    if (n_ids == 0) return SDB_OK;
+    for (int i = 0; i < n_ids; ++i)
+        if (ids[i] < 0 || ids[i] >= c->n) return fail(c, SDB_EINVAL, "point %d out of range", ids[i]);
+    HIPCHK(hipSetDevice(c->device));
+    const sdb_params& q = c->prm;
+    const int P = q.n_proj;
+    hipStream_t st = c->st;
+    ENSURE(c->pj_ids, (size_t)n_ids * 4);
+    ENSURE(c->pj_buf, (size_t)P * n_ids * 4);
+    HIPCHK(hipMemcpyAsync(c->pj_ids.p, ids, (size_t)n_ids * 4, hipMemcpyHostToDevice, st));
+    IndexArgs ia{};
+    ia.X = c->X;
+    ia.Xn = P_<float>(c->Xn);
+    ia.err = P_<int>(c->err_flag);
+    ia.signmask = P_<uint32_t>(c->bits);
+    ia.n = c->n; ia.ldx = c->ldx; ia.d = c->d; ia.metric = q.metric; ia.ker = q.ker_n_features;
+    ia.kerL = q.ker_intervalSampling;
+    ia.R = (q.metric == SDB_L1 || q.metric == SDB_L2) ? P_<float>(c->R) : nullptr;
+    ia.P = P; ia.K = q.topK;
+    ia.pt_ids = P_<int>(c->pj_ids);
+    ia.n_ids = n_ids;
+    ia.pt_buf = P_<float>(c->pj_buf);
+    const int G = (int)std::max<long long>(1, std::min<long long>(2048, ((long long)n_ids + 3) / 4));
+    LCHK(launch_index(ia, ilog2(q.fhtDim), 3, G, st));
+    // the point mode writes row e of the P x n_ids block at e * n_ids; the
+    // caller gets one column of P values per point (MATRIX_FHT's layout)
+    std::vector<float> rows((size_t)P * n_ids);
+    HIPCHK(hipMemcpyAsync(rows.data(), c->pj_buf.p, rows.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int e = 0; e < P; ++e)
+        for (int qi = 0; qi < n_ids; ++qi) out[(size_t)qi * P + e] = rows[(size_t)e * n_ids + qi];
     return SDB_OK;
 }
 

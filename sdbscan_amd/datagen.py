@@ -66,3 +66,23 @@ def sphere_mixture(n, d, k, seed, spread=0.35):
     X += centers.astype(np.float32)[comp]
     X /= np.linalg.norm(X, axis=1, keepdims=True)
     return X, comp
+
+
+def median_pairwise_l2(X, n_sample=10000, seed=1001):
+    """Median L2 distance over the pairs of a seeded sample of rows (the RFF
+    bandwidth sigma of the 1.8M x 51 L2 workload)."""
+    rng = np.random.default_rng(seed)
+    n = X.shape[0]
+    smp = X[rng.choice(n, min(n_sample, n), replace=False)].astype(np.float64)
+    sq = (smp * smp).sum(1)
+    D2 = np.maximum(sq[:, None] + sq[None, :] - 2.0 * smp @ smp.T, 0.0)
+    return float(np.sqrt(np.median(D2[np.triu_indices(len(smp), 1)])))
+
+
+def l2_eps_for_embedded_cosine(sigma, eps_cos):
+    """Input-space L2 radius whose embedded cosine distance is eps_cos.  The
+    normalised random Fourier features of N(0, 1/sigma) draws approximate the
+    Gaussian kernel, so 1 - <e(x), e(y)> ~ 1 - exp(-r^2 / (2 sigma^2)) for
+    ||x - y|| = r; eps_cos maps to r = sigma sqrt(-2 ln(1 - eps_cos)).  The
+    reference decides L2 on the input rows (src/Utilities.h:218-219)."""
+    return float(sigma * np.sqrt(-2.0 * np.log(1.0 - eps_cos)))

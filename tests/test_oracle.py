@@ -548,6 +548,13 @@ def test_distances_match_fp64_brute_force(metric):
             if metric == JS:
                 tol = 2 * (2 * d + 200) * U32 * (1 + abs(want))
             assert abs(got32 - want) <= tol, (i, j, got32, want)
+            # the device's lane order: 64 sequential sums of d / 64 terms, then
+            # a 6-level tree
+            got32l = O.dist32_lanes(X[i], X[j], metric)
+            tol_l = (d // 64 + 16) * U32 * mag + 2 * U32 * (1 + abs(want))
+            if metric == JS:
+                tol_l = 2 * (2 * d + 200) * U32 * (1 + abs(want))
+            assert abs(got32l - want) <= tol_l, (i, j, got32l, want)
 
 
 def test_distance_special_values():
@@ -583,3 +590,28 @@ def test_gauss_moments():
     v = O.gauss(1000, 100, 0.25, 1.5, 23).astype(np.float64)
     assert abs(v.mean() - 0.25) < 0.02 and abs(v.std() - 1.5) < 0.02
     assert abs(np.mean(np.abs(v - 0.25) < 1.5) - 0.6827) < 0.006
+
+
+def test_l2_eps_mapping_matches_the_embedded_cosine_distance():
+    """The L2 workload's eps: the radius r whose normalised random-feature
+    cosine distance 1 - <e(x), e(y)> / half is 0.3 (features of the oracle's
+    embedding, N(0, 1/sigma) draws)."""
+    sigma, half, d = 5.0, 8192, 51
+    r = datagen.l2_eps_for_embedded_cosine(sigma, 0.3)
+    assert abs(1.0 - np.exp(-r * r / (2 * sigma * sigma)) - 0.3) < 1e-12
+    rng = np.random.default_rng(8)
+    R = O.gauss(half, d, 0.0, 1.0 / sigma, 9)
+    dists = []
+    for t in range(8):
+        x = rng.standard_normal(d).astype(np.float32)
+        u = rng.standard_normal(d)
+        y = (x + r * u / np.linalg.norm(u)).astype(np.float32)
+        E = O.embed(np.stack([x, y]), L2, 2 * half, 0.4, R).astype(np.float64)
+        dists.append(1.0 - E[0] @ E[1] / half)
+    assert abs(np.mean(dists) - 0.3) < 0.02
+    # sigma is the median pairwise distance of the sample
+    X, _ = datagen.blobs(3000, d, seed=1001)
+    s = datagen.median_pairwise_l2(X, 1000, seed=3)
+    smp = X[np.random.default_rng(3).choice(3000, 1000, replace=False)].astype(np.float64)
+    from scipy.spatial.distance import pdist
+    assert abs(s - np.median(pdist(smp))) < 1e-9 * s
