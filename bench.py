@@ -106,6 +106,8 @@ def tile_accuracy(ctx, wl, n_samples=1 << 20):
             "max_rel_err_vs_fp64": float(rel.max()),
             "mean_abs_err_vs_fp64": float(err.mean()), "band": st["band"],
             "quantity": "1 - sum sqrt(x y) (chi2 lower bound)" if wl["metric"] == 3 else "distance",
+            "fast_arithmetic": "emulation of the tiles: the tiles' operands (BF16 or FP32) with FP32 fmaf sums "
+                               "per lane and a lane tree, not the MFMA unit's own order",
             "pairs_within_1e-6_of_eps": st["n_near_eps"]}
 
 

@@ -123,6 +123,9 @@ struct sdb_ctx {
     DevBuf occ_ptr, occ_pos, win, rev, csr_cnt, csr_ptr, csr_cur, csr_idx;
     bool transformed = false;
     int transformed_prec = -1;
+    // BF16 rounding bound of every row (bf16_row_err) and its maximum
+    DevBuf rowerr, rowerr_max;
+    float rho_max = 0.f;
     int sng_Ms = 64;
     sdb_stats stats{};
 };
@@ -254,7 +257,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
                      &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
                      &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
-                     &c->cproj, &c->noise, &c->nbest, &c->pj_ids, &c->pj_buf, &c->occ_ptr, &c->occ_pos, &c->win, &c->rev,
+                     &c->cproj, &c->noise, &c->nbest, &c->pj_ids, &c->pj_buf, &c->rowerr, &c->rowerr_max, &c->occ_ptr, &c->occ_pos, &c->win, &c->rev,
                      &c->csr_cnt, &c->csr_ptr, &c->csr_cur, &c->csr_idx};
     for (DevBuf* b : all)
         if (b->p) hipFree(b->p);
@@ -402,10 +405,17 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     if (q.precision == SDB_BF16) ENSURE(c->Xb, (size_t)n * ldx * 2);
     ENSURE(c->err_flag, 4);
     HIPCHK(hipMemsetAsync(c->err_flag.p, 0, 4, st));
+    const bool bf = q.precision == SDB_BF16;
+    if (bf) {
+        ENSURE(c->rowerr, (size_t)n * 4);
+        ENSURE(c->rowerr_max, 4);
+        HIPCHK(hipMemsetAsync(c->rowerr_max.p, 0, 4, st));
+    }
     int errf = 0;
     if (!fused) {
-        LCHK(launch_transform(c->X, n, d, ldx, q.metric, P_<float>(c->Xn),
-                              q.precision == SDB_BF16 ? P_<uint16_t>(c->Xb) : nullptr, P_<int>(c->err_flag), st));
+        LCHK(launch_transform(c->X, n, d, ldx, q.metric, P_<float>(c->Xn), bf ? P_<uint16_t>(c->Xb) : nullptr,
+                              bf ? P_<float>(c->rowerr) : nullptr, bf ? P_<uint32_t>(c->rowerr_max) : nullptr,
+                              P_<int>(c->err_flag), st));
         HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
@@ -468,7 +478,9 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ia.X = c->X;
     ia.Xn = P_<float>(c->Xn);
     ia.Xn_out = P_<float>(c->Xn);
-    ia.Xb_out = q.precision == SDB_BF16 ? P_<uint16_t>(c->Xb) : nullptr;
+    ia.Xb_out = bf ? P_<uint16_t>(c->Xb) : nullptr;
+    ia.rowerr = bf ? P_<float>(c->rowerr) : nullptr;
+    ia.rowerr_max = bf ? P_<uint32_t>(c->rowerr_max) : nullptr;
     ia.err = P_<int>(c->err_flag);
     ia.signmask = P_<uint32_t>(c->bits);
     ia.sample_stride = stride;
@@ -492,6 +504,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     LCHK(launch_index(ia, logD, 1, Gf, st));
     int ntie = 0;
     HIPCHK(hipMemcpyAsync(&ntie, c->tie_count.p, 4, hipMemcpyDeviceToHost, st));
+    if (bf) HIPCHK(hipMemcpyAsync(&c->rho_max, c->rowerr_max.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
     LCHK(launch_topm_select(ia.cand_key, ia.cand_idx, ia.cand_cnt, cap, P, M, P_<int>(c->topM), P_<int>(c->flags), st));
     std::vector<int> hflags(2 * P);
@@ -807,7 +820,9 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     if (mfma) {
         const int SE = prec == SDB_BF16 ? 32 : 16;
         da.dk = (c->d + SE - 1) / SE * SE;
-        const int steps = (int)(((kLdsBytes - 8 * M) / M - 16) / 64);
+        // LDS: M rows of steps x 64 B + 16 B, 8 B of ids / counts per
+        // candidate and one word of the list's largest rounding bound
+        const int steps = (int)(((kLdsBytes - 8 * M - 16) / M - 16) / 64);
         if (steps <= 0) return fail(c, SDB_EUNSUPPORTED, "topM too large for the LDS tile");
         da.kc = std::min(da.dk, steps * SE);
         da.nchunk = (da.dk + da.kc - 1) / da.kc;
@@ -862,6 +877,26 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     cosine_cuts(da.lo, da.hi, &da.s_hit, &da.s_near);
     da.magicS = ((1ull << 40) + (unsigned long long)S - 1) / (unsigned long long)S;
     da.n_ent = n * S;
+    // BF16 rows in one LDS tile (the pipelined tiles): a band per pair from
+    // the two rows' rounding bounds.  |s_bf16 - s| <= rho_q + rho_c + gam,
+    // with s the exact dot of the rows the decision is made on and gam the
+    // FP32 accumulation (2 (d + 2) u over unit rows, factor 2 for the unknown
+    // order inside the MFMA unit) plus the rounding of the threshold
+    // arithmetic; the exact FP64 decision is then certain outside the band.
+    if (prec == SDB_BF16 && da.nchunk == 1 && da.kc / 32 <= 25) {
+        const double u = std::ldexp(1.0, -23);
+        const double gam = 2.0 * (c->d + 2.0) * u * 1.016 + 16.0 * u + 1e-9;
+        // hit: Cosine 1 - s < eps; Chi2 (s = BC) 2 (1 - BC) + 4u < eps - 1e-6
+        const double t_hit = prefilter ? 1.0 - ((double)eps - 1e-6) / 2.0 + 2.0 * u : 1.0 - (double)eps;
+        const double t_near = 1.0 - (double)eps;  // not a certain miss
+        float fh = (float)(t_hit + gam), fn = (float)(t_near - gam);
+        if ((double)fh < t_hit + gam) fh = std::nextafter(fh, INFINITY);
+        if ((double)fn > t_near - gam) fn = std::nextafter(fn, -INFINITY);
+        da.rowerr = P_<float>(c->rowerr);
+        da.th_hit = fh;
+        da.th_near = fn;
+        band = 2.0 * (double)c->rho_max + gam;  // the widest pair band
+    }
 
     HIPCHK(hipEventRecord(c->ev[3], st));
     // queued pairs are counted in 64 bits (a pass may meet more than 2^31)
@@ -1462,14 +1497,22 @@ static int ensure_transformed(sdb_ctx* c) {
     if (c->transformed && (q.precision != SDB_BF16 || c->transformed_prec == SDB_BF16)) return 0;
     hipStream_t st = c->st;
     const long long n = c->n;
+    const bool bf = q.precision == SDB_BF16;
     ENSURE(c->Xn, (size_t)n * c->ldx * 4);
-    if (q.precision == SDB_BF16) ENSURE(c->Xb, (size_t)n * c->ldx * 2);
+    if (bf) {
+        ENSURE(c->Xb, (size_t)n * c->ldx * 2);
+        ENSURE(c->rowerr, (size_t)n * 4);
+        ENSURE(c->rowerr_max, 4);
+        HIPCHK(hipMemsetAsync(c->rowerr_max.p, 0, 4, st));
+    }
     ENSURE(c->err_flag, 4);
     HIPCHK(hipMemsetAsync(c->err_flag.p, 0, 4, st));
-    LCHK(launch_transform(c->X, n, c->d, c->ldx, q.metric, P_<float>(c->Xn),
-                          q.precision == SDB_BF16 ? P_<uint16_t>(c->Xb) : nullptr, P_<int>(c->err_flag), st));
+    LCHK(launch_transform(c->X, n, c->d, c->ldx, q.metric, P_<float>(c->Xn), bf ? P_<uint16_t>(c->Xb) : nullptr,
+                          bf ? P_<float>(c->rowerr) : nullptr, bf ? P_<uint32_t>(c->rowerr_max) : nullptr,
+                          P_<int>(c->err_flag), st));
     int errf = 0;
     HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    if (bf) HIPCHK(hipMemcpyAsync(&c->rho_max, c->rowerr_max.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
     if (errf & 2) return fail(c, SDB_EZERO, "Error: There is an zero point !");

@@ -71,6 +71,26 @@ __device__ __forceinline__ uint32_t f32_to_bf16_rne(float f) {
     return r >> 16;
 }
 
+// Bound on the BF16 rounding of one row, for the per-pair error band of the
+// BF16 distance tiles.  sq is the FP32 sum, in any order, of (b_i - s_i)^2
+// over the row, b_i the BF16 rounding of the FP32 value s_i (the differences
+// are exact).  Returns rho >= (1 + 2^-7) ||b - r||, r the exact row the tiles
+// stand for: the transformed row itself (Cosine), or the exact square roots
+// of the transformed Chi2 row, which the correctly rounded FP32 square roots s
+// approach within 2^-24 relative (||s - r|| <= 2^-23 for a unit-sum row).
+// The factor 1.001 covers the FP32 sum of at most 4096 squares, 1.0079 the
+// factor 1 + 2^-7 and the rounding of these few steps.
+__device__ __forceinline__ float bf16_row_err(float sq, bool sqrt_rows) {
+    const float e = sqrtf(sq * 1.001f) + (sqrt_rows ? 0x1p-23f : 0.0f);
+    return e * 1.0079f;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, kWave);
+    return v;
+}
+
 // Cross-lane exchange with lane ^ S without the LDS crossbar: DPP quad
 // permutes for S = 1, 2; two DPP row permutes (half-mirror / mirror, then a
 // quad or half-mirror permute) for S = 4, 8; the gfx950 permlane swaps for

@@ -228,7 +228,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
 // next block is loaded as soon as step st of the current block has been
 // consumed, so the next block's rows are in flight during the MFMAs and the
 // epilogue of the current one.
-//   hits: s > s_hit; band: s_near <= s <= s_hit (queued for the recheck);
+//   per-pair band from the rows' BF16 rounding bounds rho (bf16_row_err),
+//   rho_c the largest over the list's candidates: hit iff s > th_hit +
+//   rho_c + rho_q; queued for the recheck iff not a hit and s >= th_near -
+//   rho_c - rho_q;
 //   E words: per-lane nibbles OR-reduced across the 4 lanes of a query;
 //   colsum: per-lane byte counters (<= 255 blocks per lane and tile).
 __device__ __forceinline__ uint32_t spread4(uint32_t n) {
@@ -272,6 +275,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
     // the list's point ids are requested first, then the first block's query
     // rows, so that both latencies overlap the staging of the candidate rows
     int* pts_lds = col_lds + M;
+    // the largest rounding bound of the list's candidates (float bits)
+    uint32_t* rc_lds = (uint32_t*)(pts_lds + M);
     int pid = 0;
     if (tid < M) pid = Lm[tid];
     // (waves without a block load a valid row they never use, which keeps the
@@ -292,6 +297,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
         col_lds[tid] = 0;
         pts_lds[tid] = pid;
     }
+    if (tid == 0) *rc_lds = 0u;
 
     // candidate rows -> LDS with BATCH 16-byte loads in flight per thread (the
     // list's point ids went through LDS first, so no row load waits on the
@@ -319,7 +325,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             }
         }
     }
+    if (tid < M) atomicMax(rc_lds, __float_as_uint(a.rowerr[pts_lds[tid]]));  // bounds are >= 0
     __syncthreads();
+    // thresholds with the list's largest candidate bound; the query's own
+    // bound is added per block
+    const float rcm = __uint_as_float(*rc_lds);
+    const float th_hit0 = a.th_hit + rcm, th_near0 = a.th_near - rcm;
 
     uint32_t colacc[NT];
 #pragma unroll
@@ -334,6 +345,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             uint32_t Fw[EW];
 #pragma unroll
             for (int x = 0; x < EW; ++x) Fw[x] = (valid && x < a.W) ? a.F[(size_t)ent * a.W + x] : 0u;
+            const float eq = a.rowerr[((unsigned long long)ent * a.magicS) >> 40];
             uint32_t ent2;
             bool valid2;
             entry_of(min(b + 2 * WAVES, nblk - 1), ent2, valid2);
@@ -381,14 +393,17 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             uint32_t hw[EW];
 #pragma unroll
             for (int x = 0; x < EW; ++x) hw[x] = 0u;
+            // hit: s > th_hit + rho_c + rho_q; in the band: not a hit and
+            // s >= th_near - rho_c - rho_q (rho_c: the list's largest bound)
+            const float th_h = th_hit0 + eq, th_n = th_near0 - eq;
             uint32_t unc = 0u;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 uint32_t hr = 0u, nr = 0u;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    hr |= (acc[t][i] > a.s_hit ? 1u : 0u) << i;
-                    nr |= (acc[t][i] >= a.s_near ? 1u : 0u) << i;
+                    hr |= (acc[t][i] > th_h ? 1u : 0u) << i;
+                    nr |= (acc[t][i] >= th_n ? 1u : 0u) << i;
                 }
                 const int base = (t & 1) * 16 + 4 * g;
                 const uint32_t f = (Fw[t >> 1] >> base) & 0xFu;
@@ -448,8 +463,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
 template <int NT, int NS, bool EXACT, int PADB>
 static int launch_pipe_p(const DistArgs& a, int ntiles, hipStream_t st) {
     constexpr int WAVES = 8;
-    const size_t sm = (size_t)a.M * (NS * 64 + PADB) + (size_t)a.M * 8;
+    const size_t sm = (size_t)a.M * (NS * 64 + PADB) + (size_t)a.M * 8 + 16;
     if (sm > (size_t)kLdsBytes) return -2;
+    if (!a.rowerr) return -3;
     hipError_t e = hipFuncSetAttribute((const void*)k_dist_pipe<NT, WAVES, NS, EXACT, PADB>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return (int)e;
@@ -1031,6 +1047,9 @@ __global__ __launch_bounds__(256) void k_recheck(DistArgs a, const int* topK, in
     const int lane = lane_id();
     const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
+    // the wave's counts, added to stats once at the end (one atomic per
+    // counter and wave rather than per pair)
+    unsigned long long c32 = 0, c64 = 0, chit = 0, cnear = 0;
     for (long long e = wv; e < n_entries; e += nwv) {
         const unsigned long long qe = a.queue[e];
         const uint32_t ent = (uint32_t)(qe >> 32);
@@ -1046,23 +1065,31 @@ __global__ __launch_bounds__(256) void k_recheck(DistArgs a, const int* topK, in
             const float d32 = a.metric == kCosine ? 1.0f - dot32_wave(x, y, a.d, lane)
                                                   : 1.0f - 2.0f * chi2sum32_wave(x, y, a.d, lane);
             const double dv = (double)d32;
-            if (lane == 0) atomicAdd(&stats[0], 1ull);
+            ++c32;
             if (dv < (double)a.eps - band32) { hit = true; decided = true; }
             else if (dv > (double)a.eps + band32) { hit = false; decided = true; }
         }
         if (!decided) {
             const double d64 = dist64_wave(x, y, a.d, a.metric, lane);
-            if (lane == 0) atomicAdd(&stats[1], 1ull);
+            ++c64;
             // pairs decided in FP32 (tiles or recheck) lie at least a band from eps,
             // which exceeds 1e-6 except for L1/L2 at small eps * d
-            if (lane == 0 && fabs(d64 - (double)a.eps) < 1e-6) atomicAdd(&stats[4], 1ull);
+            if (fabs(d64 - (double)a.eps) < 1e-6) ++cnear;
             hit = d64 <= (double)a.eps;
         }
-        if (hit && lane == 0) {
-            atomicOr(&a.E[(size_t)ent * a.W + (col >> 5)], 1u << (col & 31));
-            atomicAdd(&a.colsum[(size_t)l * a.M + col], 1);
-            atomicAdd(&stats[2], 1ull);
+        if (hit) {
+            ++chit;
+            if (lane == 0) {
+                atomicOr(&a.E[(size_t)ent * a.W + (col >> 5)], 1u << (col & 31));
+                atomicAdd(&a.colsum[(size_t)l * a.M + col], 1);
+            }
         }
+    }
+    if (lane == 0) {
+        if (c32) atomicAdd(&stats[0], c32);
+        if (c64) atomicAdd(&stats[1], c64);
+        if (chit) atomicAdd(&stats[2], chit);
+        if (cnear) atomicAdd(&stats[4], cnear);
     }
 }
 

@@ -17,13 +17,17 @@ namespace sdb {
 // the FP32 rows (row stride ldx, zero padded) and, when requested, the BF16
 // rows the BF16 distance tiles read (Chi2: BF16 of the square roots, whose
 // dot product bounds chi2 from below).
+// With the BF16 rows it also writes each row's rounding bound rho
+// (bf16_row_err) and their maximum.
 // err bit 1: negative entry (Chi2/JS); err bit 2: zero point.
 __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, long long n, int d,
                                                    int ldx, int metric, float* __restrict__ Xn,
-                                                   uint16_t* __restrict__ Xb, int* __restrict__ err) {
+                                                   uint16_t* __restrict__ Xb, float* __restrict__ rowerr,
+                                                   uint32_t* __restrict__ rowerr_max, int* __restrict__ err) {
     const int lane = lane_id();
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const long long nwaves = (long long)gridDim.x * blockDim.x / kWave;
+    float rmax = 0.0f;
     for (long long p = wave; p < n; p += nwaves) {
         const float* x = X + p * d;
         float scale = 1.0f;
@@ -47,13 +51,26 @@ __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, 
         }
         float* o = Xn + p * ldx;
         uint16_t* ob = Xb ? Xb + p * ldx : nullptr;
+        float sq = 0.0f;
         for (int i = lane; i < ldx; i += kWave) {
             float v = 0.0f;
             if (i < d) v = scaled ? x[i] / scale : x[i];
             o[i] = v;
-            if (ob) ob[i] = (uint16_t)f32_to_bf16_rne(metric == kChi2 ? sqrtf(v) : v);
+            if (ob) {
+                const float sv = metric == kChi2 ? sqrtf(v) : v;
+                const uint32_t h = f32_to_bf16_rne(sv);
+                ob[i] = (uint16_t)h;
+                const float df = __uint_as_float(h << 16) - sv;
+                sq = fmaf(df, df, sq);
+            }
+        }
+        if (ob) {
+            const float rho = bf16_row_err(wave_sum_f32(sq), metric == kChi2);
+            if (lane == 0) rowerr[p] = rho;
+            rmax = fmaxf(rmax, rho);
         }
     }
+    if (Xb && lane == 0) atomicMax(rowerr_max, __float_as_uint(rmax));
 }
 
 // One Walsh-Hadamard level across lanes (logical stride S < 64, mirror lane
@@ -653,6 +670,7 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
     }
     const long long stride = MODE == kSample ? a.sample_stride : 1;
     const long long npts = MODE == kPoints ? a.n_ids : (a.n + stride - 1) / stride;
+    float rho_max = 0.0f;  // largest BF16 rounding bound of the wave's rows
     constexpr bool RAW = (EMB == kEmbCos || EMB == kEmbSqrt) && (MODE == kSample || MODE == kFull);  // reads untransformed rows
     for (long long q = gw; q < npts; q += nw) {
         const long long p = MODE == kPoints ? (long long)a.pt_ids[q] : q * stride;
@@ -712,15 +730,30 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
                         // dot product bounds chi2 from below)
                         const __amdgpu_buffer_rsrc_t rb = row_rsrc(a.Xb_out + p * a.ldx, a.ldx * 2);
                         auto bsrc = [&](int j) { return EMB == kEmbCos ? v[j] : sqrtf(v[j]); };
+                        // with the squared rounding errors of the row (the
+                        // per-pair band of the BF16 tiles); elements past d
+                        // are 0 in both
+                        float sq = 0.0f;
 #pragma unroll
                         for (int j = 0; j + 1 < VPL; j += 2) {
-                            const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){bsrc(j), bsrc(j + 1)}, bf16x2));
+                            const float s0 = bsrc(j), s1 = bsrc(j + 1);
+                            const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){s0, s1}, bf16x2));
                             __builtin_amdgcn_raw_buffer_store_b16((uint16_t)u, rb, 2 * lg + 128 * j, 0, 0);
                             __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(u >> 16), rb, 2 * lg + 128 * (j + 1), 0, 0);
+                            const float d0 = __uint_as_float(u << 16) - s0, d1 = __uint_as_float(u & 0xffff0000u) - s1;
+                            sq = fmaf(d0, d0, sq);
+                            sq = fmaf(d1, d1, sq);
                         }
-                        if constexpr (VPL % 2 == 1)
-                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)f32_to_bf16_rne(bsrc(VPL - 1)), rb,
-                                                                  2 * lg + 128 * (VPL - 1), 0, 0);
+                        if constexpr (VPL % 2 == 1) {
+                            const float s0 = bsrc(VPL - 1);
+                            const uint32_t h = f32_to_bf16_rne(s0);
+                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)h, rb, 2 * lg + 128 * (VPL - 1), 0, 0);
+                            const float d0 = __uint_as_float(h << 16) - s0;
+                            sq = fmaf(d0, d0, sq);
+                        }
+                        const float rho = bf16_row_err(wave_sum_f32(sq), EMB != kEmbCos);
+                        if (lane == 0) a.rowerr[p] = rho;
+                        rho_max = fmaxf(rho_max, rho);
                     }
                 }
             }
@@ -837,6 +870,9 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
                 if (e < P) a.pt_buf[(size_t)e * a.n_ids + q] = v[j];
             }
         }
+    }
+    if constexpr (RAW && MODE == kFull) {
+        if (a.Xb_out && lane == 0) atomicMax(a.rowerr_max, __float_as_uint(rho_max));
     }
     if constexpr (MODE == kSample) {
 #pragma unroll
@@ -1232,11 +1268,13 @@ int launch_rff_embed(const float* Xn, long long n, int d, int ldx, const float* 
 }
 
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
-                     uint16_t* Xb, int* err, hipStream_t st) {
+                     uint16_t* Xb, float* rowerr, uint32_t* rowerr_max, int* err, hipStream_t st) {
+    if (Xb && (!rowerr || !rowerr_max)) return -3;
     long long blocks = (n + 3) / 4;
     if (blocks > 65536) blocks = 65536;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_transform, dim3((unsigned)blocks), dim3(256), 0, st, X, n, d, ldx, metric, Xn, Xb, err);
+    hipLaunchKernelGGL(k_transform, dim3((unsigned)blocks), dim3(256), 0, st, X, n, d, ldx, metric, Xn, Xb, rowerr,
+                       rowerr_max, err);
     return (int)hipGetLastError();
 }
 

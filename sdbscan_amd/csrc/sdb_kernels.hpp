@@ -42,6 +42,8 @@ struct IndexArgs {
     float* pt_buf;        // P x n_ids (mode 3)
     const float* emb;     // n x ldE precomputed L1/L2 embeddings (sample / full passes), or null
     int ldE;
+    float* rowerr;        // n: BF16 rounding bound of each row (full pass with Xb_out), or null
+    uint32_t* rowerr_max; // its maximum (float bits), or null
 };
 
 struct DistArgs {
@@ -71,6 +73,12 @@ struct DistArgs {
     // hit or inside the band iff s >= s_near (equivalent to the band test on
     // fl(1 - s), see cosine_cuts)
     float s_hit, s_near;
+    // BF16 single-chunk tiles: per-pair band from the rows' BF16 rounding
+    // bounds rho (bf16_row_err): hit iff s > th_hit + rho_c + rho_q, hit or
+    // inside the band iff s >= th_near - rho_c - rho_q (rho_c: the largest
+    // bound over the tile's list)
+    const float* rowerr;
+    float th_hit, th_near;
     unsigned long long magicS;  // ceil(2^40 / S): a = (ent * magicS) >> 40
     long long n_ent;            // entries (rows of E and F)
 };
@@ -83,7 +91,7 @@ int index_embedding(int metric, int d, int ker);
 int launch_rff_embed(const float* Xn, long long n, int d, int ldx, const float* R, int half, float* E, int ldE,
                      hipStream_t st);
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
-                     uint16_t* Xb, int* err, hipStream_t st);
+                     uint16_t* Xb, float* rowerr, uint32_t* rowerr_max, int* err, hipStream_t st);
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st);
 int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,
                           uint32_t* thi, uint32_t* tlo, hipStream_t st);
