@@ -199,6 +199,19 @@ int sdb_optics_reach(sdb_ctx* ctx, const int* ordering, float* reach_out);
 int sdb_sample_pair_dist(sdb_ctx* ctx, long long n_samples, unsigned long long seed, int* a_out, int* b_out,
                          float* fast_out, double* exact_out);
 
+/* The same sample without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The sampled pairs are
+ * staged in a static device array of SDB_NOALLOC_MAX_SAMPLES x 20 bytes
+ * (5 MiB) that every GPU the library runs on holds until the process ends;
+ * larger n_samples are refused with SDB_EINVAL.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (several contexts or
+ * threads) race.  Results equal sdb_sample_pair_dist's for the same
+ * arguments. */
+#define SDB_NOALLOC_MAX_SAMPLES (1 << 18)
+int sdb_sample_pair_dist_noalloc(sdb_ctx* ctx, long long n_samples, unsigned long long seed, int* a_out,
+                                 int* b_out, float* fast_out, double* exact_out);
+
 int sdb_get_stats(const sdb_ctx* ctx, sdb_stats* out);
 
 /* Synthetic workloads generated on the device into the context's input

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libsdbscan_amd.so")
 
 COSINE, L1, L2, CHI2, JS = 0, 1, 2, 3, 4
 FP32, BF16 = 0, 1
+NOALLOC_MAX_SAMPLES = 1 << 18  # SDB_NOALLOC_MAX_SAMPLES
 METRICS = {"Cosine": COSINE, "L1": L1, "L2": L2, "Chi2": CHI2, "JS": JS}
 
 
@@ -89,6 +90,7 @@ _SIGS = {
                        ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "sdb_get_stats": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
     "sdb_sample_pair_dist": ([_P, ctypes.c_longlong, ctypes.c_ulonglong, _P, _P, _P, _P], ctypes.c_int),
+    "sdb_sample_pair_dist_noalloc": ([_P, ctypes.c_longlong, ctypes.c_ulonglong, _P, _P, _P, _P], ctypes.c_int),
     "sdb_generate": ([_P, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_ulonglong,
                       ctypes.c_float], ctypes.c_int),
     "sdb_copy_input": ([_P, ctypes.c_longlong, ctypes.c_longlong, _P], ctypes.c_int),
@@ -356,6 +358,18 @@ class Context:
         e = np.zeros(n_samples, np.float64)
         self._chk(self.lib.sdb_sample_pair_dist(self.h, n_samples, seed, a.ctypes.data, b.ctypes.data,
                                                  f.ctypes.data, e.ctypes.data))
+        return a, b, f, e
+
+    def sample_pair_dist_noalloc(self, n_samples, seed=1):
+        """sample_pair_dist through the entry point that allocates no device
+        memory (at most NOALLOC_MAX_SAMPLES pairs)."""
+        import numpy as np
+        a = np.zeros(n_samples, np.int32)
+        b = np.zeros(n_samples, np.int32)
+        f = np.zeros(n_samples, np.float32)
+        e = np.zeros(n_samples, np.float64)
+        self._chk(self.lib.sdb_sample_pair_dist_noalloc(self.h, n_samples, seed, a.ctypes.data, b.ctypes.data,
+                                                         f.ctypes.data, e.ctypes.data))
         return a, b, f, e
 
     def copy_input(self, row0, rows):

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <queue>
 #include <random>
 #include <string>
@@ -66,6 +67,38 @@ int launch_noise_assign(const int* noise, int n_noise, const unsigned long long*
 }  // namespace sdb
 
 using namespace sdb;
+
+// Scratch of sdb_sample_pair_dist_noalloc: 20 bytes per sampled pair (see
+// sample_pairs_into), held by every GPU the library runs on until the process
+// ends.  Host code never uses the name: each device's address comes from
+// hipGetSymbolAddress in sdb_create.  File scope with external linkage (in
+// namespace sdb): a static one is a local symbol of the device code object,
+// which the runtime cannot resolve by name.
+namespace sdb {
+__device__ __align__(256) unsigned char scratch[(size_t)SDB_NOALLOC_MAX_SAMPLES * 20];
+}  // namespace sdb
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+bool g_scratch_done[64] = {};
+
+// Device address of this device's scratch array (the device must be
+// current); resolved once per device.
+hipError_t scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    if (!g_scratch_done[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(sdb::scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = ptr;
+        g_scratch_done[device] = true;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+}  // namespace
 
 struct DevBuf {
     void* p = nullptr;
@@ -128,6 +161,7 @@ struct sdb_ctx {
     float rho_max = 0.f;
     int sng_Ms = 64;
     sdb_stats stats{};
+    void* scratch = nullptr;  // this device's address of the static scratch array
 };
 
 namespace {
@@ -235,8 +269,11 @@ int sdb_create(int device, sdb_ctx** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { (void)hipGetLastError(); return SDB_EHIP; }
     if (device < 0 || device >= ndev) return SDB_EINVAL;
     if (hipSetDevice(device) != hipSuccess) return SDB_EHIP;
+    void* scr = nullptr;
+    if (scratch_address(device, &scr) != hipSuccess) { (void)hipGetLastError(); return SDB_EHIP; }
     sdb_ctx* c = new sdb_ctx();
     c->device = device;
+    c->scratch = scr;
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return SDB_EHIP; }
     for (int i = 0; i < 8; ++i) hipEventCreate(&c->ev[i]);
     *out = c;
@@ -950,12 +987,11 @@ static int run_distances(sdb_ctx* c, float eps, int* nq_out, double* band_out, d
     return r;
 }
 
-int sdb_sample_pair_dist(sdb_ctx* c, long long n_samples, unsigned long long seed, int* pa_out, int* pb_out,
-                         float* fast_out, double* exact_out) {
-    if (!c || n_samples < 0 || (n_samples > 0 && (!pa_out || !pb_out || !fast_out || !exact_out))) return SDB_EINVAL;
-    if (!c->indexed || c->sng) return fail(c, SDB_ESTATE, "build_index first");
-    if (n_samples > (1ll << 26)) return fail(c, SDB_EINVAL, "at most 2^26 sampled pairs per call");
-    HIPCHK(hipSetDevice(c->device));
+// Candidate-pair sample of sdb_sample_pair_dist into the device buffer base
+// (20 bytes per sample: the FP64 values first, 8-byte aligned, then the FP32
+// values and the two index arrays), copied to the caller's host arrays.
+static int sample_pairs_into(sdb_ctx* c, long long n_samples, unsigned long long seed, void* base, int* pa_out,
+                             int* pb_out, float* fast_out, double* exact_out) {
     const sdb_params& q = c->prm;
     hipStream_t st = c->st;
     // the precision the distance tiles run at (as in run_distances_p)
@@ -967,28 +1003,49 @@ int sdb_sample_pair_dist(sdb_ctx* c, long long n_samples, unsigned long long see
     da.ldx = c->ldx; da.d = c->d; da.metric = q.metric;
     da.M = c->Mt;
     da.lists = P_<int>(c->lists);
-    // one allocation: FP64 values first (8-byte aligned), then the FP32
-    // values and the two index arrays
     const size_t ns = (size_t)std::max<long long>(n_samples, 1);
-    void* base = nullptr;
-    if (hipMalloc(&base, ns * 20) != hipSuccess) return fail(c, SDB_ENOMEM, "sample buffer");
     double* ex = (double*)base;
     float* fs = (float*)(ex + ns);
     int* pa = (int*)(fs + ns);
     int* pb = pa + ns;
-    int r = SDB_OK;
     if (launch_pair_sample(da, P_<int>(c->topK), q.topK, q.n_proj, c->n, prec, seed, n_samples, pa, pb, fs, ex, st))
-        r = SDB_EHIP;
-    if (!r && n_samples > 0) {
+        return SDB_EHIP;
+    if (n_samples > 0) {
         if (hipMemcpyAsync(pa_out, pa, (size_t)n_samples * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(pb_out, pb, (size_t)n_samples * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(fast_out, fs, (size_t)n_samples * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipMemcpyAsync(exact_out, ex, (size_t)n_samples * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-            r = SDB_EHIP;
+            hipMemcpyAsync(exact_out, ex, (size_t)n_samples * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+            return SDB_EHIP;
     }
-    (void)hipStreamSynchronize(st);
+    return hipStreamSynchronize(st) == hipSuccess ? SDB_OK : SDB_EHIP;
+}
+
+int sdb_sample_pair_dist(sdb_ctx* c, long long n_samples, unsigned long long seed, int* pa_out, int* pb_out,
+                         float* fast_out, double* exact_out) {
+    if (!c || n_samples < 0 || (n_samples > 0 && (!pa_out || !pb_out || !fast_out || !exact_out))) return SDB_EINVAL;
+    if (!c->indexed || c->sng) return fail(c, SDB_ESTATE, "build_index first");
+    if (n_samples > (1ll << 26)) return fail(c, SDB_EINVAL, "at most 2^26 sampled pairs per call");
+    HIPCHK(hipSetDevice(c->device));
+    void* base = nullptr;
+    if (hipMalloc(&base, (size_t)std::max<long long>(n_samples, 1) * 20) != hipSuccess)
+        return fail(c, SDB_ENOMEM, "sample buffer");
+    const int r = sample_pairs_into(c, n_samples, seed, base, pa_out, pb_out, fast_out, exact_out);
+    (void)hipStreamSynchronize(c->st);
     (void)hipFree(base);
+    if (r) return fail(c, r, "sampled pair distances failed");
+    return SDB_OK;
+}
+
+int sdb_sample_pair_dist_noalloc(sdb_ctx* c, long long n_samples, unsigned long long seed, int* pa_out,
+                                 int* pb_out, float* fast_out, double* exact_out) {
+    if (!c || n_samples < 0 || (n_samples > 0 && (!pa_out || !pb_out || !fast_out || !exact_out))) return SDB_EINVAL;
+    if (!c->indexed || c->sng) return fail(c, SDB_ESTATE, "build_index first");
+    if (n_samples > SDB_NOALLOC_MAX_SAMPLES)
+        return fail(c, SDB_EINVAL, "%lld sampled pairs exceed the %d of the static scratch buffer", n_samples,
+                    SDB_NOALLOC_MAX_SAMPLES);
+    if (!c->scratch) return fail(c, SDB_ESTATE, "no scratch buffer on device %d", c->device);
+    HIPCHK(hipSetDevice(c->device));
+    const int r = sample_pairs_into(c, n_samples, seed, c->scratch, pa_out, pb_out, fast_out, exact_out);
     if (r) return fail(c, r, "sampled pair distances failed");
     return SDB_OK;
 }
