@@ -122,7 +122,7 @@ struct sdb_ctx {
     DevBuf Xn, Xb;
     DevBuf emb;  // L1/L2 embeddings (n x fhtDim) of the index passes
     // index
-    DevBuf bits, R, topK, tie_idx, tie_vals, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
+    DevBuf bits, R, topK, tie_idx, tie_vals, tie_rows, tie_count, cmax, cmin, thi, tlo, cand_cnt, cand_key,
         cand_idx, topM, flags, lists, is_hub, hub_flags, hub_excl, hub_id, hub_cnt, hub_ptr,
         hub_cur, hub_mem, inv, pos_hub, pcnt, poff, pairs, pseg, F, wgcnt, wgoff, grp_ptr, grp, scan_tmp, total, err_flag,
         row_slot, row_buf;
@@ -284,7 +284,7 @@ void sdb_destroy(sdb_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
-    DevBuf* all[] = {&c->Xown, &c->Xn, &c->Xb, &c->emb, &c->bits, &c->R, &c->topK, &c->tie_idx, &c->tie_vals,
+    DevBuf* all[] = {&c->Xown, &c->Xn, &c->Xb, &c->emb, &c->bits, &c->R, &c->topK, &c->tie_idx, &c->tie_vals, &c->tie_rows,
                      &c->tie_count, &c->cmax, &c->cmin, &c->thi, &c->tlo, &c->cand_cnt, &c->cand_key,
                      &c->cand_idx, &c->topM, &c->flags, &c->lists, &c->is_hub, &c->hub_flags,
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
@@ -493,9 +493,13 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     const long long npts = (n + stride - 1) / stride;
     const int Gs = (int)std::max<long long>(1, std::min<long long>(1024, (npts + 3) / 4));
     const int Gf = (int)std::max<long long>(1, std::min<long long>(8192, (n + 3) / 4));
-    const int tie_cap = (int)std::min<long long>(n, 65536);
+    // values kept for up to 65,536 tied points per pass (SDB_TIE_CAP lowers
+    // it); further tied points are projected again in batches of that size
+    long long tie_keep = 65536;
+    if (const char* e = std::getenv("SDB_TIE_CAP")) tie_keep = std::max(1ll, std::min(tie_keep, std::atoll(e)));
+    const int tie_cap = (int)std::min<long long>(n, tie_keep);
     ENSURE(c->topK, (size_t)n * 2 * K * 4);
-    ENSURE(c->tie_idx, (size_t)tie_cap * 4);
+    ENSURE(c->tie_idx, (size_t)n * 4);
     ENSURE(c->tie_vals, (size_t)tie_cap * P * 4);
     ENSURE(c->tie_count, 4);
     ENSURE(c->cmax, (size_t)Gs * P * 4);
@@ -550,17 +554,41 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
     if (errf & 2) return fail(c, SDB_EZERO, "Error: There is an zero point !");
 
-    // ---- host replay of tied top-K points (reference heap semantics)
-    if (ntie > tie_cap) return fail(c, SDB_EUNSUPPORTED, "%d points with tied top-K (capacity %d)", ntie, tie_cap);
+    // ---- host replay of tied top-K points (reference heap semantics).  The
+    // values of the first tie_cap points come with the full pass; the rest
+    // are projected again (point mode) tie_cap at a time.  The replayed rows
+    // go back through one scatter per batch.
     if (ntie > 0) {
         std::vector<int> tidx(ntie);
-        std::vector<float> tv((size_t)ntie * P);
         HIPCHK(hipMemcpy(tidx.data(), c->tie_idx.p, (size_t)ntie * 4, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(tv.data(), c->tie_vals.p, (size_t)ntie * P * 4, hipMemcpyDeviceToHost));
-        std::vector<int> row(2 * K);
-        for (int t = 0; t < ntie; ++t) {
-            heap_select(tv.data() + (size_t)t * P, P, K, row.data(), row.data() + K);
-            HIPCHK(hipMemcpy(P_<int>(c->topK) + (size_t)tidx[t] * 2 * K, row.data(), (size_t)2 * K * 4, hipMemcpyHostToDevice));
+        ENSURE(c->tie_rows, (size_t)tie_cap * 2 * K * 4);
+        std::vector<float> tv((size_t)tie_cap * P);
+        std::vector<float> col(P);
+        std::vector<int> rows((size_t)tie_cap * 2 * K);
+        for (int t0 = 0; t0 < ntie; t0 += tie_cap) {
+            const int nb = std::min(tie_cap, ntie - t0);
+            if (t0 > 0) {
+                IndexArgs ib = ia;
+                ib.pt_ids = P_<int>(c->tie_idx) + t0;
+                ib.n_ids = nb;
+                ib.pt_buf = P_<float>(c->tie_vals);  // P x nb, row e at e * nb
+                const int G = (int)std::max<long long>(1, std::min<long long>(2048, ((long long)nb + 3) / 4));
+                LCHK(launch_index(ib, logD, 3, G, st));
+            }
+            HIPCHK(hipMemcpyAsync(tv.data(), c->tie_vals.p, (size_t)nb * P * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int t = 0; t < nb; ++t) {
+                const float* v = tv.data() + (size_t)t * P;
+                if (t0 > 0) {
+                    for (int e = 0; e < P; ++e) col[e] = tv[(size_t)e * nb + t];
+                    v = col.data();
+                }
+                int* row = rows.data() + (size_t)t * 2 * K;
+                heap_select(v, P, K, row, row + K);
+            }
+            HIPCHK(hipMemcpyAsync(c->tie_rows.p, rows.data(), (size_t)nb * 2 * K * 4, hipMemcpyHostToDevice, st));
+            LCHK(launch_scatter_rows(P_<int>(c->tie_idx) + t0, P_<int>(c->tie_rows), nb, 2 * K, P_<int>(c->topK), st));
+            HIPCHK(hipStreamSynchronize(st));
         }
     }
     c->stats.n_tie_points = ntie;

@@ -802,8 +802,11 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
                 int slot = 0;
                 if (lane == 0) slot = atomicAdd(a.tie_count, 1);
                 slot = __shfl(slot, 0, kWave);
+                // every tied point is listed (tie_idx holds n entries); the
+                // values of the first tie_cap are kept, the others are
+                // projected again for the replay
+                if (lane == 0) a.tie_idx[slot] = (int)p;
                 if (slot < a.tie_cap) {
-                    if (lane == 0) a.tie_idx[slot] = (int)p;
                     float* tv = a.tie_vals + (size_t)slot * P;
 #pragma unroll
                     for (int j = 0; j < VPL; ++j) {
@@ -1332,6 +1335,23 @@ int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t s
     }
 }
 
+
+// rows[i] (width ints) -> dst[idx[i] * width]: the host-replayed top-K rows
+__global__ __launch_bounds__(256) void k_scatter_rows(const int* __restrict__ idx, const int* __restrict__ rows,
+                                                      int count, int width, int* __restrict__ dst) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)count * width) return;
+    const int i = (int)(t / width), w = (int)(t - (long long)i * width);
+    dst[(size_t)idx[i] * width + w] = rows[t];
+}
+
+int launch_scatter_rows(const int* idx, const int* rows, int count, int width, int* dst, hipStream_t st) {
+    if (count <= 0) return 0;
+    const long long tot = (long long)count * width;
+    hipLaunchKernelGGL(k_scatter_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, idx, rows, count, width,
+                       dst);
+    return (int)hipGetLastError();
+}
 
 int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,
                           uint32_t* thi, uint32_t* tlo, hipStream_t st) {

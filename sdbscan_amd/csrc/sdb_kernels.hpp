@@ -24,8 +24,8 @@ struct IndexArgs {
     int P, K;
     int* topK;         // n x 2K (mode 0 output)
     int* tie_count;
-    int tie_cap;
-    int* tie_idx;
+    int tie_cap;       // points whose values are kept
+    int* tie_idx;      // n: every point with a tied top-K
     float* tie_vals;   // tie_cap x P
     uint32_t* chunk_max;  // G x P keys
     uint32_t* chunk_min;
@@ -93,6 +93,7 @@ int launch_rff_embed(const float* Xn, long long n, int d, int ldx, const float* 
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
                      uint16_t* Xb, float* rowerr, uint32_t* rowerr_max, int* err, hipStream_t st);
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st);
+int launch_scatter_rows(const int* idx, const int* rows, int count, int width, int* dst, hipStream_t st);
 int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,
                           uint32_t* thi, uint32_t* tlo, hipStream_t st);
 int launch_topm_select(const uint32_t* key, const int* idx, const int* cnt, int cap, int P, int M,

@@ -218,3 +218,103 @@ def test_set_params_drops_transformed_rows_of_another_metric(ctx):
     assert np.array_equal(ctx.get_transformed().view(np.uint32), Xo.view(np.uint32))
     oc, od, _ = O.sng_find_core(Xo, S.CHI2, seq, 0.3, 5)
     assert np.array_equal(deg, od) and np.array_equal(core, oc)
+
+
+def one_hot_rows(n, d, n_hot, seed):
+    X, _ = datagen.mixture(n, d, k=3, seed=seed)
+    hot = np.random.default_rng(seed).integers(0, d, n_hot)
+    X[:n_hot] = 0.0
+    X[np.arange(n_hot), hot] = 1.0   # +-1 projections: every top-K boundary is tied
+    return X
+
+
+def test_tied_topk_replayed_in_batches(ctx, monkeypatch):
+    """More tied points than the values kept per pass (cap lowered to 64 for
+    the test): the rest are projected again and replayed batch by batch;
+    the index equals the oracle's heaps."""
+    monkeypatch.setenv("SDB_TIE_CAP", "64")
+    X = one_hot_rows(3000, 96, 700, 41)
+    print("tied top-K in batches of 64: n=3000, 700 one-hot rows", flush=True)
+    p, bits, R = setup(ctx, X, n_proj=128, K=8, M=30)
+    st = ctx.stats()
+    assert st["n_tie_points"] > 3 * 64
+    tk, tm = ctx.get_index()
+    proj, Xt = oracle_proj(ctx, p, bits, R, S.COSINE)
+    tko = O.topk(proj, 8)
+    assert np.array_equal(np.sort(tk, axis=1), np.sort(tko, axis=1))
+
+
+def test_more_than_65536_tied_points_match_the_oracle(ctx):
+    """110,000 one-hot points among 112,000, most with a tied top-K boundary
+    (more than the 65,536 whose values one pass keeps): the index (the
+    reference's heap choices among equal values), degrees, cores and labels
+    equal the oracle's."""
+    X = one_hot_rows(112000, 64, 110000, 43)
+    print("tied top-K: n=112000 d=64, 110000 one-hot rows, P=128 K=4 M=20", flush=True)
+    p, bits, R = setup(ctx, X, n_proj=128, K=4, M=20)
+    st = ctx.stats()
+    print("tie points", st["n_tie_points"], "tie rows", st["n_tie_rows"], flush=True)
+    assert st["n_tie_points"] > 65536
+    tk, tm = ctx.get_index()
+    proj, Xt = oracle_proj(ctx, p, bits, R, S.COSINE)
+    tko, tmo = O.topk(proj, 4), O.topm(proj, 20)
+    # heap choices among equal values: the same sets (the order inside a
+    # list is the heap's internal order)
+    assert np.array_equal(np.sort(tk[:, :4], axis=1), np.sort(tko[:, :4], axis=1))
+    assert np.array_equal(np.sort(tk[:, 4:], axis=1), np.sort(tko[:, 4:], axis=1))
+    assert np.array_equal(np.sort(tm[:, :20], axis=1), np.sort(tmo[:, :20], axis=1))
+    assert np.array_equal(np.sort(tm[:, 20:], axis=1), np.sort(tmo[:, 20:], axis=1))
+    core, deg = ctx.find_core(0.3, 10)
+    lab, nc = ctx.form_clusters()
+    oc, od, (ptr, idx, dd), _, _ = O.find_core(Xt, S.COSINE, tk, tm, 0.3, 10, mode=1)
+    olab, onc = O.form_cluster(oc, ptr, idx)
+    assert np.array_equal(deg, od) and np.array_equal(core, oc)
+    assert np.array_equal(lab, olab) and nc == onc
+
+
+def oracle_cosine_index(X, n_proj, K, M, seed=42):
+    Xt, rc = O.transform(X, S.COSINE)
+    assert rc == 0
+    D = S.fht_dim(n_proj, X.shape[1])
+    bits = O.hd3_bits(3 * D, seed)
+    proj = O.project(O.embed(Xt, S.COSINE, X.shape[1], 0.4, None), bits, D, n_proj)
+    return Xt, O.topk(proj, K), O.topm(proj, M)
+
+
+def test_dropin_module_sweep_and_optics_files_match_the_oracle(tmp_path):
+    """The pybind module's test_sDbscan (an eps sweep on one index,
+    src/sDbscan.cpp:980-1039) and fit_sOptics (:1711-1769) with an output
+    prefix: every label file (one label per line) and the OPTICS file
+    ("order reach" per line) carry the oracle's results, under the
+    reference's file names."""
+    m = S.module()
+    X, _ = datagen.mixture(3000, 784, k=6, sigma=0.3, seed=77)
+    out = str(tmp_path / "run")
+    db = m.sDbscan(3000, 784)
+    db.set_params(n_proj=1024, topK=5, topM=50, distance="Cosine", random_seed=42, output=out)
+    eps0, rng_eps, n_tests, minPts = 0.25, 0.05, 3, 20
+    print("module test_sDbscan + fit_sOptics, cosine 3000 x 784", flush=True)
+    db.test_sDbscan(X.T, eps0, rng_eps, n_tests, minPts)
+    Xt, tk, tm = oracle_cosine_index(X, 1024, 5, 50)
+    for i in range(n_tests):
+        e = float(np.float32(eps0 + 1.0 * i * np.float32(rng_eps)))
+        name = f"{out}_Cosine_Eps_{int(np.round(np.float32(1000) * np.float32(e)))}_MinPts_{minPts}" \
+               f"_KerFeatures_784_NumProj_1024_TopM_50_TopK_5"
+        lab = np.loadtxt(name, dtype=np.int64)
+        oc, od, (ptr, idx, dd), _, _ = O.find_core(Xt, S.COSINE, tk, tm, e, minPts, mode=1)
+        olab, onc = O.form_cluster(oc, ptr, idx)
+        assert np.array_equal(lab, olab), e
+    assert np.array_equal(np.array(db.labels_), olab)   # the last sweep step
+    eps = 0.3
+    db.fit_sOptics(X.T, eps, minPts)
+    oc, od, (ptr, idx, dd), _, _ = O.find_core(Xt, S.COSINE, tk, tm, eps, minPts, mode=3)
+    occ, ocd = O.core_dist(minPts, ptr, dd)
+    order, reach = O.optics(oc, ocd, ptr, idx, dd)
+    assert np.array_equal(np.array(db.ordering_), order)
+    assert np.array_equal(np.array(db.reachability_, np.float32), reach)
+    name = f"{out}_Cosine_Eps_300_MinPts_{minPts}_NumEmbed_784_NumProj_1024_TopM_50_TopK_5"
+    rows = [l.split() for l in open(name).read().splitlines()]
+    assert len(rows) == 3000
+    assert [int(r[0]) for r in rows] == order.tolist()
+    # reach printed with the stream's default 6 significant digits
+    assert all(abs(float(r[1]) - float(v)) <= 1e-5 * max(1.0, abs(float(v))) for r, v in zip(rows, reach))
