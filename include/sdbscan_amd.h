@@ -284,6 +284,23 @@ typedef struct sdb_batch_job {
  * gets the wall time of the whole batch. */
 int sdb_batch_run(int device, sdb_batch_job* jobs, int n_jobs, int n_streams, int autotune, double* wall_ms);
 
+/* The same sweeps with the candidate distances of all jobs computed once, in
+ * one persistent launch fed by a device work queue of (job, list, rows)
+ * tiles that decides every eps of a job from the same FP32 MFMA dot products
+ * (no counterpart in the reference, whose test_sDbscan recomputes them per
+ * eps, src/sDbscan.cpp:980-1039).  Band pairs of every eps are re-decided in
+ * FP64 in one launch; each job's degree -> core -> union-find -> labels chain
+ * for all (eps, minPts) runs as one captured hipGraph; sOPTICS as in
+ * sdb_batch_run.  Outputs equal sdb_batch_run's.  Takes Cosine jobs with
+ * FP32 distances, topM <= 128 and at most 16 eps values (SDB_EUNSUPPORTED
+ * otherwise, before any work).  n_streams host threads build the indexes and
+ * run the per-job stages; with autotune set the tile rows of every row length
+ * are picked by timing the sweep kernel.  stage_ms (6 doubles, may be NULL):
+ * index build, tile autotune, distance launch (device time), recheck,
+ * cluster graphs, sOPTICS. */
+int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int n_streams, int autotune, double* stage_ms,
+                    double* wall_ms);
+
 #ifdef __cplusplus
 }
 #endif

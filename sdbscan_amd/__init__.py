@@ -86,6 +86,8 @@ _SIGS = {
     "sdb_set_tile_rows": ([_P, ctypes.c_int], ctypes.c_int),
     "sdb_tune_tile_rows": ([_P, ctypes.c_float, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                             ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "sdb_batch_sweep": ([ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "sdb_batch_run": ([ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                        ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "sdb_get_stats": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
@@ -379,25 +381,16 @@ class Context:
         return out
 
 
-def batch_run(datasets, eps, minPts, optics=True, n_streams=4, autotune=True, device=0, n_proj=1024, topK=10,
-              topM=50, metric=COSINE, seed=42, precision=FP32, hd3=None):
-    """Run sdb_batch_run over a list of (n x d) float32 arrays (Cosine,
-    Chi2 or JS), every dataset swept over the same eps and minPts values.
-    ``hd3(n_bits, seed)`` gives the HD3 sign bytes (default: hd3_bits, the
-    reference's bitHD3Generator).  Returns (results, wall_ms) with one dict
-    per dataset:
-    labels [n_eps, n_minpts, n], n_clusters [n_eps, n_minpts], ordering and
-    reach [n_minpts, n] (optics), tile_rows and ms."""
+def _batch_jobs(datasets, eps, minPts, optics, n_proj, topK, topM, metric, seed, precision, hd3):
     import numpy as np
     if hd3 is None:
         hd3 = hd3_bits
     if metric in (L1, L2):
-        raise ValueError("batch_run builds Cosine / Chi2 / JS jobs; L1 / L2 jobs need an RFF matrix per job")
-    lib = capi()
+        raise ValueError("batch runs build Cosine / Chi2 / JS jobs; L1 / L2 jobs need an RFF matrix per job")
     eps_a = np.ascontiguousarray(eps, dtype=np.float32)
     mp_a = np.ascontiguousarray(minPts, dtype=np.int32)
     jobs = (BatchJob * len(datasets))()
-    keep = []
+    keep = [eps_a, mp_a]
     out = []
     for i, X in enumerate(datasets):
         X = np.ascontiguousarray(X, dtype=np.float32)
@@ -421,10 +414,11 @@ def batch_run(datasets, eps, minPts, optics=True, n_streams=4, autotune=True, de
         j.labels, j.n_clusters = lab.ctypes.data, ncl.ctypes.data
         j.ordering, j.reach = order.ctypes.data, reach.ctypes.data
         out.append({"labels": lab, "n_clusters": ncl, "ordering": order, "reach": reach})
-    wall = ctypes.c_double()
-    rc = lib.sdb_batch_run(device, ctypes.cast(jobs, ctypes.c_void_p), len(datasets), n_streams,
-                           1 if autotune else 0, ctypes.byref(wall))
-    for i in range(len(datasets)):
+    return jobs, keep, out
+
+
+def _batch_results(jobs, out, rc):
+    for i in range(len(out)):
         out[i]["status"] = jobs[i].status
         out[i]["error"] = jobs[i].error.decode()
         out[i]["tile_rows"] = jobs[i].tile_rows
@@ -432,4 +426,37 @@ def batch_run(datasets, eps, minPts, optics=True, n_streams=4, autotune=True, de
     if rc != 0:
         bad = [o["error"] for o in out if o["status"] != 0]
         raise SdbError(rc, "; ".join(bad[:3]))
-    return out, wall.value
+    return out
+
+
+def batch_run(datasets, eps, minPts, optics=True, n_streams=4, autotune=True, device=0, n_proj=1024, topK=10,
+              topM=50, metric=COSINE, seed=42, precision=FP32, hd3=None):
+    """Run sdb_batch_run over a list of (n x d) float32 arrays (Cosine,
+    Chi2 or JS), every dataset swept over the same eps and minPts values.
+    ``hd3(n_bits, seed)`` gives the HD3 sign bytes (default: hd3_bits, the
+    reference's bitHD3Generator).  Returns (results, wall_ms) with one dict
+    per dataset:
+    labels [n_eps, n_minpts, n], n_clusters [n_eps, n_minpts], ordering and
+    reach [n_minpts, n] (optics), tile_rows and ms."""
+    jobs, keep, out = _batch_jobs(datasets, eps, minPts, optics, n_proj, topK, topM, metric, seed, precision, hd3)
+    wall = ctypes.c_double()
+    rc = capi().sdb_batch_run(device, ctypes.cast(jobs, ctypes.c_void_p), len(datasets), n_streams,
+                              1 if autotune else 0, ctypes.byref(wall))
+    return _batch_results(jobs, out, rc), wall.value
+
+
+SWEEP_STAGES = ("index", "autotune", "tiles", "recheck", "clusters", "optics")
+
+
+def batch_sweep(datasets, eps, minPts, optics=True, n_streams=4, autotune=True, device=0, n_proj=1024, topK=10,
+                topM=50, seed=42, hd3=None):
+    """The same sweep through sdb_batch_sweep: the candidate distances of all
+    datasets in one persistent launch deciding every eps, the cluster chains
+    as captured graphs (Cosine, FP32).  Returns (results, wall_ms, stages)
+    with stages a dict of per-stage milliseconds (SWEEP_STAGES)."""
+    jobs, keep, out = _batch_jobs(datasets, eps, minPts, optics, n_proj, topK, topM, COSINE, seed, FP32, hd3)
+    wall = ctypes.c_double()
+    stages = (ctypes.c_double * 6)()
+    rc = capi().sdb_batch_sweep(device, ctypes.cast(jobs, ctypes.c_void_p), len(datasets), n_streams,
+                                1 if autotune else 0, stages, ctypes.byref(wall))
+    return _batch_results(jobs, out, rc), wall.value, dict(zip(SWEEP_STAGES, list(stages)))

@@ -14,7 +14,7 @@ import sysconfig
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["sdb_index.hip", "sdb_dist.hip", "sdb_cluster.hip", "sdb_optics.hip", "sdb_noise.hip", "sdb_sng.hip", "sdb_capi.hip", "sdb_batch.hip"]
+HIP_SOURCES = ["sdb_index.hip", "sdb_dist.hip", "sdb_cluster.hip", "sdb_optics.hip", "sdb_noise.hip", "sdb_sng.hip", "sdb_capi.hip", "sdb_batch.hip", "sdb_sweep.hip", "sdb_sweep_kernels.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SDB_OFFLOAD_ARCH", "gfx950")
 

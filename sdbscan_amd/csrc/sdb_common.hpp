@@ -65,6 +65,15 @@ __device__ __forceinline__ int load_relaxed(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Point index a = ent / S of an entry ent = a * S + s (ent < 2^31, S < 512)
+// with magic = ceil(2^40 / S): bits 40 and up of the full product (a 64-bit
+// product would overflow once ent >= S * 2^24, i.e. from 2^24 points on).
+__device__ __forceinline__ unsigned long long ent_point(uint32_t ent, unsigned long long magic) {
+    const unsigned long long lo = (unsigned long long)ent * magic;
+    const unsigned long long hi = __umul64hi((unsigned long long)ent, magic);
+    return (hi << 24) | (lo >> 40);
+}
+
 __device__ __forceinline__ uint32_t f32_to_bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
     uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
@@ -89,6 +98,30 @@ __device__ __forceinline__ float wave_sum_f32(float v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, kWave);
     return v;
+}
+
+// FP64 distance of two FP32 rows in a fixed order: per lane a sequential sum
+// over k = lane, lane + 64, ..., then the xor butterfly (offsets 32..1);
+// every lane returns the same value.  The decision value of the recheck.
+__device__ __forceinline__ double dist64_wave(const float* x, const float* y, int d, int metric, int lane) {
+    double s = 0.0;
+    for (int i = lane; i < d; i += 64) {
+        const double xv = x[i], yv = y[i];
+        if (metric == kCosine) s = s + xv * yv;
+        else if (metric == kL1) s = s + fabs(xv - yv);
+        else if (metric == kL2) { double t = xv - yv; s = s + t * t; }
+        else if (metric == kChi2) { if (x[i] > 0.0f && y[i] > 0.0f) s = s + (xv * yv) / (xv + yv); }
+        else {
+            const double xe = xv + 1e-6, ye = yv + 1e-6, t = xe + ye, ln2 = 0.69314718055994530942;
+            s = s + (xe * (log(t / xe) / ln2) + ye * (log(t / ye) / ln2));
+        }
+    }
+    s = wave_sum_f64_tree(s);
+    if (metric == kCosine) return 1.0 - s;
+    if (metric == kL1) return s;
+    if (metric == kL2) return sqrt(s);
+    if (metric == kChi2) return 1.0 - 2.0 * s;
+    return 1.0 - s / 2.0;
 }
 
 // Cross-lane exchange with lane ^ S without the LDS crossbar: DPP quad

@@ -30,6 +30,23 @@ __device__ __forceinline__ float chi2_term(float x, float y) {
     return s > 0x1p-100f ? x * y * __builtin_amdgcn_rcpf(s) : 0.0f;
 }
 
+// Reserves cnt slots per lane of a wave-wide append with one atomic: returns
+// the lane's first slot (an inclusive prefix sum over the lanes, the total
+// added to *counter by the last lane).
+__device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* counter, int cnt, int lane) {
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += y;
+    }
+    unsigned long long base = 0;
+    if (lane == kWave - 1 && incl > 0) base = atomicAdd(counter, (unsigned long long)incl);
+    const uint32_t lo = __shfl((uint32_t)base, kWave - 1, kWave);
+    const uint32_t hi = __shfl((uint32_t)(base >> 32), kWave - 1, kWave);
+    return (((unsigned long long)hi << 32) | lo) + (unsigned long long)(incl - cnt);
+}
+
 __device__ __forceinline__ unsigned long long queue_entry(uint32_t ent, int l, int col) {
     return ((unsigned long long)ent << 32) | ((unsigned long long)(uint32_t)l << 8) | (unsigned)col;
 }
@@ -268,7 +285,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
         e = a.grp[v ? rr : beg];
     };
     auto row_of = [&](uint32_t e) {
-        const size_t p = (size_t)(((unsigned long long)e * a.magicS) >> 40);
+        const size_t p = (size_t)ent_point(e, a.magicS);
         return Xc + p * rowb + g * 16;
     };
     auto step_off = [&](int st) { return (size_t)(EXACT ? st : min(st, nsteps - 1)) * 64; };
@@ -345,7 +362,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             uint32_t Fw[EW];
 #pragma unroll
             for (int x = 0; x < EW; ++x) Fw[x] = (valid && x < a.W) ? a.F[(size_t)ent * a.W + x] : 0u;
-            const float eq = a.rowerr[((unsigned long long)ent * a.magicS) >> 40];
+            const float eq = a.rowerr[ent_point(ent, a.magicS)];
             uint32_t ent2;
             bool valid2;
             entry_of(min(b + 2 * WAVES, nblk - 1), ent2, valid2);
@@ -422,12 +439,14 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             for (int x = 1; x < EW; ++x) mine = g == x ? hw[x] : mine;
             if (valid && g < a.W) a.E[(size_t)ent * a.W + g] = mine;
             if (__any(unc != 0u)) {
+                // one queue reservation per wave: lane offsets by a prefix sum
+                unsigned long long pos = wave_reserve(a.qcount, __popc(unc), lane);
                 while (unc) {
                     const int bp = __ffs(unc) - 1;
                     unc &= unc - 1;
                     const int col = (bp >> 2) * 16 + 4 * g + (bp & 3);
-                    const unsigned long long pos = atomicAdd(a.qcount, 1ull);
                     if (pos < (unsigned long long)a.qcap) a.queue[pos] = queue_entry(ent, l, col);
+                    ++pos;
                 }
             }
             if (!has_next) break;
@@ -1000,27 +1019,6 @@ int launch_dist_valu(const DistArgs& a, int ntiles, hipStream_t st) {
 // decided with the FP64 value.
 // stats[0]: FP32 rechecks, stats[1]: FP64 rechecks, stats[2]: hits added,
 // stats[4]: FP64 distances within 1e-6 of eps.
-__device__ __forceinline__ double dist64_wave(const float* x, const float* y, int d, int metric, int lane) {
-    double s = 0.0;
-    for (int i = lane; i < d; i += 64) {
-        const double xv = x[i], yv = y[i];
-        if (metric == kCosine) s = s + xv * yv;
-        else if (metric == kL1) s = s + fabs(xv - yv);
-        else if (metric == kL2) { double t = xv - yv; s = s + t * t; }
-        else if (metric == kChi2) { if (x[i] > 0.0f && y[i] > 0.0f) s = s + (xv * yv) / (xv + yv); }
-        else {
-            const double xe = xv + 1e-6, ye = yv + 1e-6, t = xe + ye, ln2 = 0.69314718055994530942;
-            s = s + (xe * (log(t / xe) / ln2) + ye * (log(t / ye) / ln2));
-        }
-    }
-    s = wave_sum_f64_tree(s);
-    if (metric == kCosine) return 1.0 - s;
-    if (metric == kL1) return s;
-    if (metric == kL2) return sqrt(s);
-    if (metric == kChi2) return 1.0 - 2.0 * s;
-    return 1.0 - s / 2.0;
-}
-
 __device__ __forceinline__ float dot32_wave(const float* x, const float* y, int d, int lane) {
     float s = 0.0f;
     for (int i = lane; i < d; i += 64) s = fmaf(x[i], y[i], s);

@@ -79,9 +79,37 @@ struct DistArgs {
     // bound over the tile's list)
     const float* rowerr;
     float th_hit, th_near;
-    unsigned long long magicS;  // ceil(2^40 / S): a = (ent * magicS) >> 40
+    unsigned long long magicS;  // ceil(2^40 / S): a = ent_point(ent, magicS)
     long long n_ent;            // entries (rows of E and F)
 };
+
+// Batched sweep (sdb_batch_sweep): one job = one dataset's index and its eps
+// values; its FP32 Cosine tiles decide every eps in one pass.
+constexpr int kSweepMaxEps = 16;
+struct SweepJob {
+    const float* X;         // transformed FP32 rows, stride ldx
+    int ldx, d, dk, kc, nchunk;
+    int M, W, S, L2;
+    const uint32_t* grp;    // entries grouped by list
+    const int* lists;       // L2 x M
+    const uint32_t* F;      // first-occurrence masks
+    uint32_t* E;            // n_eps slices of n_ent x W hit words
+    int* colsum;            // n_eps slices of L2 x M
+    long long n_ent;        // n * S
+    unsigned long long magicS;
+    int n_eps;
+    float eps[kSweepMaxEps];                      // ascending
+    float s_hit[kSweepMaxEps], s_near[kSweepMaxEps];  // cuts on the raw dot (cosine_cuts)
+};
+// tiles: int4 (job, list, first group position, one past the last)
+int launch_sweep_tiles(const SweepJob* jobs, int maxM, const int4* tiles, int ntiles, int* head, uint4* queue,
+                       unsigned long long* qcount, long long qcap, size_t lds_bytes, int grid, hipStream_t st);
+int launch_sweep_recheck(const SweepJob* jobs, const uint4* queue, long long nq, unsigned long long* stats,
+                         hipStream_t st);
+int launch_fill_u32(uint32_t* p, long long count, uint32_t value, hipStream_t st);
+// exclusive scan without library calls (graph-capturable); bsum: ceil(n / 1024) ints
+int scan_exclusive_i32_graph(const int* in, int* out, long long n, int* total, int* bsum, hipStream_t st);
+int launch_copy_i32(const int* src, int* dst, long long count, hipStream_t st);
 
 // index
 // embedding kind of the index kernels for (metric, n_features, ker_n_features):

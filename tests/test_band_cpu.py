@@ -97,3 +97,19 @@ def test_bound_catches_a_dropped_term():
         bare, _ = rho(S[None, :], False)
         fails += dev > bare[0]
     assert fails == 100
+
+
+def test_entry_to_point_division_magic():
+    """a = ent / S by the 40-bit magic ceil(2^40 / S) used on the device:
+    exact for every S up to 128 and entries up to 2^31 (the full product's
+    bits 40 and up; its low 64 bits alone overflow from S * 2^24 on)."""
+    rng = np.random.default_rng(0)
+    for S in list(range(2, 129)):
+        magic = ((1 << 40) + S - 1) // S
+        ents = [int(x) for x in rng.integers(0, 1 << 31, 2000)] + [(1 << 31) - 1, S * (1 << 24), S * (1 << 24) - 1]
+        ents += [S * a + s for a in (0, 1, (1 << 31) // S - 1) for s in range(S) if S * a + s < (1 << 31)]
+        for e in ents:
+            assert (e * magic) >> 40 == e // S, (S, e)
+    S = 20
+    e = S * (1 << 24) + 3
+    assert ((e * (((1 << 40) + S - 1) // S)) & ((1 << 64) - 1)) >> 40 != e // S   # 64-bit product wraps
