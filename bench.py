@@ -213,8 +213,11 @@ def batch_bench(args):
     and d in {32, 128, 784} (3-30-component Gaussian mixtures on the unit
     sphere), each swept over 8 eps values in [0.1, 0.4] x minPts {10, 50}
     with sOPTICS at the largest eps, FP32 distances.  One JSON line: the
-    batched run (several streams, autotuned tile rows) against the same jobs
-    one dataset at a time (one stream, default tiles)."""
+    one-launch sweep (sdb_batch_sweep: every dataset's candidate distances in
+    one persistent launch deciding all eps, cluster chains as captured
+    graphs, autotuned tile rows) against the same jobs one dataset at a time
+    (sdb_batch_run, one stream, one distance pass per eps, default tiles) and
+    on a pool of streams (sdb_batch_run, autotuned)."""
     import sdbscan_amd as S
     from sdbscan_amd import datagen
     rng = np.random.default_rng(1003)
@@ -230,22 +233,47 @@ def batch_bench(args):
     log(f"[batch] generated {nds} datasets, {int(ns.sum())} points in {time.perf_counter() - t0:.1f} s")
     kw = dict(optics=True, n_proj=P, topK=10, topM=50, hd3=S.hd3_bits)
     S.batch_run(data[:2], eps, minPts, n_streams=2, autotune=False, **kw)  # warm-up
+    S.batch_sweep(data[:2], eps, minPts, n_streams=2, autotune=False, **kw)
     r1, wall1 = S.batch_run(data, eps, minPts, n_streams=1, autotune=False, **kw)
     log(f"[batch] one dataset at a time: {wall1:.0f} ms")
     rb, wallb = S.batch_run(data, eps, minPts, n_streams=args.streams, autotune=True, **kw)
-    log(f"[batch] {args.streams} streams, autotuned tiles: {wallb:.0f} ms; tile rows "
-        f"{sorted(set(r['tile_rows'] for r in rb))}")
-    same = all(np.array_equal(a["labels"], b["labels"]) and np.array_equal(a["ordering"], b["ordering"])
-               and np.array_equal(a["reach"], b["reach"]) for a, b in zip(r1, rb))
+    log(f"[batch] {args.streams} streams, autotuned tiles: {wallb:.0f} ms")
+    rs, walls, stages = S.batch_sweep(data, eps, minPts, n_streams=args.streams, autotune=True, **kw)
+    log(f"[batch] one-launch sweep: {walls:.0f} ms, stages "
+        f"{ {k: round(v, 1) for k, v in stages.items()} }, tile rows {sorted(set(r['tile_rows'] for r in rs))}")
+
+    # the same three runs without sOPTICS (whose host walks dominate the wall
+    # time): the part of the sweep the one-launch design changes
+    kn = dict(kw, optics=False)
+    n1, nwall1 = S.batch_run(data, eps, minPts, n_streams=1, autotune=False, **kn)
+    nb, nwallb = S.batch_run(data, eps, minPts, n_streams=args.streams, autotune=True, **kn)
+    nsw, nwalls, nstages = S.batch_sweep(data, eps, minPts, n_streams=args.streams, autotune=True, **kn)
+    log(f"[batch] without sOPTICS: one at a time {nwall1:.0f} ms, {args.streams} streams {nwallb:.0f} ms, "
+        f"one-launch sweep {nwalls:.0f} ms ({ {k: round(v, 1) for k, v in nstages.items()} })")
+
+    def same(x, y):
+        return all(np.array_equal(a["labels"], b["labels"]) and np.array_equal(a["n_clusters"], b["n_clusters"])
+                   and np.array_equal(a["ordering"], b["ordering"]) and np.array_equal(a["reach"], b["reach"])
+                   for a, b in zip(x, y))
     tot = int(ns.sum())
     print(json.dumps({
-        "metric": "points/sec, 64-dataset sweep (8 eps x 2 minPts + sOPTICS), FP32", "value": tot / (wallb * 1e-3),
+        "metric": "points/sec, 64-dataset sweep (8 eps x 2 minPts + sOPTICS), FP32", "value": tot / (walls * 1e-3),
         "unit": "points/sec", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"{nds} datasets, n log-uniform [2000, 500000] (sum {tot}), d in {{32,128,784}}, "
                                "eps 8 x U[0.1, 0.4], minPts {10, 50}, sOPTICS at max eps, n_proj 1024, topK 10, "
-                               "topM 50", "streams": args.streams},
-        "wall_ms": wallb, "one_by_one_wall_ms": wall1, "one_by_one_points_per_sec": tot / (wall1 * 1e-3),
-        "speedup_vs_one_by_one": wall1 / wallb, "outputs_identical": same}))
+                               "topM 50; one persistent distance launch over all datasets, cluster chains as "
+                               "captured hipGraphs", "streams": args.streams},
+        "wall_ms": walls, "stages_ms": stages, "distance_launches": 1,
+        "one_by_one_wall_ms": wall1, "one_by_one_points_per_sec": tot / (wall1 * 1e-3),
+        "one_by_one_distance_launches": nds * len(eps),
+        "speedup_vs_one_by_one": wall1 / walls,
+        "stream_pool_wall_ms": wallb, "speedup_vs_stream_pool": wallb / walls,
+        "without_optics": {"one_by_one_wall_ms": nwall1, "stream_pool_wall_ms": nwallb, "sweep_wall_ms": nwalls,
+                           "sweep_stages_ms": nstages, "speedup_vs_one_by_one": nwall1 / nwalls,
+                           "speedup_vs_stream_pool": nwallb / nwalls},
+        "outputs_identical": same(r1, rs) and same(rb, rs) and all(
+            np.array_equal(a["labels"], b["labels"]) and np.array_equal(a["labels"], c["labels"])
+            for a, b, c in zip(n1, nb, nsw))}), flush=True)
 
 
 def main():

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <queue>
 #include <random>
@@ -443,10 +444,11 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     if (ntie > 0) {
         std::vector<int> tidx(ntie);
         HIPCHK(hipMemcpy(tidx.data(), c->tie_idx.p, (size_t)ntie * 4, hipMemcpyDeviceToHost));
-        ENSURE(c->tie_rows, (size_t)tie_cap * 2 * K * 4);
-        std::vector<float> tv((size_t)tie_cap * P);
+        const int nbmax = std::min(ntie, tie_cap);
+        ENSURE(c->tie_rows, (size_t)nbmax * 2 * K * 4);
+        std::vector<float> tv((size_t)nbmax * P);
         std::vector<float> col(P);
-        std::vector<int> rows((size_t)tie_cap * 2 * K);
+        std::vector<int> rows((size_t)nbmax * 2 * K);
         for (int t0 = 0; t0 < ntie; t0 += tie_cap) {
             const int nb = std::min(tie_cap, ntie - t0);
             if (t0 > 0) {
@@ -1287,9 +1289,11 @@ int sdb_optics_orders(sdb_ctx* c, int n_minpts, const int* minPts, int* ordering
     std::vector<long long> ptr(n + 1);
     int r = sdb_get_neighbors(c, ptr.data(), nullptr, nullptr);
     if (r) return r;
-    std::vector<int> idx(std::max<long long>(ptr[n], 1));
-    std::vector<float> dd(std::max<long long>(ptr[n], 1));
-    r = sdb_get_neighbors(c, ptr.data(), idx.data(), dd.data());
+    std::vector<int> idx_v(std::max<long long>(ptr[n], 1));
+    std::vector<float> dd_v(std::max<long long>(ptr[n], 1));
+    int* idx = idx_v.data();
+    float* dd = dd_v.data();
+    r = sdb_get_neighbors(c, ptr.data(), idx, dd);
     if (r) return r;
     std::vector<std::vector<uint8_t>> core(n_minpts, std::vector<uint8_t>(n));
     std::vector<std::vector<float>> cdist(n_minpts, std::vector<float>(n));
@@ -1306,9 +1310,9 @@ int sdb_optics_orders(sdb_ctx* c, int n_minpts, const int* minPts, int* ordering
     // the walks are serial each; the settings run side by side
     std::vector<std::thread> th;
     for (int m = 1; m < n_minpts; ++m)
-        th.emplace_back(optics_walk, n, ptr.data(), idx.data(), dd.data(), core[m].data(), cdist[m].data(),
+        th.emplace_back(optics_walk, n, ptr.data(), idx, dd, core[m].data(), cdist[m].data(),
                         orderings + (size_t)m * n);
-    optics_walk(n, ptr.data(), idx.data(), dd.data(), core[0].data(), cdist[0].data(), orderings);
+    optics_walk(n, ptr.data(), idx, dd, core[0].data(), cdist[0].data(), orderings);
     for (auto& t : th) t.join();
     if (reaches) {
         for (int m = 0; m < n_minpts; ++m) {
