@@ -623,7 +623,9 @@ __global__ __launch_bounds__(256) void k_rff_embed(const float* __restrict__ Xn,
     }
 }
 
-template <int LOGD, int MODE, int EMB>
+// FULLP: n_proj == fhtDim (every element of a lane's registers is a
+// projection), which folds the ragged-end tests away at compile time.
+template <int LOGD, int MODE, int EMB, bool FULLP>
 // Occupancy: 3 workgroups per CU up to D = 1024 (at most 168 registers),
 // fewer for the longer rows so that their registers do not spill.
 __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_index(IndexArgs a) {
@@ -634,7 +636,7 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
     const int wib = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     const long long gw = (long long)blockIdx.x * 4 + wib;
     const long long nw = (long long)gridDim.x * 4;
-    const int P = a.P;
+    const int P = FULLP ? D : a.P;
     constexpr int NW = (VPL + 31) / 32;
 
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1281,17 +1283,25 @@ int launch_transform(const float* X, long long n, int d, int ldx, int metric, fl
     return (int)hipGetLastError();
 }
 
-template <int LOGD, int MODE, int EMB>
-static int launch_index_t(const IndexArgs& a, int grid, hipStream_t st) {
+template <int LOGD, int MODE, int EMB, bool FULLP>
+static int launch_index_f(const IndexArgs& a, int grid, hipStream_t st) {
     constexpr int D = 1 << LOGD;
     size_t sm = (MODE == kFull ? (size_t)4 * 2048 + (size_t)8 * D : (size_t)8 * a.P) + (EMB == kEmbLds ? (size_t)4 * D * 4 : 0);
     if (sm > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE, EMB>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE, EMB, FULLP>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_index<LOGD, MODE, EMB>), dim3(grid), dim3(256), sm, st, a);
+    hipLaunchKernelGGL((k_index<LOGD, MODE, EMB, FULLP>), dim3(grid), dim3(256), sm, st, a);
     return (int)hipGetLastError();
+}
+
+template <int LOGD, int MODE, int EMB>
+static int launch_index_t(const IndexArgs& a, int grid, hipStream_t st) {
+    // the bulk passes with n_proj == fhtDim take the folded variant
+    if constexpr (MODE == kSample || MODE == kFull)
+        if (a.P == (1 << LOGD)) return launch_index_f<LOGD, MODE, EMB, true>(a, grid, st);
+    return launch_index_f<LOGD, MODE, EMB, false>(a, grid, st);
 }
 
 template <int LOGD, int MODE>
