@@ -214,8 +214,8 @@ def batch_bench(args):
     sphere), each swept over 8 eps values in [0.1, 0.4] x minPts {10, 50}
     with sOPTICS at the largest eps, FP32 distances.  One JSON line: the
     one-launch sweep (sdb_batch_sweep: every dataset's candidate distances in
-    one persistent launch deciding all eps, cluster chains as captured
-    graphs, autotuned tile rows) against the same jobs one dataset at a time
+    one persistent launch deciding all eps, the cluster chains of all sets
+    one launch per step in one captured graph, autotuned tile rows) against the same jobs one dataset at a time
     (sdb_batch_run, one stream, one distance pass per eps, default tiles) and
     on a pool of streams (sdb_batch_run, autotuned)."""
     import sdbscan_amd as S
@@ -261,8 +261,8 @@ def batch_bench(args):
         "unit": "points/sec", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"{nds} datasets, n log-uniform [2000, 500000] (sum {tot}), d in {{32,128,784}}, "
                                "eps 8 x U[0.1, 0.4], minPts {10, 50}, sOPTICS at max eps, n_proj 1024, topK 10, "
-                               "topM 50; one persistent distance launch over all datasets, cluster chains as "
-                               "captured hipGraphs", "streams": args.streams},
+                               "topM 50; one persistent distance launch over all datasets, the cluster chains of "
+                               "all (dataset, eps, minPts) sets one launch per step in one captured hipGraph", "streams": args.streams},
         "wall_ms": walls, "stages_ms": stages, "distance_launches": 1,
         "one_by_one_wall_ms": wall1, "one_by_one_points_per_sec": tot / (wall1 * 1e-3),
         "one_by_one_distance_launches": nds * len(eps),

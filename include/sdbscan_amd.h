@@ -289,15 +289,17 @@ int sdb_batch_run(int device, sdb_batch_job* jobs, int n_jobs, int n_streams, in
  * tiles that decides every eps of a job from the same FP32 MFMA dot products
  * (no counterpart in the reference, whose test_sDbscan recomputes them per
  * eps, src/sDbscan.cpp:980-1039).  Band pairs of every eps are re-decided in
- * FP64 in one launch; each job's degree -> core -> union-find -> labels chain
- * for all (eps, minPts) runs as one captured hipGraph; sOPTICS as in
- * sdb_batch_run.  Outputs equal sdb_batch_run's.  Takes Cosine jobs with
+ * FP64 in one launch; the degree -> core -> union-find -> labels chain of
+ * every (job, eps, minPts) set runs one launch per step over all sets, the
+ * steps captured as one hipGraph; sOPTICS as in sdb_batch_run.  Outputs equal sdb_batch_run's.  Takes Cosine jobs with
  * FP32 distances, topM <= 128 and at most 16 eps values (SDB_EUNSUPPORTED
  * otherwise, before any work).  n_streams host threads build the indexes and
  * run the per-job stages; with autotune set the tile rows of every row length
- * are picked by timing the sweep kernel.  stage_ms (6 doubles, may be NULL):
- * index build, tile autotune, distance launch (device time), recheck,
- * cluster graphs, sOPTICS. */
+ * are picked by timing the sweep kernel.  stage_ms (SDB_SWEEP_STAGES doubles,
+ * may be NULL), host milliseconds unless noted: context set-up, index build,
+ * tile autotune, distance launch (device time), recheck, cluster buffers,
+ * chain capture, chain run, label copy-out, sOPTICS, teardown. */
+#define SDB_SWEEP_STAGES 11
 int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int n_streams, int autotune, double* stage_ms,
                     double* wall_ms);
 

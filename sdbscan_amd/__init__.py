@@ -445,18 +445,20 @@ def batch_run(datasets, eps, minPts, optics=True, n_streams=4, autotune=True, de
     return _batch_results(jobs, out, rc), wall.value
 
 
-SWEEP_STAGES = ("index", "autotune", "tiles", "recheck", "clusters", "optics")
+SWEEP_STAGES = ("setup", "index", "autotune", "tiles", "recheck", "cluster_buffers", "capture", "chain", "copy_out",
+                "optics", "teardown")
 
 
 def batch_sweep(datasets, eps, minPts, optics=True, n_streams=4, autotune=True, device=0, n_proj=1024, topK=10,
                 topM=50, seed=42, hd3=None):
     """The same sweep through sdb_batch_sweep: the candidate distances of all
     datasets in one persistent launch deciding every eps, the cluster chains
-    as captured graphs (Cosine, FP32).  Returns (results, wall_ms, stages)
+    of all (dataset, eps, minPts) sets one launch per step, captured as one
+    graph (Cosine, FP32).  Returns (results, wall_ms, stages)
     with stages a dict of per-stage milliseconds (SWEEP_STAGES)."""
     jobs, keep, out = _batch_jobs(datasets, eps, minPts, optics, n_proj, topK, topM, COSINE, seed, FP32, hd3)
     wall = ctypes.c_double()
-    stages = (ctypes.c_double * 6)()
+    stages = (ctypes.c_double * len(SWEEP_STAGES))()
     rc = capi().sdb_batch_sweep(device, ctypes.cast(jobs, ctypes.c_void_p), len(datasets), n_streams,
                                 1 if autotune else 0, stages, ctypes.byref(wall))
     return _batch_results(jobs, out, rc), wall.value, dict(zip(SWEEP_STAGES, list(stages)))

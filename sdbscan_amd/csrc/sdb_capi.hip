@@ -179,8 +179,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
                      &c->cproj, &c->noise, &c->nbest, &c->pj_ids, &c->pj_buf, &c->rowerr, &c->rowerr_max, &c->sw_E, &c->sw_colsum, &c->sw_labels, &c->sw_comps, &c->occ_ptr, &c->occ_pos, &c->win, &c->rev,
                      &c->csr_cnt, &c->csr_ptr, &c->csr_cur, &c->csr_idx};
-    for (DevBuf* b : all)
-        if (b->p) hipFree(b->p);
+    for (DevBuf* b : all) release(*b);
     for (int i = 0; i < 8; ++i) hipEventDestroy(c->ev[i]);
     hipStreamDestroy(c->st);
     delete c;
@@ -1106,7 +1105,7 @@ static int build_neighbors(sdb_ctx* c) {
                                                                   P_<long long>(c->nb_ptr), P_<long long>(c->nb_ptr) + 1, 0, 32, st);
         HIPCHK(hipMemcpyAsync(c->nb_idx.p, keys_out, (size_t)tot * 4, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
-        hipFree(tmp.p);
+        release(tmp);
         if (e != hipSuccess) return fail(c, SDB_EHIP, "segmented sort failed: %s", hipGetErrorString(e));
         LCHK(launch_csr_dist(P_<float>(c->Xn), c->ldx, c->d, c->prm.metric, n, P_<long long>(c->nb_ptr),
                              P_<int>(c->nb_idx), P_<float>(c->nb_dist), st));
@@ -1153,7 +1152,7 @@ int sdb_core_dist(sdb_ctx* c, int minPts, float* core_dist_out) {
         hipError_t e = hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, tb, P_<float>(c->nb_dist), sorted, (int)tot, (int)n,
                                                                   P_<long long>(c->nb_ptr), P_<long long>(c->nb_ptr) + 1, 0, 32, st);
         HIPCHK(hipStreamSynchronize(st));
-        hipFree(tmp.p);
+        release(tmp);
         if (e != hipSuccess) return fail(c, SDB_EHIP, "segmented sort failed: %s", hipGetErrorString(e));
         c->sorted_dist = true;
     }
@@ -1432,7 +1431,7 @@ static int build_neighbors_sng(sdb_ctx* c) {
         hipError_t e = hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, tb, P_<int>(c->csr_idx), sorted, (int)tot, (int)n,
                                                                   P_<long long>(c->csr_ptr), P_<long long>(c->csr_ptr) + 1, 0, 32, st);
         HIPCHK(hipStreamSynchronize(st));
-        hipFree(tmp.p);
+        release(tmp);
         if (e != hipSuccess) return fail(c, SDB_EHIP, "segmented sort failed: %s", hipGetErrorString(e));
     }
     // distinct ids per row

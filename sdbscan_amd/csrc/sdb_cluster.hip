@@ -41,9 +41,11 @@ __device__ __forceinline__ void entry_words(const uint32_t* E, size_t t, int W, 
 // block's points are contiguous, so the workgroup reads them as coalesced
 // 16-byte vectors and adds each vector's popcount to its point's LDS counter.
 // |R(a)| and the mutual pairs only exist for points that sit in a list.
-__global__ __launch_bounds__(256) void k_degree(ClusterArgs c) {
+// Each kernel body below takes its ClusterArgs and block index, so that the
+// single-dataset kernels and the batched set kernels (k_sets) share it.
+__device__ __forceinline__ void degree_block(const ClusterArgs& c, long long blk) {
     __shared__ int own[256];
-    const long long a0 = (long long)blockIdx.x * 256;
+    const long long a0 = blk * 256;
     const int npt = (int)min((long long)256, c.n - a0);
     const int SW = c.S * c.W;
     own[threadIdx.x] = 0;
@@ -100,11 +102,12 @@ __global__ __launch_bounds__(256) void k_degree(ClusterArgs c) {
     const unsigned long long b = __ballot(is_core);
     if (lane_id() == 0 && b) atomicAdd(c.n_core, (unsigned long long)__popcll(b));
 }
+__global__ __launch_bounds__(256) void k_degree(ClusterArgs c) { degree_block(c, blockIdx.x); }
 
 // corehub[l][w]: bit i set when the point at position i of list l is core.
-__global__ void k_corehub(ClusterArgs c) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= c.L2 * c.W) return;
+__device__ __forceinline__ void corehub_block(const ClusterArgs& c, long long blk) {
+    const long long t = blk * 256 + threadIdx.x;
+    if (t >= (long long)c.L2 * c.W) return;
     int l = t / c.W, w = t % c.W;
     uint32_t m = 0;
     for (int b = 0; b < 32; ++b) {
@@ -113,6 +116,7 @@ __global__ void k_corehub(ClusterArgs c) {
     }
     c.corehub[t] = m;
 }
+__global__ __launch_bounds__(256) void k_corehub(ClusterArgs c) { corehub_block(c, blockIdx.x); }
 
 // Union-find with parent[x] <= x: roots are hooked under the smaller root, so
 // every component ends with its smallest member as root.
@@ -141,12 +145,13 @@ __device__ __forceinline__ void uf_union(int* par, int a, int b) {
     }
 }
 
-__global__ void k_uf_init(ClusterArgs c) {
-    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void uf_init_block(const ClusterArgs& c, long long blk) {
+    const long long a = blk * 256 + threadIdx.x;
     if (a >= c.n) return;
     c.parent[a] = (int)a;
     c.border_min[a] = 0x7fffffff;
 }
+__global__ __launch_bounds__(256) void k_uf_init(ClusterArgs c) { uf_init_block(c, blockIdx.x); }
 
 // Per core point, slot by slot: all core hits of the slot's list are
 // connected through the point.  The point is joined to the first core hit;
@@ -185,19 +190,20 @@ __device__ __forceinline__ void uf_point(const ClusterArgs& c, long long a, Word
     }
 }
 
-__global__ void k_uf_points(ClusterArgs c) {
-    const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void uf_points_block(const ClusterArgs& c, long long blk) {
+    const long long a = blk * 256 + threadIdx.x;
     if (a >= c.n || !c.core[a]) return;
     uf_point(c, a, [&](int s, uint32_t (&m)[kMaxW]) { entry_words(c.E, (size_t)a * c.S + s, c.W, m); });
 }
+__global__ __launch_bounds__(256) void k_uf_points(ClusterArgs c) { uf_points_block(c, blockIdx.x); }
 
 // W = 4: the E rows of the workgroup's 128 points are staged through LDS with
 // coalesced 16-byte loads, instead of each thread striding through its own
 // 16 S bytes in global memory across the slot loop.
-__global__ __launch_bounds__(128) void k_uf_points_staged(ClusterArgs c) {
+__device__ __forceinline__ void uf_points_staged_block(const ClusterArgs& c, long long blk) {
     extern __shared__ uint4 erow[];
     const int S = c.S;
-    const long long a0 = (long long)blockIdx.x * 128;
+    const long long a0 = blk * 128;
     const int npt = (int)min((long long)128, c.n - a0);
     const uint4* Eb = (const uint4*)c.E + (size_t)a0 * S;
     for (int v = threadIdx.x; v < npt * S; v += 128) erow[v] = Eb[v];
@@ -212,10 +218,12 @@ __global__ __launch_bounds__(128) void k_uf_points_staged(ClusterArgs c) {
         for (int w = 4; w < kMaxW; ++w) m[w] = 0u;
     });
 }
+__global__ __launch_bounds__(128) void k_uf_points_staged(ClusterArgs c) { uf_points_staged_block(c, blockIdx.x); }
 
-__global__ void k_uf_lists(ClusterArgs c) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= c.L2 * c.M) return;
+__device__ __forceinline__ void uf_lists_block(const ClusterArgs& c, long long blk) {
+    const long long tl = blk * 256 + threadIdx.x;
+    if (tl >= (long long)c.L2 * c.M) return;
+    const int t = (int)tl;
     const int l = t / c.M;
     const int f = c.lists[t];
     for (int w = 0; w < c.W; ++w) {
@@ -227,23 +235,26 @@ __global__ void k_uf_lists(ClusterArgs c) {
         }
     }
 }
+__global__ __launch_bounds__(256) void k_uf_lists(ClusterArgs c) { uf_lists_block(c, blockIdx.x); }
 
-__global__ void k_uf_compress(ClusterArgs c) {
-    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void uf_compress_block(const ClusterArgs& c, long long blk) {
+    const long long a = blk * 256 + threadIdx.x;
     if (a >= c.n) return;
     if (c.core[a]) c.parent[a] = uf_find(c.parent, (int)a);
 }
+__global__ __launch_bounds__(256) void k_uf_compress(ClusterArgs c) { uf_compress_block(c, blockIdx.x); }
 
-__global__ void k_root_flags(ClusterArgs c, int* flags) {
-    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void root_flags_block(const ClusterArgs& c, int* flags, long long blk) {
+    const long long a = blk * 256 + threadIdx.x;
     if (a >= c.n) return;
     flags[a] = (c.core[a] && c.parent[a] == (int)a) ? 1 : 0;
 }
+__global__ __launch_bounds__(256) void k_root_flags(ClusterArgs c, int* flags) { root_flags_block(c, flags, blockIdx.x); }
 
 // Border points: the smallest root among adjacent core points, from both
 // directions of every hit pair.
-__global__ void k_border(ClusterArgs c) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void border_block(const ClusterArgs& c, long long blk) {
+    const long long t = blk * 256 + threadIdx.x;
     if (t >= c.n * c.S) return;
     const long long a = t / c.S;
     const int s = (int)(t % c.S);
@@ -272,24 +283,32 @@ __global__ void k_border(ClusterArgs c) {
     }
     if (!ca && best != 0x7fffffff && load_relaxed(&c.border_min[a]) > best) atomicMin(&c.border_min[a], best);
 }
+__global__ __launch_bounds__(256) void k_border(ClusterArgs c) { border_block(c, blockIdx.x); }
 
-__global__ void k_labels(ClusterArgs c, const int* rank) {
-    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void labels_block(const ClusterArgs& c, const int* rank, long long blk) {
+    const long long a = blk * 256 + threadIdx.x;
     if (a >= c.n) return;
     int lab;
     if (c.core[a]) lab = rank[c.parent[a]];
     else lab = c.border_min[a] == 0x7fffffff ? -1 : rank[c.border_min[a]];
     c.labels[a] = lab;
 }
+__global__ __launch_bounds__(256) void k_labels(ClusterArgs c, const int* rank) { labels_block(c, rank, blockIdx.x); }
 
 // Core test of new minPts on degrees already counted for the same eps.
-__global__ void k_core_minpts(const int* __restrict__ degree, long long n, int minPts, uint8_t* __restrict__ core,
-                              unsigned long long* __restrict__ n_core) {
-    const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void core_minpts_block(const int* __restrict__ degree, long long n, int minPts,
+                                                  uint8_t* __restrict__ core, unsigned long long* __restrict__ n_core,
+                                                  long long blk) {
+    const long long a = blk * 256 + threadIdx.x;
     const bool is_core = a < n && degree[a] >= minPts - 1;
     if (a < n) core[a] = is_core ? 1 : 0;
     const unsigned long long b = __ballot(is_core);
     if (lane_id() == 0 && b) atomicAdd(n_core, (unsigned long long)__popcll(b));
+}
+__global__ __launch_bounds__(256) void k_core_minpts(const int* __restrict__ degree, long long n, int minPts,
+                                                     uint8_t* __restrict__ core,
+                                                     unsigned long long* __restrict__ n_core) {
+    core_minpts_block(degree, n, minPts, core, n_core, blockIdx.x);
 }
 
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -340,6 +359,135 @@ int launch_border(const ClusterArgs& c, hipStream_t st) {
 int launch_labels(const ClusterArgs& c, const int* rank, hipStream_t st) {
     hipLaunchKernelGGL(k_labels, dim3(nblk(c.n, 256)), dim3(256), 0, st, c, rank);
     return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched chain (ChainSet): one launch per step for all sets.
+
+// The set of workgroup b: the largest s with pre[s] <= b (a set without
+// workgroups in this step has pre[s] == pre[s + 1] and is never chosen).
+__device__ __forceinline__ int set_of_block(const unsigned* __restrict__ pre, int nset, unsigned b) {
+    int lo = 0, hi = nset;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int OP, int BS>
+__global__ __launch_bounds__(BS) void k_sets(const ChainSet* __restrict__ sets, const unsigned* __restrict__ pre,
+                                             int nset) {
+    const int s = set_of_block(pre, nset, blockIdx.x);
+    const long long blk = (long long)(blockIdx.x - pre[s]);
+    const ChainSet& x = sets[s];
+    const ClusterArgs& c = x.a;
+    if constexpr (OP == kChDegree) {
+        degree_block(c, blk);
+    } else if constexpr (OP == kChCore) {
+        core_minpts_block(c.degree, c.n, c.minPts, c.core, c.n_core, blk);
+    } else if constexpr (OP == kChAdjClear) {  // the first step: also resets the set's core count
+        const long long i = blk * BS + threadIdx.x;
+        if (i < (long long)c.L2 * c.M * c.W) c.adj[i] = 0u;
+        if (i == 0) *c.n_core = 0ull;
+    } else if constexpr (OP == kChCorehub) {
+        corehub_block(c, blk);
+    } else if constexpr (OP == kChUfInit) {
+        uf_init_block(c, blk);
+    } else if constexpr (OP == kChUfPoints) {
+        uf_points_block(c, blk);
+    } else if constexpr (OP == kChUfPointsStaged) {
+        uf_points_staged_block(c, blk);
+    } else if constexpr (OP == kChUfLists) {
+        uf_lists_block(c, blk);
+    } else if constexpr (OP == kChUfCompress) {
+        uf_compress_block(c, blk);
+    } else if constexpr (OP == kChRootFlags) {
+        root_flags_block(c, x.rootf, blk);
+    } else if constexpr (OP == kChScanBlocks) {
+        __shared__ int wsum[16];
+        __shared__ int tot;
+        const long long i = blk * 1024 + threadIdx.x;
+        (void)block_scan_excl_1024(i < c.n ? x.rootf[i] : 0, wsum, &tot);
+        if (threadIdx.x == 0) x.bsum[blk] = tot;
+    } else if constexpr (OP == kChScanTop) {
+        __shared__ int wsum[16];
+        __shared__ int tot;
+        const int nb = (int)((c.n + 1023) / 1024);
+        int carry = 0;
+        for (int b0 = 0; b0 < nb; b0 += 1024) {
+            const int i = b0 + (int)threadIdx.x;
+            const int v = i < nb ? x.bsum[i] : 0;
+            const int ex = block_scan_excl_1024(v, wsum, &tot);
+            if (i < nb) x.bsum[i] = carry + ex;
+            carry += tot;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) *x.total = carry;
+    } else if constexpr (OP == kChScanApply) {
+        __shared__ int wsum[16];
+        __shared__ int tot;
+        const long long i = blk * 1024 + threadIdx.x;
+        const int ex = block_scan_excl_1024(i < c.n ? x.rootf[i] : 0, wsum, &tot);
+        if (i < c.n) x.rank[i] = x.bsum[blk] + ex;
+    } else if constexpr (OP == kChBorder) {
+        border_block(c, blk);
+    } else if constexpr (OP == kChLabels) {
+        labels_block(c, x.rank, blk);
+    }
+}
+
+static constexpr int chain_bs(int op) {
+    return op == kChUfPointsStaged ? 128 : (op == kChScanBlocks || op == kChScanTop || op == kChScanApply) ? 1024 : 256;
+}
+int chain_block_threads(int op) { return chain_bs(op); }
+
+long long chain_set_blocks(int op, const ClusterArgs& a, int owns_degree) {
+    const long long n = a.n;
+    const long long bs = chain_block_threads(op);
+    auto cdiv = [](long long x, long long y) { return (x + y - 1) / y; };
+    switch (op) {
+        case kChDegree: return owns_degree ? cdiv(n, bs) : 0;
+        case kChCore: return owns_degree ? 0 : cdiv(n, bs);
+        case kChAdjClear: return cdiv((long long)a.L2 * a.M * a.W, bs);
+        case kChCorehub: return cdiv((long long)a.L2 * a.W, bs);
+        case kChUfLists: return cdiv((long long)a.L2 * a.M, bs);
+        case kChScanTop: return 1;
+        case kChBorder: return cdiv(n * a.S, bs);
+        default: return cdiv(n, bs);
+    }
+}
+
+template <int OP>
+static int launch_op(const ChainSet* sets, const unsigned* pre, int nset, unsigned grid, size_t lds, hipStream_t st) {
+    constexpr int BS = chain_bs(OP);
+    hipLaunchKernelGGL((k_sets<OP, BS>), dim3(grid), dim3(BS), OP == kChUfPointsStaged ? lds : 0, st, sets, pre, nset);
+    return (int)hipGetLastError();
+}
+
+int launch_chain_op(int op, const ChainSet* sets, const unsigned* pre, int nset, unsigned grid, size_t lds_staged,
+                    hipStream_t st) {
+    if (grid == 0 || nset <= 0) return 0;
+    if (op == kChUfPointsStaged && lds_staged > 64 * 1024) return -2;
+    switch (op) {
+        case kChDegree: return launch_op<kChDegree>(sets, pre, nset, grid, lds_staged, st);
+        case kChCore: return launch_op<kChCore>(sets, pre, nset, grid, lds_staged, st);
+        case kChAdjClear: return launch_op<kChAdjClear>(sets, pre, nset, grid, lds_staged, st);
+        case kChCorehub: return launch_op<kChCorehub>(sets, pre, nset, grid, lds_staged, st);
+        case kChUfInit: return launch_op<kChUfInit>(sets, pre, nset, grid, lds_staged, st);
+        case kChUfPoints: return launch_op<kChUfPoints>(sets, pre, nset, grid, lds_staged, st);
+        case kChUfPointsStaged: return launch_op<kChUfPointsStaged>(sets, pre, nset, grid, lds_staged, st);
+        case kChUfLists: return launch_op<kChUfLists>(sets, pre, nset, grid, lds_staged, st);
+        case kChUfCompress: return launch_op<kChUfCompress>(sets, pre, nset, grid, lds_staged, st);
+        case kChRootFlags: return launch_op<kChRootFlags>(sets, pre, nset, grid, lds_staged, st);
+        case kChScanBlocks: return launch_op<kChScanBlocks>(sets, pre, nset, grid, lds_staged, st);
+        case kChScanTop: return launch_op<kChScanTop>(sets, pre, nset, grid, lds_staged, st);
+        case kChScanApply: return launch_op<kChScanApply>(sets, pre, nset, grid, lds_staged, st);
+        case kChBorder: return launch_op<kChBorder>(sets, pre, nset, grid, lds_staged, st);
+        case kChLabels: return launch_op<kChLabels>(sets, pre, nset, grid, lds_staged, st);
+        default: return -3;
+    }
 }
 
 // ---------------------------------------------------------------------------

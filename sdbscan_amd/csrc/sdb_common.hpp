@@ -174,4 +174,33 @@ __device__ __forceinline__ uint32_t xchg_l(uint32_t v, int lg) {
     return xchg<S>(v, lg);  // bits 4 and 5 of lg are those of the physical lane
 }
 
+// Exclusive scan of one int per thread over a workgroup of 1024 threads;
+// *total gets the workgroup's sum (wsum: 16 ints of LDS).
+__device__ __forceinline__ int block_scan_excl_1024(int v, int* wsum, int* total) {
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += y;
+    }
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+        int s = lane < 16 ? wsum[lane] : 0;
+        int si = s;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(si, o, kWave);
+            if (lane >= o) si += y;
+        }
+        if (lane < 16) wsum[lane] = si - s;  // exclusive wave offsets
+        if (lane == 15) *total = si;
+    }
+    __syncthreads();
+    const int r = wsum[w] + incl - v;
+    __syncthreads();
+    return r;
+}
+
 }  // namespace sdb

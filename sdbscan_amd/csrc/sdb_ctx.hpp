@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,47 @@
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool in_arena = false;  // carved from a DevArena: never freed on its own
+};
+
+// Bump allocator over large device chunks, shared by the contexts of one
+// batched sweep, so that their many buffers cost a few hipMalloc / hipFree
+// calls (each hipFree waits for the device) instead of one each.  Buffers
+// are not returned one by one (a grown buffer leaves its old space behind);
+// the chunks go with the arena.  Thread-safe.
+struct DevArena {
+    static constexpr size_t kChunk = size_t(1) << 30;
+    std::mutex mu;
+    std::vector<void*> chunks;
+    char* cur = nullptr;
+    size_t left = 0;
+    hipError_t alloc(size_t bytes, void** out) {
+        bytes = (bytes + 255) & ~size_t(255);
+        std::lock_guard<std::mutex> g(mu);
+        if (bytes > kChunk / 4) {  // large buffers get a chunk of their own
+            void* p = nullptr;
+            const hipError_t e = hipMalloc(&p, bytes);
+            if (e != hipSuccess) return e;
+            chunks.push_back(p);
+            *out = p;
+            return hipSuccess;
+        }
+        if (bytes > left) {
+            void* p = nullptr;
+            const hipError_t e = hipMalloc(&p, kChunk);
+            if (e != hipSuccess) return e;
+            chunks.push_back(p);
+            cur = (char*)p;
+            left = kChunk;
+        }
+        *out = cur;
+        cur += bytes;
+        left -= bytes;
+        return hipSuccess;
+    }
+    ~DevArena() {
+        for (void* p : chunks) (void)hipFree(p);
+    }
 };
 
 struct sdb_ctx {
@@ -77,6 +119,7 @@ struct sdb_ctx {
     int sng_Ms = 64;
     sdb_stats stats{};
     void* scratch = nullptr;  // this device's address of the static scratch array
+    DevArena* arena = nullptr;  // when set, buffers are carved from it (batched sweep)
     // batched sweep: hit masks and column counts of every eps, labels and
     // component counts of every (eps, minPts)
     DevBuf sw_E, sw_colsum, sw_labels, sw_comps;
@@ -109,16 +152,27 @@ static inline int fail(sdb_ctx* c, int code, const char* fmt, ...) {
                         _r > 0 ? hipGetErrorString((hipError_t)_r) : "");                \
     } while (0)
 
+// Frees a buffer the context allocated itself (arena buffers stay with the
+// arena).
+static inline void release(DevBuf& b) {
+    if (b.p && !b.in_arena) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    b.in_arena = false;
+}
+
 static inline int ensure(sdb_ctx* c, DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return 0;
-    if (b.p) { hipFree(b.p); b.p = nullptr; b.bytes = 0; }
-    hipError_t e = hipMalloc(&b.p, bytes);
+    release(b);
+    const hipError_t e = c->arena ? c->arena->alloc(bytes, &b.p) : hipMalloc(&b.p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
+        b.p = nullptr;
         return fail(c, SDB_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
     }
     b.bytes = bytes;
+    b.in_arena = c->arena != nullptr;
     return 0;
 }
 

@@ -190,4 +190,33 @@ int launch_root_flags(const ClusterArgs& c, int* flags, hipStream_t st);
 int launch_border(const ClusterArgs& c, hipStream_t st);
 int launch_labels(const ClusterArgs& c, const int* rank, hipStream_t st);
 
+// Batched cluster chain: many independent settings (a dataset at one eps and
+// one minPts) go through degree -> core -> union-find -> labels together, one
+// launch per step for all of them.  A set carries its own ClusterArgs (its
+// degree / core / parent / border / labels / adjacency buffers), root flags,
+// ranks, scan block sums and the component count it writes.  Sets of the
+// first minPts of an eps count that eps's degrees (kChDegree, core test at
+// a.minPts included); the other sets of the eps only test their minPts on
+// those degrees (kChCore).
+struct ChainSet {
+    ClusterArgs a;
+    int* rootf;
+    int* rank;
+    int* bsum;   // ceil(n / 1024) block sums of the root-flag scan
+    int* total;  // number of clusters (roots)
+    int owns_degree;
+};
+enum ChainOp {
+    kChDegree, kChCore, kChAdjClear, kChCorehub, kChUfInit, kChUfPoints, kChUfPointsStaged, kChUfLists,
+    kChUfCompress, kChRootFlags, kChScanBlocks, kChScanTop, kChScanApply, kChBorder, kChLabels, kChOps
+};
+// Workgroups of one set in op's launch, and the workgroup size of op.
+long long chain_set_blocks(int op, const ClusterArgs& a, int owns_degree);
+int chain_block_threads(int op);
+// One step for all sets: pre[s] is the first workgroup of set s, pre[nset]
+// the grid; lds_staged is the dynamic LDS of kChUfPointsStaged (128 points
+// x the largest S x 16 bytes).
+int launch_chain_op(int op, const ChainSet* sets, const unsigned* pre, int nset, unsigned grid, size_t lds_staged,
+                    hipStream_t st);
+
 }  // namespace sdb

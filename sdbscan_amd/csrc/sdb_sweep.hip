@@ -12,9 +12,10 @@
 //      tile rows per row length are picked by timing that kernel (autotune);
 //   3. re-decides the pairs inside the FP32 band of an eps in FP64 (one
 //      launch over all jobs);
-//   4. per job, runs the per-eps degree -> core test -> union-find -> labels
-//      chain for every (eps, minPts) as one captured hipGraph (clears by
-//      kernels, no memset nodes);
+//   4. runs the degree -> core test -> union-find -> labels chain of every
+//      (job, eps, minPts) set together, one launch per step over all sets
+//      (each workgroup finds its set from per-step workgroup offsets), the
+//      steps captured as one hipGraph (clears by kernels, no memset nodes);
 //   5. sOPTICS at the largest eps on the host walk, as sdb_optics_orders.
 // The decisions are the single-eps path's (same cuts, same FP64 recheck), so
 // the outputs equal sdb_batch_run's.
@@ -127,11 +128,10 @@ void append_tiles(int job, const JobState& s, int qb, std::vector<int4>& tiles) 
 // Device side of the batch: descriptors, tiles, work-queue head, recheck
 // queue (16-byte entries) and counters.
 struct Shared {
-    DevBuf desc, tiles, head, queue, qcount, stats;
+    DevBuf desc, tiles, head, queue, qcount, stats, chain_sets, chain_pre;
     long long qcap = 0;
     ~Shared() {
-        for (DevBuf* b : {&desc, &tiles, &head, &queue, &qcount, &stats})
-            if (b->p) (void)hipFree(b->p);
+        for (DevBuf* b : {&desc, &tiles, &head, &queue, &qcount, &stats, &chain_sets, &chain_pre}) release(*b);
     }
 };
 
@@ -156,127 +156,160 @@ int run_tiles(sdb_ctx* c, Shared& sh, const std::vector<JobState*>& jobs_used, i
     return SDB_OK;
 }
 
-// The per-(eps, minPts) cluster chain of one job, enqueued on the context's
-// stream (captured into a graph by the caller): no host synchronisation, no
-// allocation, no memset.
-int enqueue_post(sdb_batch_job& j, JobState& s) {
+// Buffers of the job's cluster chain: the degrees of every eps; per (eps,
+// minPts) set the core flags, union-find parents, border minima, root flags,
+// ranks, scan block sums, list adjacency, core-hub masks, core count, labels
+// and cluster count.
+int chain_buffers(sdb_batch_job& j, JobState& s) {
     sdb_ctx* c = s.c;
-    hipStream_t st = c->st;
-    const long long n = c->n;
-    const int ne = j.n_eps, nm = j.n_minpts;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)c->n;
+    const size_t ns = (size_t)j.n_eps * j.n_minpts;
+    const size_t nb = (n + 1023) / 1024;
     const SweepJob& d = s.desc;
-    unsigned long long* ncore = (unsigned long long*)c->stats64.p + 3;
-    int r = 0;
-    for (int oe = 0; oe < ne && !r; ++oe) {
+    ENSURE(c->degree, (size_t)j.n_eps * n * 4);
+    ENSURE(c->core, ns * n);
+    ENSURE(c->parent, ns * n * 4);
+    ENSURE(c->border, ns * n * 4);
+    ENSURE(c->rootf, ns * n * 4);
+    ENSURE(c->rank, ns * n * 4);
+    ENSURE(c->scan_tmp, ns * nb * 4);
+    ENSURE(c->adj, ns * d.L2 * d.M * d.W * 4);
+    ENSURE(c->corehub, ns * d.L2 * d.W * 4);
+    ENSURE(c->stats64, ns * 8);
+    ENSURE(c->sw_labels, ns * n * 4);
+    ENSURE(c->sw_comps, ns * 4);
+    return SDB_OK;
+}
+
+// The job's sets in ascending eps, then minPts, order; set (eps ie, minPts m)
+// writes output slot ie * n_minpts + m.
+void chain_sets(const sdb_batch_job& j, JobState& s, std::vector<ChainSet>& out) {
+    sdb_ctx* c = s.c;
+    const SweepJob& d = s.desc;
+    const size_t n = (size_t)c->n;
+    const size_t nb = (n + 1023) / 1024;
+    const int ne = j.n_eps, nm = j.n_minpts;
+    const size_t adj_words = (size_t)d.L2 * d.M * d.W, hub_words = (size_t)d.L2 * d.W;
+    for (int oe = 0; oe < ne; ++oe) {
         const int ie = s.order[oe];
-        ClusterArgs ca = cluster_args(c);
-        ca.E = d.E + (size_t)oe * d.n_ent * d.W;
-        ca.colsum = d.colsum + (size_t)oe * d.L2 * d.M;
-        ca.minPts = j.minPts[0];
-        ca.n_core = ncore;
-        r |= launch_fill_u32((uint32_t*)ncore, 2, 0u, st);
-        r |= launch_degree(ca, st);
-        for (int m = 0; m < nm && !r; ++m) {
-            ca.minPts = j.minPts[m];
-            r |= launch_fill_u32((uint32_t*)ncore, 2, 0u, st);
-            r |= launch_core_minpts(P_<int>(c->degree), n, j.minPts[m], P_<uint8_t>(c->core), ncore, st);
-            r |= launch_fill_u32(P_<uint32_t>(c->adj), (long long)d.L2 * d.M * d.W, 0u, st);
-            r |= launch_corehub(ca, st);
-            r |= launch_uf_init(ca, st);
-            r |= launch_uf_points(ca, st);
-            r |= launch_uf_lists(ca, st);
-            r |= launch_uf_compress(ca, st);
-            r |= launch_root_flags(ca, P_<int>(c->rootf), st);
-            r |= scan_exclusive_i32_graph(P_<int>(c->rootf), P_<int>(c->rank), n, P_<int>(c->total),
-                                          P_<int>(c->scan_tmp), st);
-            r |= launch_border(ca, st);
-            r |= launch_labels(ca, P_<int>(c->rank), st);
+        for (int m = 0; m < nm; ++m) {
             const size_t o = (size_t)ie * nm + m;
-            r |= launch_copy_i32(P_<int>(c->labels), P_<int>(c->sw_labels) + o * n, n, st);
-            r |= launch_copy_i32(P_<int>(c->total), P_<int>(c->sw_comps) + o, 1, st);
+            ChainSet x{};
+            x.a = cluster_args(c);
+            x.a.E = d.E + (size_t)oe * d.n_ent * d.W;
+            x.a.colsum = d.colsum + (size_t)oe * d.L2 * d.M;
+            x.a.minPts = j.minPts[m];
+            x.a.degree = P_<int>(c->degree) + (size_t)oe * n;
+            x.a.core = P_<uint8_t>(c->core) + o * n;
+            x.a.parent = P_<int>(c->parent) + o * n;
+            x.a.border_min = P_<int>(c->border) + o * n;
+            x.a.labels = P_<int>(c->sw_labels) + o * n;
+            x.a.adj = P_<uint32_t>(c->adj) + o * adj_words;
+            x.a.corehub = P_<uint32_t>(c->corehub) + o * hub_words;
+            x.a.n_core = P_<unsigned long long>(c->stats64) + o;
+            x.rootf = P_<int>(c->rootf) + o * n;
+            x.rank = P_<int>(c->rank) + o * n;
+            x.bsum = P_<int>(c->scan_tmp) + o * nb;
+            x.total = P_<int>(c->sw_comps) + o;
+            x.owns_degree = m == 0 ? 1 : 0;
+            out.push_back(x);
         }
     }
-    return r;
 }
 
-// Buffers of the cluster chain (allocated before any capture).
-int post_buffers(sdb_batch_job& j, JobState& s) {
-    sdb_ctx* c = s.c;
-    HIPCHK(hipSetDevice(c->device));
-    const long long n = c->n;
-    const int ne = j.n_eps, nm = j.n_minpts;
-    const SweepJob& d = s.desc;
-    ENSURE(c->degree, (size_t)n * 4);
-    ENSURE(c->core, (size_t)n);
-    ENSURE(c->total, 16);
-    ENSURE(c->stats64, 64);
-    ENSURE(c->corehub, (size_t)d.L2 * d.W * 4);
-    ENSURE(c->parent, (size_t)n * 4);
-    ENSURE(c->border, (size_t)n * 4);
-    ENSURE(c->labels, (size_t)n * 4);
-    ENSURE(c->adj, (size_t)d.L2 * d.M * d.W * 4);
-    ENSURE(c->rootf, (size_t)n * 4);
-    ENSURE(c->rank, (size_t)n * 4);
-    ENSURE(c->scan_tmp, (size_t)((n + 1023) / 1024) * 4 + 64);
-    ENSURE(c->sw_labels, (size_t)ne * nm * n * 4);
-    ENSURE(c->sw_comps, (size_t)ne * nm * 4);
-    HIPCHK(hipStreamSynchronize(c->st));
-    return SDB_OK;
-}
+// A run of consecutive sets launched together, and its workgroup offsets per
+// step (kChOps x (count + 1), set k of step op at pre[op * (count + 1) + k]).
+struct ChainGroup {
+    int first = 0, count = 0;
+    bool staged = false;  // union-find points through LDS (every set W = 4)
+    size_t lds = 0;
+    std::vector<unsigned> pre;
+};
 
-// Captures the job's cluster chain into an executable graph.  The captures
-// run one at a time on one host thread, while no other thread makes HIP calls.
-int post_capture(sdb_batch_job& j, JobState& s, hipGraphExec_t* exec_out) {
-    sdb_ctx* c = s.c;
-    HIPCHK(hipSetDevice(c->device));
-    hipGraph_t graph = nullptr;
-    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    const int re = enqueue_post(j, s);
-    const hipError_t ee = hipStreamEndCapture(c->st, &graph);
-    if (re || ee != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipGetLastError();
-        return fail(c, SDB_EHIP, "capture of the cluster chain failed (%d, %s)", re, hipGetErrorString(ee));
+// Splits the sets into groups whose every step launches fewer than 2^31
+// threads, and computes their workgroup offsets.
+std::vector<ChainGroup> chain_groups(const std::vector<ChainSet>& sets) {
+    std::vector<ChainGroup> groups;
+    const long long cap = 1ll << 31;
+    size_t k = 0;
+    while (k < sets.size()) {
+        ChainGroup g;
+        g.first = (int)k;
+        std::vector<long long> tot(kChOps, 0);
+        while (k < sets.size()) {
+            bool fits = true;
+            for (int op = 0; op < kChOps && fits; ++op) {
+                const long long b = chain_set_blocks(op, sets[k].a, sets[k].owns_degree);
+                fits = (tot[op] + b) * chain_block_threads(op) < cap;
+            }
+            if (!fits && g.count > 0) break;
+            for (int op = 0; op < kChOps; ++op) tot[op] += chain_set_blocks(op, sets[k].a, sets[k].owns_degree);
+            ++g.count;
+            ++k;
+        }
+        int maxS = 0;
+        g.staged = true;
+        for (int i = g.first; i < g.first + g.count; ++i) {
+            maxS = std::max(maxS, sets[i].a.S);
+            g.staged = g.staged && sets[i].a.W == 4;
+        }
+        g.lds = (size_t)128 * maxS * 16;
+        g.staged = g.staged && g.lds <= 64 * 1024;
+        g.pre.assign((size_t)kChOps * (g.count + 1), 0u);
+        for (int op = 0; op < kChOps; ++op) {
+            unsigned acc = 0;
+            for (int i = 0; i < g.count; ++i) {
+                g.pre[(size_t)op * (g.count + 1) + i] = acc;
+                acc += (unsigned)chain_set_blocks(op, sets[g.first + i].a, sets[g.first + i].owns_degree);
+            }
+            g.pre[(size_t)op * (g.count + 1) + g.count] = acc;
+        }
+        groups.push_back(std::move(g));
     }
-    const hipError_t ei = hipGraphInstantiate(exec_out, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ei != hipSuccess) return fail(c, SDB_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    return groups;
+}
+
+// The chain of every group, one launch per step, on stream st (captured by
+// the caller: kernels only, the clears included).
+int enqueue_chain(sdb_ctx* c, const ChainSet* d_sets, const unsigned* d_pre, const std::vector<ChainGroup>& groups,
+                  hipStream_t st) {
+    static const int steps[] = {kChAdjClear, kChDegree,     kChCore,       kChCorehub,  kChUfInit,
+                                kChUfPoints, kChUfLists,    kChUfCompress, kChRootFlags, kChScanBlocks,
+                                kChScanTop,  kChScanApply,  kChBorder,     kChLabels};
+    size_t pre_off = 0;
+    for (const ChainGroup& g : groups) {
+        const ChainSet* gs = d_sets + g.first;
+        for (int op : steps) {
+            const int lop = (op == kChUfPoints && g.staged) ? (int)kChUfPointsStaged : op;
+            const size_t row = (size_t)lop * (g.count + 1);
+            const unsigned grid = g.pre[row + g.count];
+            LCHK(launch_chain_op(lop, gs, d_pre + pre_off + row, g.count, grid, g.lds, st));
+        }
+        pre_off += g.pre.size();
+    }
     return SDB_OK;
 }
 
-// Launches the job's graph (replays times: every replay must give the first
-// replay's labels) and copies labels and cluster counts to the job's arrays.
-int post_run(sdb_batch_job& j, JobState& s, hipGraphExec_t exec, int replays) {
+// Copies the job's labels and cluster counts to its host arrays.
+int chain_copy_out(sdb_batch_job& j, JobState& s) {
     sdb_ctx* c = s.c;
     HIPCHK(hipSetDevice(c->device));
     hipStream_t st = c->st;
-    const long long n = c->n;
-    const int ne = j.n_eps, nm = j.n_minpts;
-    std::vector<int> lab0, labk;
-    std::vector<int> comps((size_t)ne * nm);
-    for (int k = 0; k < replays; ++k) {
-        HIPCHK(hipGraphLaunch(exec, st));
-        HIPCHK(hipStreamSynchronize(st));
-        if (replays > 1) {
-            labk.resize((size_t)ne * nm * n);
-            HIPCHK(hipMemcpyAsync(labk.data(), c->sw_labels.p, labk.size() * 4, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (k == 0) lab0 = labk;
-            else if (labk != lab0) return fail(c, SDB_EHIP, "graph replay %d gave other labels than replay 0", k);
-        }
-    }
-    if (j.labels)
-        HIPCHK(hipMemcpyAsync(j.labels, c->sw_labels.p, (size_t)ne * nm * n * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(comps.data(), c->sw_comps.p, comps.size() * 4, hipMemcpyDeviceToHost, st));
+    const size_t n = (size_t)c->n;
+    const size_t ns = (size_t)j.n_eps * j.n_minpts;
+    std::vector<int> comps(ns);
+    if (j.labels) HIPCHK(hipMemcpyAsync(j.labels, c->sw_labels.p, ns * n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(comps.data(), c->sw_comps.p, ns * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (j.n_clusters)
-        for (size_t o = 0; o < comps.size(); ++o) j.n_clusters[o] = comps[o] == 0 ? 2 : comps[o] + 1;  // src/sDbscan.cpp:633
+        for (size_t o = 0; o < ns; ++o) j.n_clusters[o] = comps[o] == 0 ? 2 : comps[o] + 1;  // src/sDbscan.cpp:633
     return SDB_OK;
 }
 
 // sOPTICS at the largest eps: the context takes that eps's hit masks, column
-// counts and degrees (the last of the ascending chain), then the host walks
-// of sdb_optics_orders.
+// counts and degrees (the last eps of the chain's degree slices), then the
+// host walks of sdb_optics_orders.
 int optics_job(sdb_batch_job& j, JobState& s) {
     sdb_ctx* c = s.c;
     HIPCHK(hipSetDevice(c->device));
@@ -287,6 +320,9 @@ int optics_job(sdb_batch_job& j, JobState& s) {
     ENSURE(c->colsum, cw * 4);
     HIPCHK(hipMemcpyAsync(c->E.p, d.E + (size_t)(ne - 1) * ew, ew * 4, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->colsum.p, d.colsum + (size_t)(ne - 1) * cw, cw * 4, hipMemcpyDeviceToDevice, c->st));
+    if (ne > 1)
+        HIPCHK(hipMemcpyAsync(c->degree.p, P_<int>(c->degree) + (size_t)(ne - 1) * c->n, (size_t)c->n * 4,
+                              hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     c->cored = true;
     c->eps = j.eps[s.order[ne - 1]];
@@ -308,7 +344,8 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
                                double* stage_ms, double* wall_ms) {
     if (n_jobs < 0 || (n_jobs > 0 && !jobs) || n_streams < 1) return SDB_EINVAL;
     const double t0 = now_ms();
-    double st_ms[6] = {0, 0, 0, 0, 0, 0};  // index, tune, tiles, recheck, clusters, optics
+    // setup, index, tune, tiles, recheck, cluster buffers, capture, chain, copy-out, optics, teardown
+    double st_ms[SDB_SWEEP_STAGES] = {};
     int bad = SDB_OK;
     for (int i = 0; i < n_jobs; ++i) {
         sdb_batch_job& j = jobs[i];
@@ -343,10 +380,15 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
     }
     if (hipSetDevice(device) != hipSuccess) return SDB_EHIP;
 
+    // every job's buffers come from one arena (declared first: it outlives
+    // the contexts and the shared buffers, and frees its chunks last)
+    DevArena arena;
     std::vector<JobState> S(n_jobs);
     int rc = SDB_OK;
+    double t = now_ms();
     for (int i = 0; i < n_jobs && rc == SDB_OK; ++i) {
         rc = sdb_create(device, &S[i].c);
+        if (rc == SDB_OK) S[i].c->arena = &arena;
         if (rc) {
             jobs[i].status = rc;
             std::snprintf(jobs[i].error, sizeof(jobs[i].error), "sdb_create failed on device %d", device);
@@ -355,6 +397,7 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
     auto cleanup = [&]() {
         for (JobState& s : S)
             if (s.c) sdb_destroy(s.c);
+        S.clear();
     };
     if (rc) {
         cleanup();
@@ -368,9 +411,10 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
                (double)jobs[y].params.n_points * jobs[y].params.n_features;
     });
     const int nw = std::min(n_streams, n_jobs);
+    st_ms[0] = now_ms() - t;
 
     // 1. indexes
-    double t = now_ms();
+    t = now_ms();
     for_jobs(order, nw, [&](int i) {
         sdb_batch_job& j = jobs[i];
         sdb_ctx* c = S[i].c;
@@ -385,7 +429,7 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
         if (r) { set_job_error(j, r, "prepare", c); return; }
         S[i].ms += now_ms() - tj;
     });
-    st_ms[0] = now_ms() - t;
+    st_ms[1] = now_ms() - t;
     for (int i = 0; i < n_jobs; ++i)
         if (jobs[i].status) rc = rc ? rc : jobs[i].status;
     if (rc) {
@@ -460,7 +504,7 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
             S[i].qb = best;
         }
     }
-    st_ms[1] = now_ms() - t;
+    st_ms[2] = now_ms() - t;
     std::vector<int4> tiles;
     for (int i : order) {
         append_tiles(i, S[i], S[i].qb, tiles);
@@ -484,7 +528,7 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
         if (ensure(c0, sh.queue, (size_t)sh.qcap * 16)) return fail_all(SDB_ENOMEM, "sweep queue");
     }
     if (nq > (unsigned long long)sh.qcap) return fail_all(SDB_ENOMEM, "sweep queue overflow");
-    st_ms[2] = tiles_ms;
+    st_ms[3] = tiles_ms;
 
     // 3. FP64 recheck of the band pairs, all jobs at once
     t = now_ms();
@@ -493,33 +537,96 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
                              P_<unsigned long long>(sh.stats), c0->st) != 0 ||
         hipStreamSynchronize(c0->st) != hipSuccess)
         return fail_all(SDB_EHIP, "sweep recheck");
-    st_ms[3] = now_ms() - t;
+    st_ms[4] = now_ms() - t;
 
-    // 4. cluster chains, one captured graph per job: buffers on the worker
-    // threads, the captures on this thread alone, then the launches
+    // 4. cluster chains: one launch per step over every (job, eps, minPts)
+    // set, all steps captured as one graph (kernels only; no memset nodes)
     int replays = 1;
     if (const char* e = std::getenv("SDB_GRAPH_REPLAYS")) replays = std::max(1, std::min(16, std::atoi(e)));
     t = now_ms();
     for_jobs(order, nw, [&](int i) {
-        const int r = post_buffers(jobs[i], S[i]);
+        const int r = chain_buffers(jobs[i], S[i]);
         if (r) set_job_error(jobs[i], r, "cluster buffers", S[i].c);
     });
-    std::vector<hipGraphExec_t> execs(n_jobs, nullptr);
-    for (int i : order) {
-        if (jobs[i].status) continue;
-        const int r = post_capture(jobs[i], S[i], &execs[i]);
-        if (r) set_job_error(jobs[i], r, "clusters", S[i].c);
+    std::vector<ChainSet> sets;
+    for (int i : order)
+        if (!jobs[i].status) chain_sets(jobs[i], S[i], sets);
+    const std::vector<ChainGroup> groups = chain_groups(sets);
+    size_t pre_words = 0;
+    for (const ChainGroup& g : groups) pre_words += g.pre.size();
+    if (!sets.empty()) {
+        if (ensure(c0, sh.chain_sets, sets.size() * sizeof(ChainSet)) ||
+            ensure(c0, sh.chain_pre, pre_words * sizeof(unsigned)))
+            return fail_all(SDB_ENOMEM, "cluster chain descriptors");
+        std::vector<unsigned> pre_all;
+        pre_all.reserve(pre_words);
+        for (const ChainGroup& g : groups) pre_all.insert(pre_all.end(), g.pre.begin(), g.pre.end());
+        if (hipMemcpy(sh.chain_sets.p, sets.data(), sets.size() * sizeof(ChainSet), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(sh.chain_pre.p, pre_all.data(), pre_words * sizeof(unsigned), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return fail_all(SDB_EHIP, "cluster chain descriptor upload");
     }
+    st_ms[5] = now_ms() - t;
+    t = now_ms();
+    hipGraphExec_t exec = nullptr;
+    if (!sets.empty()) {
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(c0->st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return fail_all(SDB_EHIP, "cluster chain capture");
+        const int re = enqueue_chain(c0, P_<ChainSet>(sh.chain_sets), P_<unsigned>(sh.chain_pre), groups, c0->st);
+        const hipError_t ee = hipStreamEndCapture(c0->st, &graph);
+        hipError_t ei = hipErrorUnknown;
+        if (re == SDB_OK && ee == hipSuccess) ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (re != SDB_OK || ee != hipSuccess || ei != hipSuccess) {
+            (void)hipGetLastError();
+            return fail_all(SDB_EHIP, "cluster chain capture");
+        }
+    }
+    st_ms[6] = now_ms() - t;
+    t = now_ms();
+    if (exec) {
+        // every replay must give the first replay's labels
+        std::vector<std::vector<int>> lab0;
+        for (int k = 0; k < replays; ++k) {
+            if (hipGraphLaunch(exec, c0->st) != hipSuccess || hipStreamSynchronize(c0->st) != hipSuccess) {
+                (void)hipGraphExecDestroy(exec);
+                return fail_all(SDB_EHIP, "cluster chain");
+            }
+            if (replays == 1) break;
+            size_t q = 0;
+            for (int i : order) {
+                if (jobs[i].status) continue;
+                sdb_ctx* c = S[i].c;
+                std::vector<int> lab((size_t)jobs[i].n_eps * jobs[i].n_minpts * c->n);
+                if (hipMemcpy(lab.data(), c->sw_labels.p, lab.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+                    (void)hipGraphExecDestroy(exec);
+                    return fail_all(SDB_EHIP, "cluster chain replay read-back");
+                }
+                if (k == 0) {
+                    lab0.push_back(std::move(lab));
+                } else if (lab != lab0[q]) {
+                    (void)hipGraphExecDestroy(exec);
+                    char msg[96];
+                    std::snprintf(msg, sizeof(msg), "graph replay %d gave other labels than replay 0", k);
+                    return fail_all(SDB_EHIP, msg);
+                }
+                ++q;
+            }
+        }
+        (void)hipGraphExecDestroy(exec);
+    }
+    st_ms[7] = now_ms() - t;
+    t = now_ms();
     for_jobs(order, nw, [&](int i) {
-        if (jobs[i].status || !execs[i]) return;
+        if (jobs[i].status) return;
         const double tj = now_ms();
-        const int r = post_run(jobs[i], S[i], execs[i], replays);
+        const int r = chain_copy_out(jobs[i], S[i]);
         if (r) set_job_error(jobs[i], r, "clusters", S[i].c);
         S[i].ms += now_ms() - tj;
     });
-    for (hipGraphExec_t e : execs)
-        if (e) (void)hipGraphExecDestroy(e);
-    st_ms[4] = now_ms() - t;
+    st_ms[8] = now_ms() - t;
 
     // 5. sOPTICS at the largest eps
     t = now_ms();
@@ -531,9 +638,11 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
         if (r) set_job_error(j, r, "optics", S[i].c);
         S[i].ms += now_ms() - tj;
     });
-    st_ms[5] = now_ms() - t;
+    st_ms[9] = now_ms() - t;
     for (int i = 0; i < n_jobs; ++i) jobs[i].ms = S[i].ms;
+    t = now_ms();
     cleanup();
+    st_ms[10] = now_ms() - t;
     if (stage_ms) std::memcpy(stage_ms, st_ms, sizeof(st_ms));
     if (wall_ms) *wall_ms = now_ms() - t0;
     for (int i = 0; i < n_jobs; ++i)

@@ -245,33 +245,6 @@ __global__ __launch_bounds__(256) void k_sweep_recheck(const SweepJob* __restric
 // host round trips (graph-capturable): per-block sums of 1024 elements, one
 // workgroup scanning those sums (and writing the total), then every block
 // scanning its elements from its offset.
-__device__ __forceinline__ int block_scan_excl_1024(int v, int* wsum, int* total) {
-    const int lane = lane_id(), w = threadIdx.x >> 6;
-    int incl = v;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const int y = __shfl_up(incl, o, kWave);
-        if (lane >= o) incl += y;
-    }
-    if (lane == kWave - 1) wsum[w] = incl;
-    __syncthreads();
-    if (w == 0) {
-        int s = lane < 16 ? wsum[lane] : 0;
-        int si = s;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            const int y = __shfl_up(si, o, kWave);
-            if (lane >= o) si += y;
-        }
-        if (lane < 16) wsum[lane] = si - s;  // exclusive wave offsets
-        if (lane == 15) *total = si;
-    }
-    __syncthreads();
-    const int r = wsum[w] + incl - v;
-    __syncthreads();
-    return r;
-}
-
 __global__ __launch_bounds__(1024) void k_scan_blocks(const int* __restrict__ in, long long n, int* __restrict__ bsum) {
     __shared__ int wsum[16];
     __shared__ int tot;
