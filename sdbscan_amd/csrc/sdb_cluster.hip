@@ -197,26 +197,40 @@ __device__ __forceinline__ void uf_points_block(const ClusterArgs& c, long long 
 }
 __global__ __launch_bounds__(256) void k_uf_points(ClusterArgs c) { uf_points_block(c, blockIdx.x); }
 
-// W = 4: the E rows of the workgroup's 128 points are staged through LDS with
-// coalesced 16-byte loads, instead of each thread striding through its own
-// 16 S bytes in global memory across the slot loop.
+// The E rows of the workgroup's 128 points (S x W words each) are staged
+// through LDS with coalesced loads (16-byte vectors when S W is a multiple of
+// 4), instead of each thread striding through its own row in global memory
+// across the slot loop.  LDS: 128 x S x W x 4 bytes.
 __device__ __forceinline__ void uf_points_staged_block(const ClusterArgs& c, long long blk) {
-    extern __shared__ uint4 erow[];
-    const int S = c.S;
+    extern __shared__ __align__(16) uint32_t erow[];
+    const int SW = c.S * c.W;
     const long long a0 = blk * 128;
     const int npt = (int)min((long long)128, c.n - a0);
-    const uint4* Eb = (const uint4*)c.E + (size_t)a0 * S;
-    for (int v = threadIdx.x; v < npt * S; v += 128) erow[v] = Eb[v];
+    const uint32_t* Eb = c.E + (size_t)a0 * SW;
+    if ((SW & 3) == 0) {
+        const int nv = npt * (SW >> 2);
+        for (int v = threadIdx.x; v < nv; v += 128) ((uint4*)erow)[v] = ((const uint4*)Eb)[v];
+    } else {
+        for (int v = threadIdx.x; v < npt * SW; v += 128) erow[v] = Eb[v];
+    }
     __syncthreads();
     const long long a = a0 + threadIdx.x;
     if ((int)threadIdx.x >= npt || !c.core[a]) return;
-    const uint4* mine = erow + (size_t)threadIdx.x * S;
-    uf_point(c, a, [&](int s, uint32_t (&m)[kMaxW]) {
-        const uint4 v = mine[s];
-        m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
+    const uint32_t* mine = erow + (size_t)threadIdx.x * SW;
+    if (c.W == 4) {
+        uf_point(c, a, [&](int s, uint32_t (&m)[kMaxW]) {
+            const uint4 v = ((const uint4*)mine)[s];
+            m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
 #pragma unroll
-        for (int w = 4; w < kMaxW; ++w) m[w] = 0u;
-    });
+            for (int w = 4; w < kMaxW; ++w) m[w] = 0u;
+        });
+    } else {
+        const int W = c.W;
+        uf_point(c, a, [&](int s, uint32_t (&m)[kMaxW]) {
+#pragma unroll
+            for (int w = 0; w < kMaxW; ++w) m[w] = w < W ? mine[s * W + w] : 0u;
+        });
+    }
 }
 __global__ __launch_bounds__(128) void k_uf_points_staged(ClusterArgs c) { uf_points_staged_block(c, blockIdx.x); }
 
@@ -332,8 +346,8 @@ int launch_uf_init(const ClusterArgs& c, hipStream_t st) {
     return (int)hipGetLastError();
 }
 int launch_uf_points(const ClusterArgs& c, hipStream_t st) {
-    const size_t sm = (size_t)128 * c.S * 16;
-    if (c.W == 4 && sm <= 64 * 1024) {
+    const size_t sm = (size_t)128 * c.S * c.W * 4;
+    if (sm <= 64 * 1024) {
         hipLaunchKernelGGL(k_uf_points_staged, dim3(nblk(c.n, 128)), dim3(128), sm, st, c);
     } else {
         hipLaunchKernelGGL(k_uf_points, dim3(nblk(c.n, 256)), dim3(256), 0, st, c);
@@ -364,6 +378,83 @@ int launch_labels(const ClusterArgs& c, const int* rank, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Batched chain (ChainSet): one launch per step for all sets.
 
+// Degrees of all eps of a dataset (x is its first set): per eps the own hits
+// are counted from the coalesced E rows as in degree_block; for the points in
+// a list, the mutual-pair mask of every (slot, word) is built once and applied
+// to every eps's E words.
+__device__ __forceinline__ void degree_eps_block(const ChainSet& x, long long blk) {
+    __shared__ int own[kSweepMaxEps][256];
+    const ClusterArgs& c = x.a;
+    const int ne = x.n_eps, nm = x.n_minpts;
+    const long long a0 = blk * 256;
+    const int npt = (int)min((long long)256, c.n - a0);
+    const int SW = c.S * c.W;
+    const size_t es = (size_t)c.n * SW, cs = (size_t)c.L2 * c.M;
+    for (int e = 0; e < ne; ++e) own[e][threadIdx.x] = 0;
+    __syncthreads();
+    for (int e = 0; e < ne; ++e) {
+        const uint32_t* Eb = c.E + e * es + (size_t)a0 * SW;
+        if ((SW & 3) == 0) {
+            const int per = SW >> 2;
+            for (int v = threadIdx.x; v < npt * per; v += 256) {
+                const uint4 w4 = ((const uint4*)Eb)[v];
+                const int pc = __popc(w4.x) + __popc(w4.y) + __popc(w4.z) + __popc(w4.w);
+                if (pc) atomicAdd(&own[e][v / per], pc);
+            }
+        } else {
+            for (int v = threadIdx.x; v < npt * SW; v += 256) {
+                const int pc = __popc(Eb[v]);
+                if (pc) atomicAdd(&own[e][v / SW], pc);
+            }
+        }
+    }
+    __syncthreads();
+    const long long a = a0 + threadIdx.x;
+    const bool live = threadIdx.x < npt;
+    int dx[kSweepMaxEps];
+#pragma unroll
+    for (int e = 0; e < kSweepMaxEps; ++e) dx[e] = 0;  // rev - mutual per eps
+    if (live) {
+        const int h = c.hub_id[a];
+        if (h >= 0) {
+            const int hb = c.hub_ptr[h], he = c.hub_ptr[h + 1];
+            for (int q = hb; q < he; ++q) {
+                const int pos = c.hub_mem[q];
+#pragma unroll
+                for (int e = 0; e < kSweepMaxEps; ++e)
+                    if (e < ne) dx[e] += c.colsum[e * cs + pos];
+            }
+            for (int s = 0; s < c.S; ++s) {
+                const int l = slot_list_c(c.topK, a, s, c.K, c.P);
+                for (int w = 0; w < c.W; ++w) {
+                    uint32_t m = 0;
+                    for (int q = hb; q < he; ++q) {
+                        const int L = c.hub_mem[q] / c.M;
+                        m |= c.inv[((size_t)l * c.L2 + L) * c.W + w];
+                    }
+                    if (!m) continue;
+                    const uint32_t* Ea = c.E + (size_t)a * SW + s * c.W + w;
+#pragma unroll
+                    for (int e = 0; e < kSweepMaxEps; ++e)
+                        if (e < ne) dx[e] -= __popc(Ea[e * es] & m);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < kSweepMaxEps; ++e) {
+        if (e >= ne) break;
+        const int deg = live ? own[e][threadIdx.x] + dx[e] : 0;
+        const bool is_core = live && deg >= c.minPts - 1;
+        if (live) {
+            c.degree[(size_t)e * c.n + a] = deg;
+            c.core[(size_t)e * nm * c.n + a] = is_core ? 1 : 0;
+        }
+        const unsigned long long b = __ballot(is_core);
+        if (lane_id() == 0 && b) atomicAdd(c.n_core + (size_t)e * nm, (unsigned long long)__popcll(b));
+    }
+}
+
 // The set of workgroup b: the largest s with pre[s] <= b (a set without
 // workgroups in this step has pre[s] == pre[s + 1] and is never chosen).
 __device__ __forceinline__ int set_of_block(const unsigned* __restrict__ pre, int nset, unsigned b) {
@@ -384,13 +475,12 @@ __global__ __launch_bounds__(BS) void k_sets(const ChainSet* __restrict__ sets, 
     const ChainSet& x = sets[s];
     const ClusterArgs& c = x.a;
     if constexpr (OP == kChDegree) {
-        degree_block(c, blk);
+        degree_eps_block(x, blk);
     } else if constexpr (OP == kChCore) {
         core_minpts_block(c.degree, c.n, c.minPts, c.core, c.n_core, blk);
-    } else if constexpr (OP == kChAdjClear) {  // the first step: also resets the set's core count
+    } else if constexpr (OP == kChAdjClear) {
         const long long i = blk * BS + threadIdx.x;
         if (i < (long long)c.L2 * c.M * c.W) c.adj[i] = 0u;
-        if (i == 0) *c.n_core = 0ull;
     } else if constexpr (OP == kChCorehub) {
         corehub_block(c, blk);
     } else if constexpr (OP == kChUfInit) {
@@ -443,13 +533,14 @@ static constexpr int chain_bs(int op) {
 }
 int chain_block_threads(int op) { return chain_bs(op); }
 
-long long chain_set_blocks(int op, const ClusterArgs& a, int owns_degree) {
+long long chain_set_blocks(int op, const ChainSet& x) {
+    const ClusterArgs& a = x.a;
     const long long n = a.n;
     const long long bs = chain_block_threads(op);
-    auto cdiv = [](long long x, long long y) { return (x + y - 1) / y; };
+    auto cdiv = [](long long u, long long v) { return (u + v - 1) / v; };
     switch (op) {
-        case kChDegree: return owns_degree ? cdiv(n, bs) : 0;
-        case kChCore: return owns_degree ? 0 : cdiv(n, bs);
+        case kChDegree: return x.eps_i == 0 && x.minpts_i == 0 ? cdiv(n, bs) : 0;
+        case kChCore: return x.minpts_i > 0 ? cdiv(n, bs) : 0;
         case kChAdjClear: return cdiv((long long)a.L2 * a.M * a.W, bs);
         case kChCorehub: return cdiv((long long)a.L2 * a.W, bs);
         case kChUfLists: return cdiv((long long)a.L2 * a.M, bs);

@@ -194,28 +194,31 @@ int launch_labels(const ClusterArgs& c, const int* rank, hipStream_t st);
 // one minPts) go through degree -> core -> union-find -> labels together, one
 // launch per step for all of them.  A set carries its own ClusterArgs (its
 // degree / core / parent / border / labels / adjacency buffers), root flags,
-// ranks, scan block sums and the component count it writes.  Sets of the
-// first minPts of an eps count that eps's degrees (kChDegree, core test at
-// a.minPts included); the other sets of the eps only test their minPts on
-// those degrees (kChCore).
+// ranks, scan block sums and the component count it writes.  The sets of a
+// dataset are numbered eps-major (set eps_i * n_minpts + minpts_i), and its
+// per-eps arrays are strided: E and colsum by one eps slice, degrees by n,
+// core flags by n_minpts x n, core counts by n_minpts.  The dataset's first
+// set counts the degrees of all its eps in one pass (kChDegree: the eps-free
+// mutual-pair masks once per point; core test of the first minPts included);
+// the sets of the other minPts test theirs on those degrees (kChCore).
 struct ChainSet {
     ClusterArgs a;
     int* rootf;
     int* rank;
     int* bsum;   // ceil(n / 1024) block sums of the root-flag scan
     int* total;  // number of clusters (roots)
-    int owns_degree;
+    int n_eps, n_minpts, eps_i, minpts_i;
 };
 enum ChainOp {
     kChDegree, kChCore, kChAdjClear, kChCorehub, kChUfInit, kChUfPoints, kChUfPointsStaged, kChUfLists,
     kChUfCompress, kChRootFlags, kChScanBlocks, kChScanTop, kChScanApply, kChBorder, kChLabels, kChOps
 };
 // Workgroups of one set in op's launch, and the workgroup size of op.
-long long chain_set_blocks(int op, const ClusterArgs& a, int owns_degree);
+long long chain_set_blocks(int op, const ChainSet& x);
 int chain_block_threads(int op);
 // One step for all sets: pre[s] is the first workgroup of set s, pre[nset]
 // the grid; lds_staged is the dynamic LDS of kChUfPointsStaged (128 points
-// x the largest S x 16 bytes).
+// x the largest S x W x 4 bytes).
 int launch_chain_op(int op, const ChainSet* sets, const unsigned* pre, int nset, unsigned grid, size_t lds_staged,
                     hipStream_t st);
 

@@ -117,12 +117,16 @@ int prepare_job(sdb_batch_job& j, JobState& s) {
     return SDB_OK;
 }
 
-void append_tiles(int job, const JobState& s, int qb, std::vector<int4>& tiles) {
+// The job's tiles of at most qb rows; with max_rows > 0 only the groups
+// (query block, list) that start within the first max_rows entries.
+void append_tiles(int job, const JobState& s, int qb, std::vector<int4>& tiles, long long max_rows = 0) {
     const sdb_ctx* c = s.c;
     const int NG = c->nblk * c->L2;
-    for (int g = 0; g < NG; ++g)
+    for (int g = 0; g < NG; ++g) {
+        if (max_rows > 0 && c->h_grp_ptr[g] >= max_rows) break;
         for (int b = c->h_grp_ptr[g]; b < c->h_grp_ptr[g + 1]; b += qb)
             tiles.push_back(int4{job, g % c->L2, b, std::min(b + qb, c->h_grp_ptr[g + 1])});
+    }
 }
 
 // Device side of the batch: descriptors, tiles, work-queue head, recheck
@@ -182,8 +186,9 @@ int chain_buffers(sdb_batch_job& j, JobState& s) {
     return SDB_OK;
 }
 
-// The job's sets in ascending eps, then minPts, order; set (eps ie, minPts m)
-// writes output slot ie * n_minpts + m.
+// The job's sets in ascending eps, then minPts, order: set q = oe * n_minpts
+// + m (the oe-th smallest eps) uses the q-th slice of the chain buffers and
+// writes output slot ie * n_minpts + m (ie: that eps's position in the job).
 void chain_sets(const sdb_batch_job& j, JobState& s, std::vector<ChainSet>& out) {
     sdb_ctx* c = s.c;
     const SweepJob& d = s.desc;
@@ -194,25 +199,28 @@ void chain_sets(const sdb_batch_job& j, JobState& s, std::vector<ChainSet>& out)
     for (int oe = 0; oe < ne; ++oe) {
         const int ie = s.order[oe];
         for (int m = 0; m < nm; ++m) {
-            const size_t o = (size_t)ie * nm + m;
+            const size_t o = (size_t)ie * nm + m, q = (size_t)oe * nm + m;
             ChainSet x{};
             x.a = cluster_args(c);
             x.a.E = d.E + (size_t)oe * d.n_ent * d.W;
             x.a.colsum = d.colsum + (size_t)oe * d.L2 * d.M;
             x.a.minPts = j.minPts[m];
             x.a.degree = P_<int>(c->degree) + (size_t)oe * n;
-            x.a.core = P_<uint8_t>(c->core) + o * n;
-            x.a.parent = P_<int>(c->parent) + o * n;
-            x.a.border_min = P_<int>(c->border) + o * n;
+            x.a.core = P_<uint8_t>(c->core) + q * n;
+            x.a.parent = P_<int>(c->parent) + q * n;
+            x.a.border_min = P_<int>(c->border) + q * n;
             x.a.labels = P_<int>(c->sw_labels) + o * n;
-            x.a.adj = P_<uint32_t>(c->adj) + o * adj_words;
-            x.a.corehub = P_<uint32_t>(c->corehub) + o * hub_words;
-            x.a.n_core = P_<unsigned long long>(c->stats64) + o;
-            x.rootf = P_<int>(c->rootf) + o * n;
-            x.rank = P_<int>(c->rank) + o * n;
-            x.bsum = P_<int>(c->scan_tmp) + o * nb;
+            x.a.adj = P_<uint32_t>(c->adj) + q * adj_words;
+            x.a.corehub = P_<uint32_t>(c->corehub) + q * hub_words;
+            x.a.n_core = P_<unsigned long long>(c->stats64) + q;  // core counts: not read by the sweep
+            x.rootf = P_<int>(c->rootf) + q * n;
+            x.rank = P_<int>(c->rank) + q * n;
+            x.bsum = P_<int>(c->scan_tmp) + q * nb;
             x.total = P_<int>(c->sw_comps) + o;
-            x.owns_degree = m == 0 ? 1 : 0;
+            x.n_eps = ne;
+            x.n_minpts = nm;
+            x.eps_i = oe;
+            x.minpts_i = m;
             out.push_back(x);
         }
     }
@@ -222,7 +230,7 @@ void chain_sets(const sdb_batch_job& j, JobState& s, std::vector<ChainSet>& out)
 // step (kChOps x (count + 1), set k of step op at pre[op * (count + 1) + k]).
 struct ChainGroup {
     int first = 0, count = 0;
-    bool staged = false;  // union-find points through LDS (every set W = 4)
+    bool staged = false;  // union-find points with the E rows staged through LDS
     size_t lds = 0;
     std::vector<unsigned> pre;
 };
@@ -240,28 +248,24 @@ std::vector<ChainGroup> chain_groups(const std::vector<ChainSet>& sets) {
         while (k < sets.size()) {
             bool fits = true;
             for (int op = 0; op < kChOps && fits; ++op) {
-                const long long b = chain_set_blocks(op, sets[k].a, sets[k].owns_degree);
+                const long long b = chain_set_blocks(op, sets[k]);
                 fits = (tot[op] + b) * chain_block_threads(op) < cap;
             }
             if (!fits && g.count > 0) break;
-            for (int op = 0; op < kChOps; ++op) tot[op] += chain_set_blocks(op, sets[k].a, sets[k].owns_degree);
+            for (int op = 0; op < kChOps; ++op) tot[op] += chain_set_blocks(op, sets[k]);
             ++g.count;
             ++k;
         }
-        int maxS = 0;
-        g.staged = true;
-        for (int i = g.first; i < g.first + g.count; ++i) {
-            maxS = std::max(maxS, sets[i].a.S);
-            g.staged = g.staged && sets[i].a.W == 4;
-        }
-        g.lds = (size_t)128 * maxS * 16;
-        g.staged = g.staged && g.lds <= 64 * 1024;
+        int maxSW = 0;
+        for (int i = g.first; i < g.first + g.count; ++i) maxSW = std::max(maxSW, sets[i].a.S * sets[i].a.W);
+        g.lds = (size_t)128 * maxSW * 4;
+        g.staged = g.lds <= 64 * 1024;
         g.pre.assign((size_t)kChOps * (g.count + 1), 0u);
         for (int op = 0; op < kChOps; ++op) {
             unsigned acc = 0;
             for (int i = 0; i < g.count; ++i) {
                 g.pre[(size_t)op * (g.count + 1) + i] = acc;
-                acc += (unsigned)chain_set_blocks(op, sets[g.first + i].a, sets[g.first + i].owns_degree);
+                acc += (unsigned)chain_set_blocks(op, sets[g.first + i]);
             }
             g.pre[(size_t)op * (g.count + 1) + g.count] = acc;
         }
@@ -472,8 +476,10 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
         if (hipMemcpy(sh.desc.p, hd.data(), sizeof(SweepJob) * n_jobs, hipMemcpyHostToDevice) != hipSuccess)
             return fail_all(SDB_EHIP, "descriptor upload");
     }
-    // autotune: the first job of every row length times the sweep kernel on
-    // its own tiles for each candidate tile height
+    // autotune: the first (largest) job of every row length times the sweep
+    // kernel for each candidate tile height on the same entries, its groups
+    // within the first 4M entries (enough tiles of every height for all
+    // workgroups of the launch)
     t = now_ms();
     if (autotune) {
         std::map<int, int> best_by_d;
@@ -489,7 +495,7 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
             int best = S[i].qb;
             for (int qb : cand) {
                 std::vector<int4> tl;
-                append_tiles(i, S[i], qb, tl);
+                append_tiles(i, S[i], qb, tl, 4ll << 20);
                 if (ensure(c0, sh.tiles, tl.size() * sizeof(int4)) ||
                     hipMemcpy(sh.tiles.p, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
                     return fail_all(SDB_EHIP, "tile upload");
