@@ -281,6 +281,47 @@ __device__ __forceinline__ void hd3_rotate(float (&v)[(1 << LOGD) / 64], int lg,
     for (int j = 0; j < VPL; ++j) v[j] = __uint_as_float(__float_as_uint(v[j]) ^ pend);
 }
 
+// The same rotation with the sign words of each round read from an LDS table
+// (one xor per element instead of a shift and a masked xor): tbl holds, per
+// round r and register quad k, one 16-byte word per lane at
+// tbl[(r * VPL / 4 + k) * 64 + lane] (hd3_sign_table), the sign bit of
+// register 4 k + c in component c.
+template <int LOGD>
+__device__ __forceinline__ void hd3_rotate_tbl(float (&v)[(1 << LOGD) / 64], int lg, const uint4* tbl) {
+    constexpr int VPL = (1 << LOGD) / 64;
+    static_assert(VPL % 4 == 0, "sign table rows are register quads");
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int k = 0; k < VPL / 4; ++k) {
+            const uint4 m = tbl[(r * (VPL / 4) + k) * 64];
+            v[4 * k + 0] = __uint_as_float(__float_as_uint(v[4 * k + 0]) ^ m.x);
+            v[4 * k + 1] = __uint_as_float(__float_as_uint(v[4 * k + 1]) ^ m.y);
+            v[4 * k + 2] = __uint_as_float(__float_as_uint(v[4 * k + 2]) ^ m.z);
+            v[4 * k + 3] = __uint_as_float(__float_as_uint(v[4 * k + 3]) ^ m.w);
+        }
+        ffht_wave<LOGD>(v, lg);
+    }
+    const uint32_t pend = (uint32_t)(__popc(lg & 15) & 1) << 31;
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) v[j] = __uint_as_float(__float_as_uint(v[j]) ^ pend);
+}
+
+// Fills the sign table of hd3_rotate_tbl from the lane's sign masks (called
+// by one wave of the workgroup; every lane writes its own words).
+template <int LOGD>
+__device__ __forceinline__ void hd3_sign_table(const uint32_t (&sg)[3][((1 << LOGD) / 64 + 31) / 32], int lane,
+                                               uint4* tbl) {
+    constexpr int VPL = (1 << LOGD) / 64;
+    auto word = [&](int r, int j) { return ((((sg[r][j >> 5] >> (j & 31)) & 1u) ^ 1u) << 31); };
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < VPL / 4; ++k)
+            tbl[(r * (VPL / 4) + k) * 64 + lane] =
+                uint4{word(r, 4 * k), word(r, 4 * k + 1), word(r, 4 * k + 2), word(r, 4 * k + 3)};
+}
+
 // Bitonic sorting network across the 64 lanes of a wavefront, descending:
 // stage k merges blocks of size k; lanes with (lane & k) == 0 sort their
 // block descending, so after the last stage lane 0 holds the largest key.
@@ -430,15 +471,15 @@ __device__ __forceinline__ void rank_select2(const uint2* shc, int nc, const uin
     __builtin_amdgcn_wave_barrier();
 }
 
-// Candidate compaction of both sides into LDS slots [0, 64); lanes without a
-// candidate in a register store to a slot of their own past 64, so the stores
-// need no branch (a count past 64 sends the side to the general selection,
-// which does not read the slots).  FULL: every register holds P elements.
+// Candidate compaction of both sides into LDS slots [0, 64): only the
+// candidate lanes store (a register with no candidate on a side skips its
+// stores on an empty exec mask).  A side's slots wrap inside its own 128-slot
+// area; past 64 candidates the side goes to the general selection, which does
+// not read the slots.  FULL: every register holds P elements.
 template <int VPL, bool FULL>
 __device__ __forceinline__ void compact_both(const uint32_t (&kc)[VPL], int P, int lane, int lg, uint32_t tc,
                                              uint32_t tfn, uint32_t* shc, uint32_t* shf, int& nc_out, int& nf_out) {
     int nc = 0, nf = 0;
-    const uint32_t dummy = (uint32_t)(64 + lane);
 #pragma unroll
     for (int j = 0; j < VPL; ++j) {
         const uint32_t ne = ~(uint32_t)(lg + 64 * j);
@@ -452,14 +493,16 @@ __device__ __forceinline__ void compact_both(const uint32_t (&kc)[VPL], int P, i
             cf = live && kc[j] <= tfn;
         }
         const unsigned long long bc = __ballot(cc), bf = __ballot(cf);
-        const uint32_t pc = __builtin_amdgcn_mbcnt_hi((uint32_t)(bc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bc, (uint32_t)nc));
-        const uint32_t pf = __builtin_amdgcn_mbcnt_hi((uint32_t)(bf >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bf, (uint32_t)nf));
-        const uint32_t sc = cc ? (pc & 63u) : dummy;
-        const uint32_t sf = cf ? (pf & 63u) : dummy;
-        shc[2 * sc] = kc[j];
-        shc[2 * sc + 1] = ne;
-        shf[2 * sf] = kc[j];
-        shf[2 * sf + 1] = ne;
+        if (cc) {
+            const uint32_t pc = __builtin_amdgcn_mbcnt_hi((uint32_t)(bc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bc, (uint32_t)nc)) & 127u;
+            shc[2 * pc] = kc[j];
+            shc[2 * pc + 1] = ne;
+        }
+        if (cf) {
+            const uint32_t pf = __builtin_amdgcn_mbcnt_hi((uint32_t)(bf >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bf, (uint32_t)nf)) & 127u;
+            shf[2 * pf] = kc[j];
+            shf[2 * pf + 1] = ne;
+        }
         nc += (int)__popcll(bc);
         nf += (int)__popcll(bf);
     }
@@ -648,12 +691,20 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
     uint64_t* tks = (uint64_t*)smem + (size_t)wib * 256;
     uint32_t* red = (uint32_t*)(smem + off_red);
     float* emb = (float*)(smem + off_emb) + (size_t)wib * D;
+    // sign table of the HD3 rounds (register quads: D >= 256), after the
+    // embedding slices
+    constexpr bool SGN_LDS = VPL % 4 == 0 && LOGD <= 11;
+    const size_t off_sgn = (off_emb + (EMB == kEmbLds ? (size_t)4 * D * 4 : 0) + 15) & ~(size_t)15;
+    uint4* sgn_tbl = (uint4*)(smem + off_sgn);
 
     uint32_t sg[3][NW];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int w = 0; w < NW; ++w) sg[r][w] = a.signmask[(r * NW + w) * 64 + lane];
+    if constexpr (SGN_LDS) {
+        if (wib == 0) hd3_sign_table<LOGD>(sg, lane, sgn_tbl);
+    }
 
 
     if (MODE == kSample) {
@@ -700,7 +751,8 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
 #pragma unroll
                 for (int j = 0; j < VPL; ++j) {
                     const double t = v[j];
-                    if constexpr (EMB == kEmbCos) acc = acc + t * t;
+                    // t * t is exact in FP64, so the fma rounds acc + t * t once, as the sum does
+                    if constexpr (EMB == kEmbCos) acc = fma(t, t, acc);
                     else { acc = acc + t; neg |= v[j] < 0.0f; }
                 }
                 const double tot = wave_sum_f64(acc, lg);
@@ -779,7 +831,15 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
             for (int j = 0; j < VPL; ++j) v[j] = emb[lg + 64 * j];
             __builtin_amdgcn_wave_barrier();
         }
-        hd3_rotate<LOGD>(v, lg, sg);
+        if constexpr (SGN_LDS) {
+            // the lane's table offset, opaque to the compiler so that the
+            // table reads stay in the loop (48 registers if hoisted)
+            int so = lane;
+            asm volatile("" : "+v"(so));
+            hd3_rotate_tbl<LOGD>(v, lg, sgn_tbl + so);
+        } else {
+            hd3_rotate<LOGD>(v, lg, sg);
+        }
 
         if constexpr (MODE == kSample) {
 #pragma unroll
@@ -1286,7 +1346,9 @@ int launch_transform(const float* X, long long n, int d, int ldx, int metric, fl
 template <int LOGD, int MODE, int EMB, bool FULLP>
 static int launch_index_f(const IndexArgs& a, int grid, hipStream_t st) {
     constexpr int D = 1 << LOGD;
-    size_t sm = (MODE == kFull ? (size_t)4 * 2048 + (size_t)8 * D : (size_t)8 * a.P) + (EMB == kEmbLds ? (size_t)4 * D * 4 : 0);
+    constexpr bool SGN_LDS = (D / 64) % 4 == 0 && LOGD <= 11;
+    size_t sm = (MODE == kFull ? (size_t)4 * 2048 + (size_t)8 * D : (size_t)8 * a.P) + (EMB == kEmbLds ? (size_t)4 * D * 4 : 0) +
+                (SGN_LDS ? (size_t)3 * D * 4 + 16 : 0);
     if (sm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k_index<LOGD, MODE, EMB, FULLP>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
