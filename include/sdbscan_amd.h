@@ -22,6 +22,8 @@
  *   sdb_get_index      <- the protected matrix_topK / matrix_topM (src/sDbscan.h:54-55)
  *   sdb_get_projections <- columns of the local MATRIX_FHT of rp_parIndex
  *                         (src/sDbscan.cpp:31, 74-91)
+ *   sdb_transform      <- transformData (src/Utilities.h:289-335) as loadtxtData
+ *                         calls it (src/Utilities.cpp:94)
  *
  * All entry points return 0 on success and a nonzero SDB_E* code otherwise;
  * sdb_last_error() describes the last failure.  No exception or exit() crosses
@@ -168,6 +170,14 @@ int sdb_get_projections(sdb_ctx* ctx, const int* ids, int n_ids, float* out);
 
 /* Transformed points (n x n_features) as used by the distance tiles. */
 int sdb_get_transformed(sdb_ctx* ctx, float* X_out);
+
+/* transformData (src/Utilities.h:289-335) of the loaded input under the
+ * parameters' metric, as loadtxtData applies it to a text dataset
+ * (src/Utilities.cpp:94): Cosine rows divided by their norm, Chi2 / JS rows
+ * by their sum (SDB_ENEGATIVE / SDB_EZERO with the reference's messages),
+ * L1 / L2 rows unchanged.  Writes the n x n_features result to X_out; the
+ * context keeps it as its transformed rows. */
+int sdb_transform(sdb_ctx* ctx, float* X_out);
 
 /* Their BF16 copy (n x n_features, round to nearest even) read by the BF16
  * distance tiles; for Chi2 the BF16 of their square roots (the tiles bound

@@ -77,6 +77,7 @@ _SIGS = {
     "sdb_get_index": ([_P, _P, _P], ctypes.c_int),
     "sdb_get_projections": ([_P, _P, ctypes.c_int, _P], ctypes.c_int),
     "sdb_get_transformed": ([_P, _P], ctypes.c_int),
+    "sdb_transform": ([_P, _P], ctypes.c_int),
     "sdb_get_transformed_bf16": ([_P, _P], ctypes.c_int),
     "sdb_get_neighbors": ([_P, _P, _P, _P], ctypes.c_int),
     "sdb_core_dist": ([_P, ctypes.c_int, _P], ctypes.c_int),
@@ -278,6 +279,15 @@ class Context:
         p = self.params
         X = np.zeros((p.n_points, p.n_features), np.float32)
         self._chk(self.lib.sdb_get_transformed(self.h, X.ctypes.data))
+        return X
+
+    def transform(self):
+        """transformData (src/Utilities.h:289-335) of the loaded input; returns
+        the transformed n x d rows."""
+        import numpy as np
+        p = self.params
+        X = np.zeros((p.n_points, p.n_features), np.float32)
+        self._chk(self.lib.sdb_transform(self.h, X.ctypes.data))
         return X
 
     def get_transformed_bf16(self):

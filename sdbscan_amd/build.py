@@ -5,6 +5,7 @@
 Outputs (next to this file):
   libsdbscan_amd.so          -- HIP kernels + C ABI (include/sdbscan_amd.h), gfx950
   sDbscan.cpython-*.so       -- pybind11 module `sDbscan` wrapping the C++ class
+  sdbscan                    -- command-line front end (the reference's main.cpp)
 """
 import os
 import subprocess
@@ -68,9 +69,20 @@ def build_module():
     return out
 
 
+def build_cli():
+    out = os.path.join(HERE, "sdbscan")
+    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp",
+           "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+           os.path.join(CSRC, "sDbscan.cpp"), os.path.join(CSRC, "sdb_main.cpp"),
+           "-L", HERE, "-lsdbscan_amd", "-Wl,-rpath,$ORIGIN", "-o", out]
+    run(cmd)
+    return out
+
+
 def main():
     build_lib()
     build_module()
+    build_cli()
 
 
 if __name__ == "__main__":

@@ -112,6 +112,11 @@ void sDbscan::set_precision(const std::string& p) {
     else throw std::invalid_argument("precision must be FP32 or BF16");
 }
 
+void sDbscan::transform_data(float* X) {
+    prepare(X);
+    check(sdb_transform(ctx, X), "transform");
+}
+
 void sDbscan::set_device(int dev) {
     if (ctx) { sdb_destroy(ctx); ctx = nullptr; }
     device = dev;

@@ -118,6 +118,10 @@ public:
 
     // Extensions of this build (no counterpart in the reference)
     void set_precision(const std::string& p);  // "FP32" (default) or "BF16" distance tiles
+    // transformData (src/Utilities.h:289-335) of an n_points x n_features host
+    // matrix, in place, on the device: what loadtxtData applies to a text
+    // dataset (src/Utilities.cpp:94) before main's fits transform it again
+    void transform_data(float* X);
     void set_device(int dev);
     sdb_stats get_stats() const;
     void get_index(std::vector<int>& topK_out, std::vector<int>& topM_out);

@@ -1395,6 +1395,15 @@ static int ensure_transformed(sdb_ctx* c) {
     return 0;
 }
 
+int sdb_transform(sdb_ctx* c, float* X_out) {
+    if (!c || !X_out) return SDB_EINVAL;
+    if (!c->X) return fail(c, SDB_ESTATE, "no input loaded");
+    HIPCHK(hipSetDevice(c->device));
+    int r = ensure_transformed(c);
+    if (r) return r;
+    return sdb_get_transformed(c, X_out);
+}
+
 // Deduplicated neighbourhood CSR of the sample-sequence hits (sorted by index)
 // with FP32 distances.
 static int build_neighbors_sng(sdb_ctx* c) {
