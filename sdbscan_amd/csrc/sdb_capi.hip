@@ -173,7 +173,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
                      &c->inv, &c->pos_hub, &c->pcnt, &c->poff, &c->pairs, &c->pseg, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
                      &c->scan_tmp, &c->total, &c->err_flag, &c->row_slot, &c->row_buf, &c->E,
-                     &c->colsum, &c->queue, &c->qcount, &c->tiles, &c->degree, &c->core,
+                     &c->colsum, &c->queue, &c->qcount, &c->qkeys, &c->qvals, &c->qsort_tmp, &c->tiles, &c->degree, &c->core,
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
                      &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
                      &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
@@ -794,6 +794,24 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     if (nq64 > (unsigned long long)c->qcap) return fail(c, SDB_ENOMEM, "recheck queue overflow");
     const int nq = (int)nq64;
     c->stats.t_dist_ms = elapsed(c->ev[4], c->ev[5]);
+    // a long queue is sorted by candidate point first: the recheck then reads
+    // a candidate's FP32 row from memory once and from L2 for the rest of its
+    // pairs (the pairs' order does not change any result: hits are ORed into
+    // E and added to colsum)
+    if (nq >= (1 << 14)) {
+        ENSURE(c->qkeys, (size_t)nq * 8);
+        ENSURE(c->qvals, (size_t)nq * 8);
+        uint32_t* k0 = P_<uint32_t>(c->qkeys);
+        LCHK(launch_queue_keys(da.queue, nq, da.lists, da.M, k0, st));
+        hipcub::DoubleBuffer<uint32_t> kb(k0, k0 + nq);
+        hipcub::DoubleBuffer<unsigned long long> vb(da.queue, (unsigned long long*)c->qvals.p);
+        const int nbits = std::max(1, 32 - __builtin_clz((unsigned)std::max<long long>(1, n - 1)));
+        size_t tb = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, nq, 0, nbits, st));
+        ENSURE(c->qsort_tmp, std::max<size_t>(tb, 16));
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->qsort_tmp.p, tb, kb, vb, nq, 0, nbits, st));
+        da.queue = vb.Current();
+    }
     HIPCHK(hipMemsetAsync(c->stats64.p, 0, 64, st));
     LCHK(launch_recheck_k(da, P_<int>(c->topK), q.topK, q.n_proj, prec, band32, nq, (unsigned long long*)c->stats64.p, st));
 

@@ -103,18 +103,61 @@ __device__ __forceinline__ float wave_sum_f32(float v) {
 // FP64 distance of two FP32 rows in a fixed order: per lane a sequential sum
 // over k = lane, lane + 64, ..., then the xor butterfly (offsets 32..1);
 // every lane returns the same value.  The decision value of the recheck.
-__device__ __forceinline__ double dist64_wave(const float* x, const float* y, int d, int metric, int lane) {
-    double s = 0.0;
-    for (int i = lane; i < d; i += 64) {
-        const double xv = x[i], yv = y[i];
-        if (metric == kCosine) s = s + xv * yv;
-        else if (metric == kL1) s = s + fabs(xv - yv);
-        else if (metric == kL2) { double t = xv - yv; s = s + t * t; }
-        else if (metric == kChi2) { if (x[i] > 0.0f && y[i] > 0.0f) s = s + (xv * yv) / (xv + yv); }
-        else {
-            const double xe = xv + 1e-6, ye = yv + 1e-6, t = xe + ye, ln2 = 0.69314718055994530942;
-            s = s + (xe * (log(t / xe) / ln2) + ye * (log(t / ye) / ln2));
+// s <- op(s, x[i], y[i]) for i = lane, lane + 64, ... < d, in that order, with
+// the loads of U iterations issued before their terms: the arithmetic of the
+// plain loop without one memory round trip per iteration.
+template <int U, class T, class Op>
+__device__ __forceinline__ T wave_row_fold(const float* x, const float* y, int d, int lane, T s, Op op) {
+    int i = lane;
+    for (; i + (U - 1) * 64 < d; i += U * 64) {
+        float xv[U], yv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xv[u] = x[i + 64 * u];
+            yv[u] = y[i + 64 * u];
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) s = op(s, xv[u], yv[u]);
+    }
+    float xv[U], yv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int k = i + 64 * u;
+        xv[u] = k < d ? x[k] : 0.0f;
+        yv[u] = k < d ? y[k] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (i + 64 * u < d) s = op(s, xv[u], yv[u]);
+    return s;
+}
+
+template <int METRIC>
+__device__ __forceinline__ double sum64_wave(const float* x, const float* y, int d, int lane) {
+    return wave_row_fold<8>(x, y, d, lane, 0.0, [](double s, float a, float b) {
+        const double xv = a, yv = b;
+        if constexpr (METRIC == kCosine) return s + xv * yv;
+        else if constexpr (METRIC == kL1) return s + fabs(xv - yv);
+        else if constexpr (METRIC == kL2) {
+            const double t = xv - yv;
+            return s + t * t;
+        } else if constexpr (METRIC == kChi2) {
+            return (a > 0.0f && b > 0.0f) ? s + (xv * yv) / (xv + yv) : s;
+        } else {
+            const double xe = xv + 1e-6, ye = yv + 1e-6, t = xe + ye, ln2 = 0.69314718055994530942;
+            return s + (xe * (log(t / xe) / ln2) + ye * (log(t / ye) / ln2));
+        }
+    });
+}
+
+__device__ __forceinline__ double dist64_wave(const float* x, const float* y, int d, int metric, int lane) {
+    double s;
+    switch (metric) {
+        case kCosine: s = sum64_wave<kCosine>(x, y, d, lane); break;
+        case kL1: s = sum64_wave<kL1>(x, y, d, lane); break;
+        case kL2: s = sum64_wave<kL2>(x, y, d, lane); break;
+        case kChi2: s = sum64_wave<kChi2>(x, y, d, lane); break;
+        default: s = sum64_wave<kJS>(x, y, d, lane); break;
     }
     s = wave_sum_f64_tree(s);
     if (metric == kCosine) return 1.0 - s;

@@ -93,6 +93,9 @@ struct sdb_ctx {
     std::vector<int> h_grp_ptr;
     // find_core
     DevBuf E, colsum, queue, qcount, tiles, degree, core, stats64;
+    // the queued pairs sorted by candidate point for the recheck: two key
+    // buffers, the alternate value buffer and the sort's scratch
+    DevBuf qkeys, qvals, qsort_tmp;
     long long qcap = 0;
     int tile_rows = 0;   // distance tile rows (0: the default for the row length)
     bool cored = false;

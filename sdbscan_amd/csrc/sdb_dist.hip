@@ -10,14 +10,13 @@
 // precision makes the same decision.  Hits are stored only for the first
 // occurrence of a candidate among the point's slots (F masks), which is the
 // reference's per-point dedup (src/sDbscan.cpp:260-282).
-#include <cstdlib>
-
 #include "sdb_kernels.hpp"
 
 namespace sdb {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 // The Chi2 term x y / (x + y) of the FP32 tiles through the hardware
 // reciprocal (1 ulp): about 5u relative per term against 3u for a correctly
@@ -318,7 +317,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
 
     // candidate rows -> LDS with BATCH 16-byte loads in flight per thread (the
     // list's point ids went through LDS first, so no row load waits on the
-    // load of its id)
+    // load of its id).  The candidate rows, the F masks and the E words are
+    // touched once per tile: their accesses are non-temporal, so that they
+    // do not push the query block's rows out of the last-level cache
+    // (measured 0.7-1.1 ms faster at 8M x 784)
     __syncthreads();
     {
         constexpr int BATCH = 10;
@@ -331,7 +333,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
                 // of the padding steps is a select
                 const int v = min(v0 + u * WAVES * 64, total - 1);
                 const int row = v / (NS * 4), cv = v - row * (NS * 4);
-                const uint4 x = ((const uint4*)(Xc + (size_t)pts_lds[row] * rowb))[min(cv, vpr - 1)];
+                const u32x4_t* src = ((const u32x4_t*)(Xc + (size_t)pts_lds[row] * rowb)) + min(cv, vpr - 1);
+                const uint4 x = __builtin_bit_cast(uint4, __builtin_nontemporal_load(src));
                 xs[u] = cv < vpr ? x : uint4{0u, 0u, 0u, 0u};
             }
 #pragma unroll
@@ -361,7 +364,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             const bool has_next = b + WAVES < nblk;
             uint32_t Fw[EW];
 #pragma unroll
-            for (int x = 0; x < EW; ++x) Fw[x] = (valid && x < a.W) ? a.F[(size_t)ent * a.W + x] : 0u;
+            for (int x = 0; x < EW; ++x)
+                Fw[x] = (valid && x < a.W) ? __builtin_nontemporal_load(&a.F[(size_t)ent * a.W + x]) : 0u;
             const float eq = a.rowerr[ent_point(ent, a.magicS)];
             uint32_t ent2;
             bool valid2;
@@ -437,7 +441,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             uint32_t mine = hw[0];
 #pragma unroll
             for (int x = 1; x < EW; ++x) mine = g == x ? hw[x] : mine;
-            if (valid && g < a.W) a.E[(size_t)ent * a.W + g] = mine;
+            if (valid && g < a.W) __builtin_nontemporal_store(mine, &a.E[(size_t)ent * a.W + g]);
             if (__any(unc != 0u)) {
                 // one queue reservation per wave: lane offsets by a prefix sum
                 unsigned long long pos = wave_reserve(a.qcount, __popc(unc), lane);
@@ -1020,8 +1024,7 @@ int launch_dist_valu(const DistArgs& a, int ntiles, hipStream_t st) {
 // stats[0]: FP32 rechecks, stats[1]: FP64 rechecks, stats[2]: hits added,
 // stats[4]: FP64 distances within 1e-6 of eps.
 __device__ __forceinline__ float dot32_wave(const float* x, const float* y, int d, int lane) {
-    float s = 0.0f;
-    for (int i = lane; i < d; i += 64) s = fmaf(x[i], y[i], s);
+    float s = wave_row_fold<8>(x, y, d, lane, 0.0f, [](float t, float a, float b) { return fmaf(a, b, t); });
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) s = s + __shfl_xor(s, m, kWave);
     return s;
@@ -1029,11 +1032,9 @@ __device__ __forceinline__ float dot32_wave(const float* x, const float* y, int 
 
 // sum of x y / (x + y) over the entries positive in both rows, FP32
 __device__ __forceinline__ float chi2sum32_wave(const float* x, const float* y, int d, int lane) {
-    float s = 0.0f;
-    for (int i = lane; i < d; i += 64) {
-        const float xv = x[i], yv = y[i];
-        if (xv > 0.0f && yv > 0.0f) s = s + xv * yv / (xv + yv);
-    }
+    float s = wave_row_fold<8>(x, y, d, lane, 0.0f, [](float t, float a, float b) {
+        return (a > 0.0f && b > 0.0f) ? t + a * b / (a + b) : t;
+    });
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) s = s + __shfl_xor(s, m, kWave);
     return s;
@@ -1173,6 +1174,23 @@ int launch_pair_sample(const DistArgs& a, const int* topK, int K, int P, long lo
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_pair_sample, dim3((unsigned)blocks), dim3(256), 0, st, a, topK, K, P, n, precision,
                        (uint64_t)seed, n_samples, pa, pb, fast, exact);
+    return (int)hipGetLastError();
+}
+
+// Sort key of every queued pair: its candidate point (lists[l][col]).
+__global__ void k_queue_keys(const unsigned long long* __restrict__ queue, int nq, const int* __restrict__ lists,
+                             int M, uint32_t* __restrict__ keys) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nq) return;
+    const unsigned long long qe = queue[e];
+    const int l = (int)((qe >> 8) & 0xffffffu), col = (int)(qe & 0xffu);
+    keys[e] = (uint32_t)lists[(size_t)l * M + col];
+}
+
+int launch_queue_keys(const unsigned long long* queue, int nq, const int* lists, int M, uint32_t* keys,
+                      hipStream_t st) {
+    if (nq <= 0) return 0;
+    hipLaunchKernelGGL(k_queue_keys, dim3((nq + 255) / 256), dim3(256), 0, st, queue, nq, lists, M, keys);
     return (int)hipGetLastError();
 }
 

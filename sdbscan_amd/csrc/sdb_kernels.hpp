@@ -141,6 +141,8 @@ int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, 
                       uint32_t* F, hipStream_t st);
 int launch_group_count(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
                        int Gx, int* wgcnt, hipStream_t st);
+int launch_queue_keys(const unsigned long long* queue, int nq, const int* lists, int M, uint32_t* keys,
+                      hipStream_t st);
 int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
                          int Gx, const int* wgoff, uint32_t* grp, int* grp_ptr, hipStream_t st);
 

@@ -57,18 +57,17 @@ __global__ __launch_bounds__(256) void k_csr_dist(const float* X, int ldx, int d
         const float* x = X + (size_t)a * ldx;
         for (long long j = ptr[a]; j < ptr[a + 1]; ++j) {
             const float* y = X + (size_t)idx[j] * ldx;
-            float s = 0.0f;
-            for (int i = lane; i < d; i += 64) {
-                const float xv = x[i], yv = y[i];
-                if (metric == kCosine) s = fmaf(xv, yv, s);
-                else if (metric == kL1) s = s + fabsf(xv - yv);
-                else if (metric == kL2) { float t = xv - yv; s = fmaf(t, t, s); }
-                else if (metric == kChi2) { if (xv > 0.0f && yv > 0.0f) s = s + xv * yv / (xv + yv); }
-                else {
-                    const float xe = xv + 1e-6f, ye = yv + 1e-6f, u = xe + ye;
-                    s = s + xe * (logf(u / xe) * 1.4426950408889634f) + ye * (logf(u / ye) * 1.4426950408889634f);
+            float s = wave_row_fold<8>(x, y, d, lane, 0.0f, [metric](float t, float xv, float yv) {
+                if (metric == kCosine) return fmaf(xv, yv, t);
+                if (metric == kL1) return t + fabsf(xv - yv);
+                if (metric == kL2) {
+                    const float df = xv - yv;
+                    return fmaf(df, df, t);
                 }
-            }
+                if (metric == kChi2) return (xv > 0.0f && yv > 0.0f) ? t + xv * yv / (xv + yv) : t;
+                const float xe = xv + 1e-6f, ye = yv + 1e-6f, u = xe + ye;
+                return t + xe * (logf(u / xe) * 1.4426950408889634f) + ye * (logf(u / ye) * 1.4426950408889634f);
+            });
 #pragma unroll
             for (int m = 32; m >= 1; m >>= 1) s = s + __shfl_xor(s, m, kWave);
             float r;

@@ -337,6 +337,18 @@ def test_chi2_bf16_prefilter_matches_oracle(ctx, n, d, K, M, eps, minPts):
     assert np.abs(xb - rne).max() <= 1
 
 
+def test_sorted_recheck_queue_matches_oracle(ctx):
+    """A populated error band (mixture noise 0.2, eps 0.27 inside the
+    within-cluster distances): more than 2^14 pairs go through the recheck,
+    which decides them in candidate order (the queue sorted by candidate
+    point); degrees, cores and labels equal the oracle's FP64 decision."""
+    X, _ = datagen.mixture(40000, 784, sigma=0.2, seed=1002)
+    print("populated band n=40000 cosine bf16 eps 0.27", flush=True)
+    p, bits, R, tk, tm, Xt, st = full_check(ctx, X, S.COSINE, 0.27, 20, K=10, M=100, precision=S.BF16)
+    print("queue", st["n_queue"], "fp32", st["n_recheck32"], "fp64", st["n_recheck64"], flush=True)
+    assert st["n_queue"] >= 1 << 14
+
+
 def test_chi2_bf16_queue_overflow_falls_back_to_fp32_tiles(ctx, monkeypatch):
     """More pairs near eps than the recheck queue may hold: the pass is
     redone on the FP32 tiles, with the same decisions."""
