@@ -132,6 +132,26 @@ __device__ __forceinline__ T wave_row_fold(const float* x, const float* y, int d
     return s;
 }
 
+// The same over one row: s <- op(s, x[i]).
+template <int U, class T, class Op>
+__device__ __forceinline__ T wave_row_fold1(const float* x, int d, int lane, T s, Op op) {
+    int i = lane;
+    for (; i + (U - 1) * 64 < d; i += U * 64) {
+        float xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[i + 64 * u];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s = op(s, xv[u]);
+    }
+    float xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = i + 64 * u < d ? x[i + 64 * u] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (i + 64 * u < d) s = op(s, xv[u]);
+    return s;
+}
+
 template <int METRIC>
 __device__ __forceinline__ double sum64_wave(const float* x, const float* y, int d, int lane) {
     return wave_row_fold<8>(x, y, d, lane, 0.0, [](double s, float a, float b) {

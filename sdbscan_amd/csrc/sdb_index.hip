@@ -33,16 +33,20 @@ __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, 
         float scale = 1.0f;
         bool scaled = false;
         if (metric == kCosine) {
-            double acc = 0.0;
-            for (int i = lane; i < d; i += kWave) { double v = x[i]; acc = acc + v * v; }
+            const double acc = wave_row_fold1<8>(x, d, lane, 0.0, [](double s, float f) {
+                const double v = f;
+                return s + v * v;
+            });
             double t = wave_sum_f64_tree(acc);
             scale = (float)sqrt(t);
             scaled = true;
             if (!(scale > 0.0f) && lane == 0) atomicOr(err, 2);
         } else if (metric == kChi2 || metric == kJS) {
-            double acc = 0.0;
             int neg = 0;
-            for (int i = lane; i < d; i += kWave) { float v = x[i]; neg |= v < 0.0f; acc = acc + (double)v; }
+            const double acc = wave_row_fold1<8>(x, d, lane, 0.0, [&neg](double s, float v) {
+                neg |= v < 0.0f;
+                return s + (double)v;
+            });
             if (__any(neg) && lane == 0) atomicOr(err, 1);
             double t = wave_sum_f64_tree(acc);
             scale = (float)t;
@@ -52,6 +56,7 @@ __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, 
         float* o = Xn + p * ldx;
         uint16_t* ob = Xb ? Xb + p * ldx : nullptr;
         float sq = 0.0f;
+#pragma unroll 4
         for (int i = lane; i < ldx; i += kWave) {
             float v = 0.0f;
             if (i < d) v = scaled ? x[i] / scale : x[i];
