@@ -798,19 +798,25 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     // a candidate's FP32 row from memory once and from L2 for the rest of its
     // pairs (the pairs' order does not change any result: hits are ORed into
     // E and added to colsum)
+    // (an optimisation only: without the memory for it the pairs are
+    // rechecked in queue order)
     if (nq >= (1 << 14)) {
-        ENSURE(c->qkeys, (size_t)nq * 8);
-        ENSURE(c->qvals, (size_t)nq * 8);
-        uint32_t* k0 = P_<uint32_t>(c->qkeys);
-        LCHK(launch_queue_keys(da.queue, nq, da.lists, da.M, k0, st));
-        hipcub::DoubleBuffer<uint32_t> kb(k0, k0 + nq);
-        hipcub::DoubleBuffer<unsigned long long> vb(da.queue, (unsigned long long*)c->qvals.p);
         const int nbits = std::max(1, 32 - __builtin_clz((unsigned)std::max<long long>(1, n - 1)));
         size_t tb = 0;
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, nq, 0, nbits, st));
-        ENSURE(c->qsort_tmp, std::max<size_t>(tb, 16));
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->qsort_tmp.p, tb, kb, vb, nq, 0, nbits, st));
-        da.queue = vb.Current();
+        hipcub::DoubleBuffer<uint32_t> kb0(nullptr, nullptr);
+        hipcub::DoubleBuffer<unsigned long long> vb0(nullptr, nullptr);
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb0, vb0, nq, 0, nbits, st));
+        if (!ensure(c, c->qkeys, (size_t)nq * 8) && !ensure(c, c->qvals, (size_t)nq * 8) &&
+            !ensure(c, c->qsort_tmp, std::max<size_t>(tb, 16))) {
+            uint32_t* k0 = P_<uint32_t>(c->qkeys);
+            LCHK(launch_queue_keys(da.queue, nq, da.lists, da.M, k0, st));
+            hipcub::DoubleBuffer<uint32_t> kb(k0, k0 + nq);
+            hipcub::DoubleBuffer<unsigned long long> vb(da.queue, (unsigned long long*)c->qvals.p);
+            HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->qsort_tmp.p, tb, kb, vb, nq, 0, nbits, st));
+            da.queue = vb.Current();
+        } else {
+            c->err.clear();  // the failed allocation is not an error of this call
+        }
     }
     HIPCHK(hipMemsetAsync(c->stats64.p, 0, 64, st));
     LCHK(launch_recheck_k(da, P_<int>(c->topK), q.topK, q.n_proj, prec, band32, nq, (unsigned long long*)c->stats64.p, st));

@@ -86,3 +86,34 @@ def test_cli_reports_negative_chi2_input(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 1
     assert "Error: X is not non-negative !" in r.stderr
+
+
+def test_transform_entry_point_matches_the_oracle(ctx):
+    """sdb_transform (transformData, src/Utilities.h:289-335, as loadtxtData
+    applies it): Cosine rows divided by their norm, Chi2 / JS rows by their
+    sum, L2 rows unchanged -- bit-equal to the oracle's transform -- and the
+    reference's errors for a negative entry and a zero point."""
+    Xc, _ = datagen.mixture(3000, 100, seed=7)
+    Xh, _ = datagen.dirichlet(3000, 100, k=5, seed=8)
+    for metric, X in [(S.COSINE, Xc), (S.CHI2, Xh), (S.JS, Xh), (S.L2, Xc)]:
+        print("sdb_transform metric", metric, flush=True)
+        ctx.set_params(3000, 100, n_proj=256, topK=5, topM=20, metric=metric, ker_n_features=200)
+        ctx.load(X)
+        got = ctx.transform()
+        want, rc = O.transform(X, metric)
+        assert rc == 0
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), metric
+    Xn = Xh.copy()
+    Xn[17, 3] = -0.5
+    ctx.set_params(3000, 100, n_proj=256, topK=5, topM=20, metric=S.CHI2, ker_n_features=200)
+    ctx.load(Xn)
+    with pytest.raises(S.SdbError) as e:
+        ctx.transform()
+    assert e.value.code == 4 and "not non-negative" in str(e.value)
+    Xz = Xc.copy()
+    Xz[5] = 0.0
+    ctx.set_params(3000, 100, n_proj=256, topK=5, topM=20, metric=S.COSINE)
+    ctx.load(Xz)
+    with pytest.raises(S.SdbError) as e:
+        ctx.transform()
+    assert e.value.code == 5
