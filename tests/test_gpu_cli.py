@@ -28,30 +28,38 @@ def lround(x):
     return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
 
 
-def test_cli_label_files_match_the_oracle(tmp_path):
-    n, d, K, M, P, minPts, seed = 2000, 64, 5, 40, 256, 10, 7
-    eps0, prob = np.float32(0.2), np.float32(0.1)
-    X, _ = datagen.mixture(n, d, k=5, sigma=0.3, seed=211)
+@pytest.mark.parametrize("dist", ["Cosine", "Chi2"])
+def test_cli_label_files_match_the_oracle(tmp_path, dist):
+    n, K, M, P, minPts, seed = 2000, 5, 40, 256, 10, 7
+    if dist == "Cosine":
+        d, eps0, metric = 64, np.float32(0.2), S.COSINE
+        X, _ = datagen.mixture(n, d, k=5, sigma=0.3, seed=211)
+    else:
+        # Chi2: kernel features default to n_features (square-root features),
+        # interval sampling 0.4
+        d, eps0, metric = 32, np.float32(0.3), S.CHI2
+        X, _ = datagen.dirichlet(n, d, k=5, n_boost=6, seed=212)
+    prob = np.float32(0.1)
     data = tmp_path / "data.txt"
     np.savetxt(data, X, fmt="%.9g")
     out = str(tmp_path / "run")
-    cmd = [CLI, "--n_points", n, "--n_features", d, "--eps", "0.2", "--minPts", minPts, "--topK", K,
-           "--topM", M, "--n_proj", P, "--distance", "Cosine", "--samplingProb", "0.1",
+    cmd = [CLI, "--n_points", n, "--n_features", d, "--eps", f"{float(eps0):.9g}", "--minPts", minPts,
+           "--topK", K, "--topM", M, "--n_proj", P, "--distance", dist, "--samplingProb", "0.1",
            "--random_seed", seed, "--verbose", "--output", out, "--dataset", data]
-    print("sdbscan CLI, cosine 2000 x 64, 10 eps steps", flush=True)
+    print(f"sdbscan CLI, {dist} {n} x {d}, 10 eps steps", flush=True)
     r = subprocess.run([str(c) for c in cmd], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr[-2000:]
     assert r.stdout.count("sDBSCAN Wall Clock = ") == 10
     assert r.stdout.count("sngDBSCAN Wall Clock = ") == 10
 
     # the rows the fits see: the loader's transform, then the fit's own
-    X1, rc = O.transform(X, S.COSINE)
+    X1, rc = O.transform(X, metric)
     assert rc == 0
-    Xt, rc = O.transform(X1, S.COSINE)
+    Xt, rc = O.transform(X1, metric)
     assert rc == 0
     D = S.fht_dim(P, d)
     bits = O.hd3_bits(3 * D, seed)
-    proj = O.project(O.embed(Xt, S.COSINE, d, 0.4, None), bits, D, P)
+    proj = O.project(O.embed(Xt, metric, d, 0.4, None), bits, D, P)
     tk, tm = O.topk(proj, K), O.topm(proj, M)
 
     S_ = int(math.ceil(1.0 * n * float(prob)))
@@ -60,15 +68,15 @@ def test_cli_label_files_match_the_oracle(tmp_path):
     n_clustered = 0
     for i in range(10):
         e = np.float32(eps0 + np.float32(i) * np.float32(0.01))   # float arithmetic of main
-        tag = f"{out}_Cosine_Eps_{lround(np.float32(1000) * e)}_MinPts_{minPts}"
+        tag = f"{out}_{dist}_Eps_{lround(np.float32(1000) * e)}_MinPts_{minPts}"
         lab = np.loadtxt(f"{tag}_KerFeatures_{d}_NumProj_{P}_TopM_{M}_TopK_{K}", dtype=np.int64)
-        oc, od, (ptr, idx, dd), _, _ = O.find_core(Xt, S.COSINE, tk, tm, float(e), minPts, mode=1)
+        oc, od, (ptr, idx, dd), _, _ = O.find_core(Xt, metric, tk, tm, float(e), minPts, mode=1)
         olab, onc = O.form_cluster(oc, ptr, idx)
         assert np.array_equal(lab, olab), f"sDbscan labels differ at eps {e}"
         n_clustered += int((olab >= 0).sum())
 
         slab = np.loadtxt(f"{tag}_Prob_{lround(np.float32(1000) * prob)}", dtype=np.int64)
-        sc, sd, (sptr, sidx, sdd) = O.sng_find_core(Xt, S.COSINE, seq, float(e), minPts, win_end=win)
+        sc, sd, (sptr, sidx, sdd) = O.sng_find_core(Xt, metric, seq, float(e), minPts, win_end=win)
         solab, _ = O.form_cluster(sc, sptr, sidx)
         assert np.array_equal(slab, solab), f"sngDbscan labels differ at eps {e}"
     assert n_clustered > 0   # the sweep is not all noise
