@@ -173,7 +173,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
                      &c->inv, &c->pos_hub, &c->pcnt, &c->poff, &c->pairs, &c->pseg, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
                      &c->scan_tmp, &c->total, &c->err_flag, &c->row_slot, &c->row_buf, &c->E,
-                     &c->colsum, &c->queue, &c->qcount, &c->qkeys, &c->qvals, &c->qsort_tmp, &c->tiles, &c->degree, &c->core,
+                     &c->colsum, &c->queue, &c->qcount, &c->qkeys, &c->qvals, &c->qsort_tmp, &c->rnorm2, &c->rnorm2_max, &c->tiles, &c->degree, &c->core,
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
                      &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
                      &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
@@ -219,6 +219,7 @@ int sdb_set_params(sdb_ctx* c, const sdb_params* p) {
     // the transformed rows depend on the metric, the precision and the kernel
     // parameters: made again by the next pass that needs them
     c->transformed = false;
+    c->rnorm2_of = nullptr;
     c->prm = q;
     c->have_params = true;
     c->indexed = false;
@@ -236,6 +237,7 @@ static int set_input_dims(sdb_ctx* c, long long n, int d) {
     c->ldx = (d + 31) / 32 * 32;
     c->indexed = c->cored = c->have_nb = false;
     c->transformed = c->sng = c->clustered = false;
+    c->rnorm2_of = nullptr;
     return 0;
 }
 
@@ -586,6 +588,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     c->stats.t_index_ms = elapsed(c->ev[1], c->ev[2]);
     c->indexed = true;
     c->transformed = true;
+    c->rnorm2_of = nullptr;  // the rows were written again
     c->transformed_prec = q.precision;
     c->cored = false;
     c->have_nb = false;
@@ -670,10 +673,36 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     const long long n = c->n;
     const int M = c->Mt, W = c->W, S = c->S, L2 = c->L2;
     hipStream_t st = c->st;
+    // L2 on FP32 MFMA tiles through |x - y|^2 = |x|^2 + |y|^2 - 2 x.y, with a
+    // band of kappa (|x|^2 + |y|^2) per pair (k_dist_mfma); rows far from the
+    // origin relative to eps would widen that band, and the VALU tiles, whose
+    // error is relative to the distance itself, keep those inputs
+    bool l2x = false;
+    double l2_kappa = 0.0, l2_maxn = 0.0;
+    if (q.metric == SDB_L2 && M <= 128 && !std::getenv("SDB_L2_VALU")) {
+        // the norms of the rows are kept until the rows change
+        if (c->rnorm2_of != P_<float>(c->Xn) || !c->transformed) {
+            ENSURE(c->rnorm2, (size_t)n * 4);
+            ENSURE(c->rnorm2_max, 4);
+            HIPCHK(hipMemsetAsync(c->rnorm2_max.p, 0, 4, st));
+            LCHK(launch_row_norm2(P_<float>(c->Xn), n, c->d, c->ldx, P_<float>(c->rnorm2),
+                                  P_<uint32_t>(c->rnorm2_max), st));
+            uint32_t mb = 0;
+            HIPCHK(hipMemcpyAsync(&mb, c->rnorm2_max.p, 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            std::memcpy(&c->rnorm2_maxv, &mb, 4);
+            c->rnorm2_of = P_<float>(c->Xn);
+        }
+        l2_maxn = c->rnorm2_maxv;
+        l2_kappa = 1.02 * (2.0 * c->d + 16.0) * std::ldexp(1.0, -23);
+        const double eps2 = (double)eps * (double)eps;
+        l2x = std::isfinite(l2_maxn) && 2.0 * l2_kappa * l2_maxn <= 0.05 * eps2;
+    }
     // Cosine on MFMA tiles at either precision; Chi2 with BF16 on the BF16
-    // tiles through bounds on chi2 (compute_bands); everything else on the
-    // VALU tiles in FP32
-    const bool mfma = (q.metric == SDB_COSINE || (q.metric == SDB_CHI2 && precision == SDB_BF16)) && M <= 128;
+    // tiles through bounds on chi2 (compute_bands); L2 as above; everything
+    // else on the VALU tiles in FP32
+    const bool mfma = (q.metric == SDB_COSINE || (q.metric == SDB_CHI2 && precision == SDB_BF16) || l2x) && M <= 128;
+    c->l2x_used = l2x;
     const int prec = mfma ? precision : SDB_FP32;
     const bool prefilter = q.metric == SDB_CHI2 && prec == SDB_BF16;
 
@@ -740,6 +769,15 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     da.lo = prefilter ? ((double)eps - 1e-6) / 2.0 - band : (double)eps - band;
     da.hi = (double)eps + band;
     cosine_cuts(da.lo, da.hi, &da.s_hit, &da.s_near);
+    if (l2x) {
+        da.rnorm2 = P_<float>(c->rnorm2);
+        da.eps2 = (double)eps * (double)eps;
+        da.kappa = l2_kappa;
+        // FP64 rounding of the recheck's distance stays far inside this margin
+        da.eps0 = 1e-9 * da.eps2 + 1e-300;
+        // the widest pair band, as a distance near eps
+        band = (2.0 * l2_kappa * l2_maxn + da.eps0) / (2.0 * (double)eps);
+    }
     da.magicS = ((1ull << 40) + (unsigned long long)S - 1) / (unsigned long long)S;
     da.n_ent = n * S;
     // BF16 rows in one LDS tile (the pipelined tiles): a band per pair from
@@ -855,6 +893,7 @@ static int sample_pairs_into(sdb_ctx* c, long long n_samples, unsigned long long
     da.ldx = c->ldx; da.d = c->d; da.metric = q.metric;
     da.M = c->Mt;
     da.lists = P_<int>(c->lists);
+    if (q.metric == SDB_L2 && c->l2x_used) da.rnorm2 = P_<float>(c->rnorm2);
     const size_t ns = (size_t)std::max<long long>(n_samples, 1);
     double* ex = (double*)base;
     float* fs = (float*)(ex + ns);
@@ -1415,6 +1454,7 @@ static int ensure_transformed(sdb_ctx* c) {
     if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
     if (errf & 2) return fail(c, SDB_EZERO, "Error: There is an zero point !");
     c->transformed = true;
+    c->rnorm2_of = nullptr;  // the rows were written again
     c->transformed_prec = q.precision;
     return 0;
 }

@@ -96,6 +96,14 @@ struct sdb_ctx {
     // the queued pairs sorted by candidate point for the recheck: two key
     // buffers, the alternate value buffer and the sort's scratch
     DevBuf qkeys, qvals, qsort_tmp;
+    // squared row norms for the L2 MFMA tiles and their maximum; whether the
+    // last distance pass used those tiles (the accuracy sample follows it)
+    DevBuf rnorm2, rnorm2_max;
+    bool l2x_used = false;
+    // the rows the norms were computed from (cleared whenever the rows are
+    // transformed again) and their largest norm
+    const float* rnorm2_of = nullptr;
+    float rnorm2_maxv = 0.0f;
     long long qcap = 0;
     int tile_rows = 0;   // distance tile rows (0: the default for the row length)
     bool cored = false;

@@ -10,6 +10,8 @@
 // precision makes the same decision.  Hits are stored only for the first
 // occurrence of a candidate among the point's slots (F masks), which is the
 // reference's per-point dedup (src/sDbscan.cpp:260-282).
+#include <cstdlib>
+
 #include "sdb_kernels.hpp"
 
 namespace sdb {
@@ -56,7 +58,13 @@ __device__ __forceinline__ unsigned long long queue_entry(uint32_t ent, int l, i
 // 16 bytes of query row (l & 15) and of candidate column (l & 15) at k-group
 // (l >> 4) per step; A and B use the same k assignment, so the sum over k is
 // the dot product whatever order the unit uses internally.
-template <bool BF, int QT, int NT, int WAVES, int PF, int BATCH>
+// L2X (FP32 rows, metric L2): the tiles compute x.y and decide on
+// D = fl(fl(|x|^2 + |y|^2) - 2 x.y) against a per-pair band.  With the
+// squared norms rounded once to FP32 (u each) and the MFMA dot within
+// 2(d + 2)u sum|x_i y_i| <= (d + 2)u (|x|^2 + |y|^2) of x.y, the two
+// roundings of D add at most 3u (|x|^2 + |y|^2), so |D - ||x - y||^2| <=
+// (2d + 8)u (|x|^2 + |y|^2); the host's kappa = 1.02 (2d + 16)u covers it.
+template <bool BF, int QT, int NT, int WAVES, int PF, int BATCH, bool L2X>
 __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     constexpr int SZ = BF ? 2 : 4;
     constexpr int SE = 64 / SZ;  // elements per 64-byte k-step
@@ -75,9 +83,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     const double lo = a.lo, hi = a.hi;
 
     int* pts_lds = col_lds + M;
+    float* cn_lds = (float*)(pts_lds + M);  // L2X: squared norms of the candidates
     if (tid < M) {  // M <= 256 < WAVES * 64
         col_lds[tid] = 0;
         pts_lds[tid] = Lm[tid];
+        if constexpr (L2X) cn_lds[tid] = a.rnorm2[Lm[tid]];
     }
     int colacc[NT];
 #pragma unroll
@@ -89,12 +99,14 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
         uint32_t ent[QT];
         long long pa[QT];
         bool valid[QT];
+        float nq[QT];  // L2X: squared norm of the lane's query row
 #pragma unroll
         for (int q = 0; q < QT; ++q) {
             int rr = beg + set * rows_per_set + (w * QT + q) * 16 + r16;
             valid[q] = rr < end;
             ent[q] = a.grp[valid[q] ? rr : beg];
             pa[q] = (long long)(ent[q] / (uint32_t)a.S);
+            nq[q] = L2X ? a.rnorm2[pa[q]] : 0.0f;
         }
         f32x4_t acc[QT][NT];
 #pragma unroll
@@ -196,10 +208,21 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
                     const uint32_t fword = __shfl(Fw[t >> 1], row, kWave);
                     const int vr = __shfl((int)valid[q], row, kWave);
                     const bool act = vr && col < M && ((fword >> (col & 31)) & 1u);
-                    const float dd = 1.0f - acc[q][t][i];
-                    const double dv = (double)dd;
-                    const bool hit = act && dv < lo;
-                    const bool unc = act && !(dv < lo) && !(dv > hi);
+                    bool hit, unc;
+                    if constexpr (L2X) {
+                        const float nqr = __shfl(nq[q], row, kWave);
+                        const float ncc = cn_lds[col < M ? col : M - 1];
+                        const float D2 = (nqr + ncc) - 2.0f * acc[q][t][i];
+                        const double band = a.kappa * ((double)nqr + (double)ncc) + a.eps0;
+                        const double dv = (double)D2;
+                        hit = act && dv < a.eps2 - band;
+                        unc = act && !hit && !(dv > a.eps2 + band);
+                    } else {
+                        const float dd = 1.0f - acc[q][t][i];
+                        const double dv = (double)dd;
+                        hit = act && dv < lo;
+                        unc = act && !(dv < lo) && !(dv > hi);
+                    }
                     const unsigned long long bh = __ballot(hit);
                     if ((r16 & 3) == i)
                         Ew[t >> 1] |= (uint32_t)((bh >> (16 * (r16 >> 2))) & 0xffffull) << (16 * (t & 1));
@@ -255,12 +278,18 @@ __device__ __forceinline__ uint32_t spread4(uint32_t n) {
     return (n * 0x00204081u) & 0x01010101u;
 }
 
-template <int NT, int WAVES, int NS, bool EXACT, int PADB>
+// F32L2: the same tiles on FP32 rows for L2, with 4 x v_mfma_f32_16x16x4f32
+// per 64-byte k-step (lane (r16, g) feeds float c of its 16 bytes to the c-th
+// MFMA, on both operands) and the decision of k_dist_mfma's L2X epilogue on
+// D = fl(fl(|x|^2 + |y|^2) - 2 x.y) with the per-pair band kappa (|x|^2 +
+// |y|^2) + eps0; the query's squared norm is lane-local, the candidates'
+// come from LDS.
+template <int NT, int WAVES, int NS, bool EXACT, int PADB, bool F32L2 = false>
 __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
     // NS k-steps per row; EXACT: the rows have exactly NS steps, otherwise
     // fewer, with the LDS rows zero-padded to NS steps and the query loads of
     // the missing steps clamped to the last step (their products are zero)
-    constexpr int SZ = 2;
+    constexpr int SZ = F32L2 ? 4 : 2;
     constexpr int EW = (NT + 1) / 2;
     constexpr int RSB = NS * 64 + PADB;  // row stride in LDS, padded against bank conflicts
     static_assert(NT <= 8, "candidate bits per lane exceed 32");
@@ -274,7 +303,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
     const int beg = a.tile_beg[blockIdx.x], end = a.tile_end[blockIdx.x];
     const int* Lm = a.lists + (size_t)l * M;
     const char* Xc = (const char*)a.X;
-    const int nsteps = EXACT ? NS : a.dk / 32;
+    const int nsteps = EXACT ? NS : a.dk / (64 / SZ);
     const size_t rowb = (size_t)a.ldx * SZ;
 
     const int nblk = (end - beg + 15) >> 4;
@@ -293,6 +322,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
     int* pts_lds = col_lds + M;
     // the largest rounding bound of the list's candidates (float bits)
     uint32_t* rc_lds = (uint32_t*)(pts_lds + M);
+    // F32L2: the candidates' squared norms, NT * 16 of them (0 past M), at a
+    // 16-byte boundary so that a lane reads its four as one vector
+    float* cn_lds = (float*)(smem + (((size_t)M * RSB + (size_t)M * 8 + 16 + 15) & ~(size_t)15));
     int pid = 0;
     if (tid < M) pid = Lm[tid];
     // (waves without a block load a valid row they never use, which keeps the
@@ -312,6 +344,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
     if (tid < M) {  // M <= 256 < WAVES * 64
         col_lds[tid] = 0;
         pts_lds[tid] = pid;
+    }
+    if constexpr (F32L2) {
+        if (tid < NT * 16) cn_lds[tid] = tid < M ? a.rnorm2[pid] : 0.0f;
     }
     if (tid == 0) *rc_lds = 0u;
 
@@ -345,12 +380,15 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             }
         }
     }
-    if (tid < M) atomicMax(rc_lds, __float_as_uint(a.rowerr[pts_lds[tid]]));  // bounds are >= 0
+    if constexpr (!F32L2) {
+        if (tid < M) atomicMax(rc_lds, __float_as_uint(a.rowerr[pts_lds[tid]]));  // bounds are >= 0
+    }
     __syncthreads();
     // thresholds with the list's largest candidate bound; the query's own
     // bound is added per block
     const float rcm = __uint_as_float(*rc_lds);
     const float th_hit0 = a.th_hit + rcm, th_near0 = a.th_near - rcm;
+
 
     uint32_t colacc[NT];
 #pragma unroll
@@ -366,7 +404,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
 #pragma unroll
             for (int x = 0; x < EW; ++x)
                 Fw[x] = (valid && x < a.W) ? __builtin_nontemporal_load(&a.F[(size_t)ent * a.W + x]) : 0u;
-            const float eq = a.rowerr[ent_point(ent, a.magicS)];
+            // BF16: the query row's rounding bound; F32L2: its squared norm
+            const float eq = F32L2 ? a.rnorm2[ent_point(ent, a.magicS)] : a.rowerr[ent_point(ent, a.magicS)];
             uint32_t ent2;
             bool valid2;
             entry_of(min(b + 2 * WAVES, nblk - 1), ent2, valid2);
@@ -394,14 +433,25 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
 #pragma unroll
                     for (int t = 0; t < NT; ++t) cn[t] = *(const uint4*)(smem + crow[t] + (st + 1) * 64);
                 }
-                const bf16x8_t qv = __builtin_bit_cast(bf16x8_t, af[st]);
+                if constexpr (F32L2) {
+                    const uint4 qv = af[st];
 #pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, cf[t]), qv,
-                                                                     acc[t], 0, 0, 0);
+                    for (int t = 0; t < NT; ++t) {
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(cf[t].x), __uint_as_float(qv.x), acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(cf[t].y), __uint_as_float(qv.y), acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(cf[t].z), __uint_as_float(qv.z), acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(cf[t].w), __uint_as_float(qv.w), acc[t], 0, 0, 0);
+                    }
+                } else {
+                    const bf16x8_t qv = __builtin_bit_cast(bf16x8_t, af[st]);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, cf[t]), qv,
+                                                                         acc[t], 0, 0, 0);
+                }
                 af[st] = *(const uint4*)(np + step_off(st));
                 if (st + 1 < NS) __builtin_amdgcn_sched_group_barrier(0x100, NT, 0);
-                __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, F32L2 ? 4 * NT : NT, 0);
                 __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
                 __builtin_amdgcn_sched_barrier(0);
                 if (st + 1 < NS) {
@@ -421,10 +471,23 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 uint32_t hr = 0u, nr = 0u;
+                // F32L2: the squared norms of candidates t*16 + 4g + i
+                float4 cnt4 = float4{0.f, 0.f, 0.f, 0.f};
+                if constexpr (F32L2) cnt4 = *(const float4*)(cn_lds + t * 16 + 4 * g);
+                const float cnv[4] = {cnt4.x, cnt4.y, cnt4.z, cnt4.w};
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    hr |= (acc[t][i] > th_h ? 1u : 0u) << i;
-                    nr |= (acc[t][i] >= th_n ? 1u : 0u) << i;
+                    if constexpr (F32L2) {
+                        // D = fl(fl(|q|^2 + |c|^2) - 2 s) against eps^2 -+ band
+                        const float D2 = (eq + cnv[i]) - 2.0f * acc[t][i];
+                        const double band = a.kappa * ((double)eq + (double)cnv[i]) + a.eps0;
+                        const double dv = (double)D2;
+                        hr |= (dv < a.eps2 - band ? 1u : 0u) << i;
+                        nr |= (!(dv > a.eps2 + band) ? 1u : 0u) << i;
+                    } else {
+                        hr |= (acc[t][i] > th_h ? 1u : 0u) << i;
+                        nr |= (acc[t][i] >= th_n ? 1u : 0u) << i;
+                    }
                 }
                 const int base = (t & 1) * 16 + 4 * g;
                 const uint32_t f = (Fw[t >> 1] >> base) & 0xFu;
@@ -483,17 +546,30 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
     }
 }
 
-template <int NT, int NS, bool EXACT, int PADB>
+template <int NT, int NS, bool EXACT, int PADB, bool F32L2 = false>
 static int launch_pipe_p(const DistArgs& a, int ntiles, hipStream_t st) {
     constexpr int WAVES = 8;
-    const size_t sm = (size_t)a.M * (NS * 64 + PADB) + (size_t)a.M * 8 + 16;
+    const size_t sm = (((size_t)a.M * (NS * 64 + PADB) + (size_t)a.M * 8 + 16 + 15) & ~(size_t)15) +
+                      (F32L2 ? (size_t)NT * 16 * 4 : 0);
     if (sm > (size_t)kLdsBytes) return -2;
-    if (!a.rowerr) return -3;
-    hipError_t e = hipFuncSetAttribute((const void*)k_dist_pipe<NT, WAVES, NS, EXACT, PADB>,
+    if (F32L2 ? !a.rnorm2 : !a.rowerr) return -3;
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_pipe<NT, WAVES, NS, EXACT, PADB, F32L2>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL((k_dist_pipe<NT, WAVES, NS, EXACT, PADB>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
+    hipLaunchKernelGGL((k_dist_pipe<NT, WAVES, NS, EXACT, PADB, F32L2>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
     return (int)hipGetLastError();
+}
+
+// L2 on FP32 rows in one LDS tile (rows of up to 256 dims at M <= 128):
+// NS 64-byte k-steps of 16 floats
+template <int NT>
+static int launch_pipe_l2(const DistArgs& a, int ntiles, hipStream_t st) {
+    const int nsteps = a.kc / 16;
+    if (a.nchunk != 1) return -4;
+    if (nsteps == 4) return launch_pipe_p<NT, 4, true, 16, true>(a, ntiles, st);
+    if (nsteps <= 8) return launch_pipe_p<NT, 8, false, 16, true>(a, ntiles, st);
+    if (nsteps <= 16) return launch_pipe_p<NT, 16, false, 16, true>(a, ntiles, st);
+    return -4;  // (25 steps spill registers: the k-chunked tiles take those rows)
 }
 
 template <int NT, int NS, bool EXACT>
@@ -503,15 +579,16 @@ static int launch_pipe(const DistArgs& a, int ntiles, hipStream_t st) {
 
 // (staging one 16-byte row piece per thread and step measured faster for these
 // tiles than deeper batches: 271 vs 310 ms for the FP32 8M x 784 tiles)
-template <bool BF, int NT, int QT, int PF, int BATCH = 1>
+template <bool BF, int NT, int QT, int PF, int BATCH = 1, bool L2X = false>
 static int launch_mfma_v(const DistArgs& a, int ntiles, hipStream_t st) {
     constexpr int WAVES = 8;
-    const size_t sm = (size_t)a.M * ((size_t)a.kc * (BF ? 2 : 4) + 16) + (size_t)a.M * 8;
+    const size_t sm = (size_t)a.M * ((size_t)a.kc * (BF ? 2 : 4) + 16) + (size_t)a.M * (L2X ? 12 : 8);
     if (sm > (size_t)kLdsBytes) return -2;
-    hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES, PF, BATCH>,
+    if (L2X && !a.rnorm2) return -3;
+    hipError_t e = hipFuncSetAttribute((const void*)k_dist_mfma<BF, QT, NT, WAVES, PF, BATCH, L2X>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL((k_dist_mfma<BF, QT, NT, WAVES, PF, BATCH>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
+    hipLaunchKernelGGL((k_dist_mfma<BF, QT, NT, WAVES, PF, BATCH, L2X>), dim3(ntiles), dim3(WAVES * 64), sm, st, a);
     return (int)hipGetLastError();
 }
 
@@ -526,6 +603,13 @@ static int launch_mfma_nt(const DistArgs& a, int ntiles, hipStream_t st) {
         if (a.nchunk == 1 && nsteps <= 25) return launch_pipe<NT, 25, false>(a, ntiles, st);
         return launch_mfma_v<BF, NT, 2, 5>(a, ntiles, st);
     } else {
+        if (a.rnorm2) {
+            // short rows: the pipelined tiles (the query lane owns its norm,
+            // masks and entry); longer rows: k-chunked tiles
+            const int r = std::getenv("SDB_L2_CHUNKED") ? -4 : launch_pipe_l2<NT>(a, ntiles, st);
+            if (r != -4) return r;
+            return launch_mfma_v<BF, NT, 2, 1, 1, true>(a, ntiles, st);
+        }
         return launch_mfma_v<BF, NT, 2, 1>(a, ntiles, st);
     }
 }
@@ -1145,6 +1229,11 @@ __global__ __launch_bounds__(256) void k_pair_sample(DistArgs a, const int* topK
             fs = 1.0f - dot32_wave(x, y, a.d, lane);
         } else if (a.metric == kChi2) {
             fs = 1.0f - 2.0f * chi2sum32_wave(x, y, a.d, lane);
+        } else if (a.metric == kL2 && a.rnorm2) {
+            // the L2 MFMA tiles' arithmetic, D = fl(fl(|x|^2 + |y|^2) - 2 x.y)
+            // with the tiles' squared norms (the dot's order emulated)
+            const float D2 = (a.rnorm2[p] + a.rnorm2[c]) - 2.0f * dot32_wave(x, y, a.d, lane);
+            fs = sqrtf(fmaxf(D2, 0.0f));
         } else if (a.metric == kL1 || a.metric == kL2) {
             float t = 0.0f;
             for (int k = lane; k < a.d; k += 64) {
@@ -1177,6 +1266,47 @@ int launch_pair_sample(const DistArgs& a, const int* topK, int K, int P, long lo
     return (int)hipGetLastError();
 }
 
+// Squared norms of the rows for the L2 MFMA tiles: 16 lanes per row, each
+// summing float4 pieces of the zero-padded row in FP64, a 16-lane xor tree,
+// one rounding to FP32 (the order of an FP64 sum of d squares moves it by
+// far less than that rounding); and their maximum (float bits, norms >= 0).
+// One workgroup per 16 rows: every row's load is in flight at once.
+__global__ __launch_bounds__(256) void k_row_norm2(const float* __restrict__ X, long long n, int ldx,
+                                                   float* __restrict__ out, uint32_t* __restrict__ maxbits) {
+    const int lane = lane_id(), sub = lane & 15;
+    const long long p = (long long)blockIdx.x * 16 + (threadIdx.x >> 4);
+    double s = 0.0;
+    if (p < n) {
+        const float4* r = (const float4*)(X + (size_t)p * ldx);
+        for (int k = sub; k < ldx / 4; k += 16) {
+            const float4 v = r[k];
+            s = fma((double)v.x, (double)v.x, s);
+            s = fma((double)v.y, (double)v.y, s);
+            s = fma((double)v.z, (double)v.z, s);
+            s = fma((double)v.w, (double)v.w, s);
+        }
+    }
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) s = s + __shfl_xor(s, m, kWave);
+    const float f = p < n ? (float)s : 0.0f;
+    if (sub == 0 && p < n) out[p] = f;
+    float mx = f;
+#pragma unroll
+    for (int m = 32; m >= 16; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m, kWave));
+    // one address for every wave: the atomic only when it raises the maximum
+    if (lane == 0 && mx > 0.0f && __hip_atomic_load(maxbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < __float_as_uint(mx))
+        atomicMax(maxbits, __float_as_uint(mx));
+}
+
+int launch_row_norm2(const float* X, long long n, int d, int ldx, float* out, uint32_t* maxbits, hipStream_t st) {
+    if (n <= 0) return 0;
+    if (ldx % 4 != 0 || d > ldx) return -1;
+    const long long blocks = (n + 15) / 16;
+    if (blocks > 0x7fffffffll) return -3;
+    hipLaunchKernelGGL(k_row_norm2, dim3((unsigned)blocks), dim3(256), 0, st, X, n, ldx, out, maxbits);
+    return (int)hipGetLastError();
+}
+
 // Sort key of every queued pair: its candidate point (lists[l][col]).
 __global__ void k_queue_keys(const unsigned long long* __restrict__ queue, int nq, const int* __restrict__ lists,
                              int M, uint32_t* __restrict__ keys) {
@@ -1197,8 +1327,11 @@ int launch_queue_keys(const unsigned long long* queue, int nq, const int* lists,
 int launch_recheck_k(const DistArgs& a, const int* topK, int K, int P, int precision,
                      double band32, int n_entries, unsigned long long* stats, hipStream_t st) {
     if (n_entries <= 0) return 0;
+    // 4 waves per block, grid-stride; 16,384 blocks (64 k waves, 64 per CU)
+    // keep every CU busy while the per-wave counter atomics, all on the same
+    // four words, stay few
     long long blocks = ((long long)n_entries + 3) / 4;
-    if (blocks > 65536) blocks = 65536;
+    if (blocks > 16384) blocks = 16384;
     hipLaunchKernelGGL(k_recheck, dim3((unsigned)blocks), dim3(256), 0, st, a, topK, K, P, precision, band32, n_entries, stats);
     return (int)hipGetLastError();
 }

@@ -79,6 +79,11 @@ struct DistArgs {
     // bound over the tile's list)
     const float* rowerr;
     float th_hit, th_near;
+    // L2 on the FP32 MFMA tiles: squared row norms (FP32) and the per-pair
+    // decision on D = |x|^2 + |y|^2 - 2 x.y: hit iff D < eps2 - band, queued
+    // iff not a hit and D <= eps2 + band, band = kappa (|x|^2 + |y|^2) + eps0
+    const float* rnorm2;
+    double eps2, kappa, eps0;
     unsigned long long magicS;  // ceil(2^40 / S): a = ent_point(ent, magicS)
     long long n_ent;            // entries (rows of E and F)
 };
@@ -141,6 +146,7 @@ int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, 
                       uint32_t* F, hipStream_t st);
 int launch_group_count(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
                        int Gx, int* wgcnt, hipStream_t st);
+int launch_row_norm2(const float* X, long long n, int d, int ldx, float* out, uint32_t* maxbits, hipStream_t st);
 int launch_queue_keys(const unsigned long long* queue, int nq, const int* lists, int M, uint32_t* keys,
                       hipStream_t st);
 int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,

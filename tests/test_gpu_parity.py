@@ -337,6 +337,30 @@ def test_chi2_bf16_prefilter_matches_oracle(ctx, n, d, K, M, eps, minPts):
     assert np.abs(xb - rne).max() <= 1
 
 
+@pytest.mark.parametrize("shift", [0.0, 100.0, 3000.0])
+def test_l2_mfma_tiles_and_valu_fallback_match_oracle(ctx, shift):
+    """L2 distances on the FP32 MFMA tiles through |x|^2 + |y|^2 - 2 x.y with
+    a per-pair band of kappa (|x|^2 + |y|^2) (shift 0; shift 100 widens the
+    band and sends many pairs to the FP64 recheck), and on the VALU tiles
+    when rows far from the origin would widen it past 5% of eps^2 (shift
+    3000).  Degrees, cores and labels equal the oracle's FP64 decision in
+    every case; the reported band tells the two tile kinds apart."""
+    X, _ = datagen.blobs(6000, 51, k=12, noise=0.05, seed=1001)
+    sigma = datagen.median_pairwise_l2(X, 4000, seed=1001)
+    eps = float(np.float32(datagen.l2_eps_for_embedded_cosine(sigma, 0.3)))
+    X = (X + np.float32(shift)).astype(np.float32)
+    print(f"L2 tiles shift={shift} n=6000 d=51 eps={eps:.3f}", flush=True)
+    p, bits, R, tk, tm, Xt, st = full_check(ctx, X, S.L2, eps, 20, K=10, M=100, ker=1024, sigma=sigma)
+    band_valu = 2.0 * eps * (51 + 8) * 2.0 ** -23
+    print("band", st["band"], "valu band", band_valu, "queue", st["n_queue"], flush=True)
+    # the MFMA tiles report their widest per-pair band, (2 kappa max|x|^2 +
+    # eps0) / (2 eps); the VALU tiles report 2 eps (d + 8) u
+    if shift < 1000:
+        assert abs(st["band"] - band_valu) > 1e-6 * band_valu
+    else:
+        assert abs(st["band"] - band_valu) <= 1e-12 * band_valu
+
+
 def test_sorted_recheck_queue_matches_oracle(ctx):
     """A populated error band (mixture noise 0.2, eps 0.27 inside the
     within-cluster distances): more than 2^14 pairs go through the recheck,
