@@ -173,7 +173,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
                      &c->inv, &c->pos_hub, &c->pcnt, &c->poff, &c->pairs, &c->pseg, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
                      &c->scan_tmp, &c->total, &c->err_flag, &c->row_slot, &c->row_buf, &c->E,
-                     &c->colsum, &c->queue, &c->qcount, &c->qkeys, &c->qvals, &c->qsort_tmp, &c->rnorm2, &c->rnorm2_max, &c->tiles, &c->degree, &c->core,
+                     &c->colsum, &c->queue, &c->qcount, &c->qkeys, &c->qvals, &c->qsort_tmp, &c->inter, &c->rnorm2, &c->rnorm2_max, &c->tiles, &c->degree, &c->core,
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
                      &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
                      &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
@@ -555,8 +555,20 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
                           P_<uint32_t>(c->pairs), st));
     ENSURE(c->F, (size_t)n * S * W * 4);
     ENSURE(c->pseg, (size_t)L2 * 8);
+    // lists that share many points (long pair segments, e.g. the extreme
+    // points of blobs in many lists): the dense intersections replace one
+    // binary search per earlier slot with one load (at most 256 MB)
+    const size_t inter_bytes = (size_t)L2 * L2 * W * 4;
+    const bool use_inter = (long long)npairs >= 16ll * L2 && inter_bytes <= ((size_t)256 << 20);
+    if (use_inter) {
+        ENSURE(c->inter, inter_bytes);
+        HIPCHK(hipMemsetAsync(c->inter.p, 0, inter_bytes, st));
+        LCHK(launch_inter_fill(P_<int>(c->pos_hub), P_<int>(c->hub_ptr), P_<int>(c->hub_mem), L2, M, W,
+                               P_<uint32_t>(c->inter), st));
+    }
     LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, W, P_<int>(c->poff), P_<uint32_t>(c->pairs), P_<int2>(c->pseg),
-                           P_<int>(c->hub_id), P_<int>(c->hub_ptr), P_<int>(c->hub_mem), P_<uint32_t>(c->F), st));
+                           P_<int>(c->hub_id), P_<int>(c->hub_ptr), P_<int>(c->hub_mem),
+                           use_inter ? P_<uint32_t>(c->inter) : nullptr, P_<uint32_t>(c->F), st));
     // group the (point, slot) entries by (query block, list): the distance
     // tiles then sweep the points block by block, and a 192 MB block of query
     // rows stays in the 256 MB last-level cache, so most of the 2K visits of a

@@ -96,6 +96,8 @@ struct sdb_ctx {
     // the queued pairs sorted by candidate point for the recheck: two key
     // buffers, the alternate value buffer and the sort's scratch
     DevBuf qkeys, qvals, qsort_tmp;
+    // dense list intersections (inter[l][L], W words) when lists share many points
+    DevBuf inter;
     // squared row norms for the L2 MFMA tiles and their maximum; whether the
     // last distance pass used those tiles (the accuracy sample follows it)
     DevBuf rnorm2, rnorm2_max;

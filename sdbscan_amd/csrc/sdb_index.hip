@@ -1136,6 +1136,23 @@ __global__ void k_pair_fill(const int* __restrict__ pos_hub, const int* __restri
     }
 }
 
+// Dense list intersections, for inputs whose lists share many points: word w
+// of inter[l][L] has bit i set when the point at position 32w + i of list l
+// is also a member of list L != l (the same facts as the pair table, indexed
+// directly).  One thread per list position.
+__global__ void k_inter_fill(const int* __restrict__ pos_hub, const int* __restrict__ hub_ptr,
+                             const int* __restrict__ hub_mem, int LM, int M, int L2, int W,
+                             uint32_t* __restrict__ inter) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= LM) return;
+    const int l = t / M, i = t - l * M;
+    const int h = pos_hub[t];
+    for (int q = hub_ptr[h]; q < hub_ptr[h + 1]; ++q) {
+        const int L = hub_mem[q] / M;
+        if (L != l) atomicOr(&inter[((size_t)l * L2 + L) * W + (i >> 5)], 1u << (i & 31));
+    }
+}
+
 // First-occurrence mask per (point, slot): bit i of F[a][s] is set when the
 // point at position i of the slot's list is not a itself and does not occur
 // in an earlier slot of a (the reference's per-point dedup bitset,
@@ -1186,6 +1203,7 @@ __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK
                                                     const int* __restrict__ hub_id,
                                                     const int* __restrict__ hub_ptr,
                                                     const int* __restrict__ hub_mem,
+                                                    const uint32_t* __restrict__ inter,
                                                     uint32_t* __restrict__ F) {
     extern __shared__ int lids[];
     const int S = 2 * K;
@@ -1209,7 +1227,22 @@ __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK
         // when L is one of the earlier slots' lists
         const int2 sg = seg[l];
         const int q0 = sg.x, q1 = sg.y & 0x7fffffff;
-        if (q1 - q0 > 32 && sg.y < 0) {
+        if (q1 - q0 > 32 && inter) {
+            // long segment with the dense intersections: one masked load per
+            // earlier slot's list, independent of the others
+            const int L2 = 2 * P;
+            for (int s2 = 0; s2 < s; ++s2) {
+                const uint32_t* iw = inter + ((size_t)l * L2 + mine[s2]) * W;
+                if (W == 4) {
+                    const uint4 v = *(const uint4*)iw;
+                    m[0] &= ~v.x; m[1] &= ~v.y; m[2] &= ~v.z; m[3] &= ~v.w;
+                } else {
+#pragma unroll
+                    for (int x = 0; x < kMaxW; ++x)
+                        if (x < W) m[x] &= ~iw[x];
+                }
+            }
+        } else if (q1 - q0 > 32 && sg.y < 0) {
             // long segment, sorted by (L, position): one lower bound per
             // earlier slot's list
             for (int s2 = 0; s2 < s; ++s2) {
@@ -1491,15 +1524,23 @@ int launch_pair_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem,
     return (int)hipGetLastError();
 }
 
+int launch_inter_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem, int L2, int M, int W,
+                      uint32_t* inter, hipStream_t st) {
+    const int LM = L2 * M;
+    hipLaunchKernelGGL(k_inter_fill, dim3((LM + 255) / 256), dim3(256), 0, st, pos_hub, hub_ptr, hub_mem, LM, M, L2, W,
+                       inter);
+    return (int)hipGetLastError();
+}
+
 int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const int* poff,
                       uint32_t* pairs, int2* seg, const int* hub_id, const int* hub_ptr, const int* hub_mem,
-                      uint32_t* F, hipStream_t st) {
+                      const uint32_t* inter, uint32_t* F, hipStream_t st) {
     const int S = 2 * K;
     hipLaunchKernelGGL(k_pair_sort, dim3(2 * P), dim3(256), 0, st, poff, M, pairs, seg);
     const int ppb = std::max(1, 512 / S);
     const long long blocks = (n + ppb - 1) / ppb;
     hipLaunchKernelGGL(k_first_mask, dim3((unsigned)blocks), dim3(ppb * S), (size_t)ppb * S * 4, st, topK, n, P, K, M,
-                       W, ppb, poff, pairs, seg, hub_id, hub_ptr, hub_mem, F);
+                       W, ppb, poff, pairs, seg, hub_id, hub_ptr, hub_mem, inter, F);
     return (int)hipGetLastError();
 }
 

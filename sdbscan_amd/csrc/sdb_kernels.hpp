@@ -141,9 +141,11 @@ int launch_pair_count(const int* lists, const int* hub_id, const int* hub_ptr, i
                       int* pcnt, int* poff, int* total, void* tmp, size_t tmp_bytes, hipStream_t st);
 int launch_pair_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem, int L2, int M, const int* poff,
                      uint32_t* pairs, hipStream_t st);
+int launch_inter_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem, int L2, int M, int W,
+                      uint32_t* inter, hipStream_t st);
 int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const int* poff,
                       uint32_t* pairs, int2* seg, const int* hub_id, const int* hub_ptr, const int* hub_mem,
-                      uint32_t* F, hipStream_t st);
+                      const uint32_t* inter, uint32_t* F, hipStream_t st);
 int launch_group_count(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
                        int Gx, int* wgcnt, hipStream_t st);
 int launch_row_norm2(const float* X, long long n, int d, int ldx, float* out, uint32_t* maxbits, hipStream_t st);
