@@ -173,7 +173,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
                      &c->inv, &c->pos_hub, &c->pcnt, &c->poff, &c->pairs, &c->pseg, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
                      &c->scan_tmp, &c->total, &c->err_flag, &c->row_slot, &c->row_buf, &c->E,
-                     &c->colsum, &c->queue, &c->qcount, &c->qkeys, &c->qvals, &c->qsort_tmp, &c->inter, &c->rnorm2, &c->rnorm2_max, &c->tiles, &c->degree, &c->core,
+                     &c->colsum, &c->queue, &c->qcount, &c->qkeys, &c->qvals, &c->qsort_tmp, &c->inter, &c->core_spread, &c->rnorm2, &c->rnorm2_max, &c->tiles, &c->degree, &c->core,
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
                      &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
                      &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
@@ -966,8 +966,11 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
         // same eps: the hit masks and degrees stand; only the core test moves
         HIPCHK(hipEventRecord(c->ev[3], st));
         HIPCHK(hipMemsetAsync((unsigned long long*)c->stats64.p + 3, 0, 8, st));
+        ENSURE(c->core_spread, (size_t)kSpread * kSpreadStride * 8);
+        HIPCHK(hipMemsetAsync(c->core_spread.p, 0, c->core_spread.bytes, st));
         LCHK(launch_core_minpts(P_<int>(c->degree), n, minPts, P_<uint8_t>(c->core),
-                                (unsigned long long*)c->stats64.p + 3, st));
+                                P_<unsigned long long>(c->core_spread), kSpread - 1, st));
+        LCHK(launch_spread_fold(P_<unsigned long long>(c->core_spread), (unsigned long long*)c->stats64.p + 3, st));
         HIPCHK(hipEventRecord(c->ev[6], st));
         unsigned long long nc = 0;
         HIPCHK(hipMemcpyAsync(&nc, (unsigned long long*)c->stats64.p + 3, 8, hipMemcpyDeviceToHost, st));
@@ -995,8 +998,12 @@ int sdb_find_core(sdb_ctx* c, float eps, int minPts, uint8_t* core_out, int* deg
     ca.colsum = P_<int>(c->colsum); ca.hub_id = P_<int>(c->hub_id); ca.hub_ptr = P_<int>(c->hub_ptr);
     ca.hub_mem = P_<int>(c->hub_mem); ca.inv = P_<uint32_t>(c->inv); ca.minPts = minPts;
     ca.degree = P_<int>(c->degree); ca.core = P_<uint8_t>(c->core);
-    ca.n_core = (unsigned long long*)c->stats64.p + 3;
+    ENSURE(c->core_spread, (size_t)kSpread * kSpreadStride * 8);
+    HIPCHK(hipMemsetAsync(c->core_spread.p, 0, c->core_spread.bytes, st));
+    ca.n_core = P_<unsigned long long>(c->core_spread);
+    ca.spread = kSpread - 1;
     LCHK(launch_degree(ca, st));
+    LCHK(launch_spread_fold(P_<unsigned long long>(c->core_spread), (unsigned long long*)c->stats64.p + 3, st));
     HIPCHK(hipEventRecord(c->ev[6], st));
     unsigned long long hs[8] = {0};
     HIPCHK(hipMemcpyAsync(hs, c->stats64.p, 64, hipMemcpyDeviceToHost, st));
@@ -1606,7 +1613,9 @@ int sdb_sng_find_core(sdb_ctx* c, const int* samples, int n_samples, const int* 
     if (r) return r;
     ENSURE(c->degree, (size_t)n * 4);
     ENSURE(c->core, (size_t)n);
-    unsigned long long* ncore = (unsigned long long*)c->stats64.p + 3;
+    ENSURE(c->core_spread, (size_t)kSpread * kSpreadStride * 8);
+    HIPCHK(hipMemsetAsync(c->core_spread.p, 0, c->core_spread.bytes, st));
+    unsigned long long* ncore = P_<unsigned long long>(c->core_spread);
     if (!dedup) {
         // sngDbscan: |N(a)| with duplicates (src/sDbscan.cpp:487-501)
         ENSURE(c->rev, (size_t)n * 4);
@@ -1621,6 +1630,7 @@ int sdb_sng_find_core(sdb_ctx* c, const int* samples, int n_samples, const int* 
         LCHK(launch_rowsize_core(n, P_<long long>(c->nb_ptr), minPts, P_<int>(c->degree), P_<uint8_t>(c->core), ncore,
                                  st));
     }
+    LCHK(launch_spread_fold(ncore, (unsigned long long*)c->stats64.p + 3, st));
     HIPCHK(hipEventRecord(c->ev[6], st));
     unsigned long long hs[8] = {0};
     HIPCHK(hipMemcpyAsync(hs, c->stats64.p, 64, hipMemcpyDeviceToHost, st));

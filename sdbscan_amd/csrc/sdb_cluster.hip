@@ -51,7 +51,26 @@ __device__ __forceinline__ void degree_block(const ClusterArgs& c, long long blk
     own[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t* Eb = c.E + (size_t)a0 * SW;
-    if ((SW & 3) == 0) {
+    if ((SW & 15) == 0) {
+        // four lanes per point, each summing a quarter of the point's E row
+        // (consecutive lanes read consecutive 16-byte vectors), then a
+        // reduction over the four lanes: no conflicting LDS atomics
+        const int per = SW >> 4;
+        const int part = threadIdx.x & 3;
+        for (int pt = threadIdx.x >> 2; pt < 256; pt += 64) {
+            int pc = 0;
+            if (pt < npt) {
+                const uint4* v = (const uint4*)(Eb + (size_t)pt * SW) + part * per;
+                for (int k = 0; k < per; ++k) {
+                    const uint4 e = v[k];
+                    pc += __popc(e.x) + __popc(e.y) + __popc(e.z) + __popc(e.w);
+                }
+            }
+            pc += __shfl_xor(pc, 1, kWave);
+            pc += __shfl_xor(pc, 2, kWave);
+            if (part == 0 && pt < npt) own[pt] = pc;
+        }
+    } else if ((SW & 3) == 0) {
         const int per = SW >> 2;
         const int nv = npt * per;
         for (int v = threadIdx.x; v < nv; v += 256) {
@@ -66,33 +85,39 @@ __device__ __forceinline__ void degree_block(const ClusterArgs& c, long long blk
             if (pc) atomicAdd(&own[v / SW], pc);
         }
     }
-    __syncthreads();
     const long long a = a0 + threadIdx.x;
     const bool live = threadIdx.x < npt;
-    int deg = 0;
-    if (live) {
-        deg = own[threadIdx.x];
-        const int h = c.hub_id[a];
-        if (h >= 0) {
-            const uint32_t* Ea = c.E + (size_t)a * SW;
-            int rev = 0, mutual = 0;
-            const int hb = c.hub_ptr[h], he = c.hub_ptr[h + 1];
-            for (int q = hb; q < he; ++q) rev += c.colsum[c.hub_mem[q]];
-            for (int s = 0; s < c.S; ++s) {
-                const int l = slot_list_c(c.topK, a, s, c.K, c.P);
-                for (int w = 0; w < c.W; ++w) {
-                    const uint32_t e = Ea[s * c.W + w];
-                    if (!e) continue;
-                    uint32_t m = 0;
-                    for (int q = hb; q < he; ++q) {
-                        const int L = c.hub_mem[q] / c.M;
-                        m |= c.inv[((size_t)l * c.L2 + L) * c.W + w];
-                    }
-                    mutual += __popc(e & m);
-                }
+    const int h = live ? c.hub_id[a] : -1;
+    __syncthreads();
+    int deg = live ? own[threadIdx.x] : 0;
+    // Points in a list (a few percent): the whole wave takes one such point at
+    // a time, one lane per (slot, word) of its E row and per membership, so
+    // that its loads are issued together rather than one after another.
+    unsigned long long hubs = __ballot(h >= 0);
+    const int lane = lane_id();
+    while (hubs) {
+        const int src = __builtin_ctzll(hubs);
+        hubs &= hubs - 1;
+        const long long ah = a - lane + src;
+        const int hh = __builtin_amdgcn_readlane(h, src);
+        const int hb = c.hub_ptr[hh], he = c.hub_ptr[hh + 1];
+        int part = 0;
+        for (int q = hb + lane; q < he; q += kWave) part += c.colsum[c.hub_mem[q]];  // |R(a)|
+        const uint32_t* Ea = c.E + (size_t)ah * SW;
+        for (int i = lane; i < SW; i += kWave) {
+            const uint32_t e = Ea[i];
+            if (!e) continue;
+            const int s = i / c.W, w = i - s * c.W;
+            const int l = slot_list_c(c.topK, ah, s, c.K, c.P);
+            uint32_t m = 0;
+            for (int q = hb; q < he; ++q) {
+                const int L = c.hub_mem[q] / c.M;
+                m |= c.inv[((size_t)l * c.L2 + L) * c.W + w];
             }
-            deg += rev - mutual;
+            part -= __popc(e & m);  // |O(a) n R(a)|
         }
+        part = wave_sum_i32(part);
+        if (lane == src) deg += part;
     }
     const bool is_core = live && deg >= c.minPts - 1;
     if (live) {
@@ -100,7 +125,7 @@ __device__ __forceinline__ void degree_block(const ClusterArgs& c, long long blk
         c.core[a] = is_core ? 1 : 0;
     }
     const unsigned long long b = __ballot(is_core);
-    if (lane_id() == 0 && b) atomicAdd(c.n_core, (unsigned long long)__popcll(b));
+    if (lane_id() == 0 && b) spread_add(c.n_core, c.spread, (unsigned long long)__popcll(b));
 }
 __global__ __launch_bounds__(256) void k_degree(ClusterArgs c) { degree_block(c, blockIdx.x); }
 
@@ -312,24 +337,37 @@ __global__ __launch_bounds__(256) void k_labels(ClusterArgs c, const int* rank) 
 // Core test of new minPts on degrees already counted for the same eps.
 __device__ __forceinline__ void core_minpts_block(const int* __restrict__ degree, long long n, int minPts,
                                                   uint8_t* __restrict__ core, unsigned long long* __restrict__ n_core,
-                                                  long long blk) {
+                                                  int spread, long long blk) {
     const long long a = blk * 256 + threadIdx.x;
     const bool is_core = a < n && degree[a] >= minPts - 1;
     if (a < n) core[a] = is_core ? 1 : 0;
     const unsigned long long b = __ballot(is_core);
-    if (lane_id() == 0 && b) atomicAdd(n_core, (unsigned long long)__popcll(b));
+    if (lane_id() == 0 && b) spread_add(n_core, spread, (unsigned long long)__popcll(b));
 }
 __global__ __launch_bounds__(256) void k_core_minpts(const int* __restrict__ degree, long long n, int minPts,
                                                      uint8_t* __restrict__ core,
-                                                     unsigned long long* __restrict__ n_core) {
-    core_minpts_block(degree, n, minPts, core, n_core, blockIdx.x);
+                                                     unsigned long long* __restrict__ n_core, int spread) {
+    core_minpts_block(degree, n, minPts, core, n_core, spread, blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_spread_fold(const unsigned long long* __restrict__ ctr,
+                                                    unsigned long long* __restrict__ out) {
+    unsigned long long v = ctr[(size_t)threadIdx.x * kSpreadStride];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    if (threadIdx.x == 0) *out += v;
 }
 
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 int launch_core_minpts(const int* degree, long long n, int minPts, uint8_t* core, unsigned long long* n_core,
-                       hipStream_t st) {
-    hipLaunchKernelGGL(k_core_minpts, dim3(nblk(n, 256)), dim3(256), 0, st, degree, n, minPts, core, n_core);
+                       int spread, hipStream_t st) {
+    hipLaunchKernelGGL(k_core_minpts, dim3(nblk(n, 256)), dim3(256), 0, st, degree, n, minPts, core, n_core, spread);
+    return (int)hipGetLastError();
+}
+int launch_spread_fold(const unsigned long long* ctr, unsigned long long* out, hipStream_t st) {
+    static_assert(kSpread == kWave, "one lane per copy");
+    hipLaunchKernelGGL(k_spread_fold, dim3(1), dim3(kSpread), 0, st, ctr, out);
     return (int)hipGetLastError();
 }
 
@@ -477,7 +515,7 @@ __global__ __launch_bounds__(BS) void k_sets(const ChainSet* __restrict__ sets, 
     if constexpr (OP == kChDegree) {
         degree_eps_block(x, blk);
     } else if constexpr (OP == kChCore) {
-        core_minpts_block(c.degree, c.n, c.minPts, c.core, c.n_core, blk);
+        core_minpts_block(c.degree, c.n, c.minPts, c.core, c.n_core, c.spread, blk);
     } else if constexpr (OP == kChAdjClear) {
         const long long i = blk * BS + threadIdx.x;
         if (i < (long long)c.L2 * c.M * c.W) c.adj[i] = 0u;

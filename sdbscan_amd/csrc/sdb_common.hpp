@@ -48,6 +48,17 @@ __device__ __forceinline__ double wave_sum_f64_tree(double v) {
     return v;
 }
 
+// A counter that every wave of a large grid adds to, kept as kSpread copies
+// 128 bytes apart: the workgroup adds to copy (blockIdx.x & spread), so with
+// spread = kSpread - 1 no single address serialises every wave's atomic (one
+// address took about 1.5 ms for the 125k waves of an 8M-point pass); spread =
+// 0 uses copy 0 only, i.e. the counter itself.  k_spread_fold adds the copies
+// into the final counter.
+constexpr int kSpread = 64, kSpreadStride = 16;  // stride in 8-byte words
+__device__ __forceinline__ void spread_add(unsigned long long* ctr, int spread, unsigned long long v) {
+    atomicAdd(ctr + (size_t)(blockIdx.x & (unsigned)spread) * kSpreadStride, v);
+}
+
 __device__ __forceinline__ int wave_sum_i32(int v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);

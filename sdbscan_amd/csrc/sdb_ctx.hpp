@@ -98,6 +98,7 @@ struct sdb_ctx {
     DevBuf qkeys, qvals, qsort_tmp;
     // dense list intersections (inter[l][L], W words) when lists share many points
     DevBuf inter;
+    DevBuf core_spread;  // kSpread copies of the core count (spread_add)
     // squared row norms for the L2 MFMA tiles and their maximum; whether the
     // last distance pass used those tiles (the accuracy sample follows it)
     DevBuf rnorm2, rnorm2_max;

@@ -75,7 +75,8 @@ __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, 
             rmax = fmaxf(rmax, rho);
         }
     }
-    if (Xb && lane == 0) atomicMax(rowerr_max, __float_as_uint(rmax));
+    if (Xb && lane == 0 && (int)__float_as_uint(rmax) > load_relaxed((const int*)rowerr_max))
+        atomicMax(rowerr_max, __float_as_uint(rmax));
 }
 
 // One Walsh-Hadamard level across lanes (logical stride S < 64, mirror lane
@@ -942,7 +943,10 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
         }
     }
     if constexpr (RAW && MODE == kFull) {
-        if (a.Xb_out && lane == 0) atomicMax(a.rowerr_max, __float_as_uint(rho_max));
+        // every wave ends here: the atomic only when it would raise the
+        // maximum, so the waves do not queue on one address
+        if (a.Xb_out && lane == 0 && (int)__float_as_uint(rho_max) > load_relaxed((const int*)a.rowerr_max))
+            atomicMax(a.rowerr_max, __float_as_uint(rho_max));
     }
     if constexpr (MODE == kSample) {
 #pragma unroll
@@ -1159,7 +1163,9 @@ __global__ void k_inter_fill(const int* __restrict__ pos_hub, const int* __restr
 // src/sDbscan.cpp:260-282).  Only points held by several lists can repeat;
 // the (position, other list) pairs of the slot's list are checked against the
 // lists of the earlier slots.  A workgroup takes ppb consecutive points with one thread per (point, slot)
-// entry: the slot lists go to LDS for the duplicate checks, and the W mask
+// entry: the slot lists and a bitmap of each point's lists go to LDS for the
+// duplicate checks (a pair whose other list is not one of the point's costs
+// one bit test, whatever the slot), and the W mask
 // words of consecutive entries are written by consecutive threads (one
 // 16-byte store each when W = 4).
 // Each list's pair segment sorted by (other list, position) in LDS, one
@@ -1196,26 +1202,14 @@ __global__ __launch_bounds__(256) void k_pair_sort(const int* __restrict__ poff,
     if (threadIdx.x == 0) seg[l] = make_int2(q0, (int)((uint32_t)q1 | 0x80000000u));
 }
 
-__global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK, long long n, int P, int K,
-                                                    int M, int W, int ppb, const int* __restrict__ poff,
-                                                    const uint32_t* __restrict__ pairs,
-                                                    const int2* __restrict__ seg,
-                                                    const int* __restrict__ hub_id,
-                                                    const int* __restrict__ hub_ptr,
-                                                    const int* __restrict__ hub_mem,
-                                                    const uint32_t* __restrict__ inter,
-                                                    uint32_t* __restrict__ F) {
-    extern __shared__ int lids[];
-    const int S = 2 * K;
-    const int tid = threadIdx.x;
-    const int pl = tid / S, s = tid - pl * S;
-    const long long a = (long long)blockIdx.x * ppb + pl;
-    const bool live = pl < ppb && a < n;
-    if (live) lids[tid] = slot_list(topK, a, s, K, P);
-    __syncthreads();
-    if (!live) return;
-    const int l = lids[tid];
-    const int* mine = lids + pl * S;
+// The mask words of entry (a, s) with list l: mine = the lists of a's
+// slots, bm = the bitmap of those lists, hs = hub_id[a].
+__device__ __forceinline__ void first_mask_entry(int l, const int* mine, const uint32_t* bm, int s, int S, int P,
+                                                 int M, int W, const int2* __restrict__ seg,
+                                                 const uint32_t* __restrict__ pairs,
+                                                 const uint32_t* __restrict__ inter, long long a, int hs,
+                                                 const int* __restrict__ hub_ptr, const int* __restrict__ hub_mem,
+                                                 uint32_t* __restrict__ F) {
     uint32_t m[kMaxW];
 #pragma unroll
     for (int w = 0; w < kMaxW; ++w) {
@@ -1263,9 +1257,13 @@ __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK
                 }
             }
         } else {
+            // short segment: a pair's other list is looked up in the point's
+            // bitmap first; only lists of the point are searched among the
+            // earlier slots
             for (int q = q0; q < q1; ++q) {
                 const uint32_t pr = pairs[q];
                 const int L = (int)(pr >> 8), i = (int)(pr & 255u);
+                if (!((bm[L >> 5] >> (L & 31)) & 1u)) continue;
                 bool dup = false;
                 for (int s2 = 0; s2 < s; ++s2) dup |= mine[s2] == L;
                 if (dup) {
@@ -1276,7 +1274,6 @@ __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK
             }
         }
     }
-    const int hs = hub_id[a];
     if (hs >= 0) {
         for (int q = hub_ptr[hs]; q < hub_ptr[hs + 1]; ++q) {
             const int li = hub_mem[q];
@@ -1296,6 +1293,34 @@ __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK
         for (int w = 0; w < kMaxW; ++w)
             if (w < W) Fo[w] = m[w];
     }
+}
+
+__global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK, long long n, int P, int K,
+                                                    int M, int W, int ppb, const uint32_t* __restrict__ pairs,
+                                                    const int2* __restrict__ seg,
+                                                    const int* __restrict__ hub_id,
+                                                    const int* __restrict__ hub_ptr,
+                                                    const int* __restrict__ hub_mem,
+                                                    const uint32_t* __restrict__ inter,
+                                                    uint32_t* __restrict__ F) {
+    // LDS: the slot lists of the workgroup's points, then per point a bitmap
+    // over the 2P list ids with the bits of its own slot lists set
+    extern __shared__ int lids[];
+    const int S = 2 * K, BW = (2 * P + 31) >> 5;
+    uint32_t* bm = (uint32_t*)(lids + ppb * S);
+    for (int i = threadIdx.x; i < ppb * BW; i += blockDim.x) bm[i] = 0u;
+    const int tid = threadIdx.x;
+    const int pl = tid / S, s = tid - pl * S;
+    const long long a = (long long)blockIdx.x * ppb + pl;
+    const bool live = pl < ppb && a < n;
+    const int l = live ? slot_list(topK, a, s, K, P) : 0;
+    const int hs = live ? hub_id[a] : -1;
+    if (live) lids[tid] = l;
+    __syncthreads();
+    if (live) atomicOr(&bm[pl * BW + (l >> 5)], 1u << (l & 31));
+    __syncthreads();
+    if (!live) return;
+    first_mask_entry(l, lids + pl * S, bm + pl * BW, s, S, P, M, W, seg, pairs, inter, a, hs, hub_ptr, hub_mem, F);
 }
 
 // Grouping of the (point, slot) entries by (query block, list) without
@@ -1539,8 +1564,11 @@ int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, 
     hipLaunchKernelGGL(k_pair_sort, dim3(2 * P), dim3(256), 0, st, poff, M, pairs, seg);
     const int ppb = std::max(1, 512 / S);
     const long long blocks = (n + ppb - 1) / ppb;
-    hipLaunchKernelGGL(k_first_mask, dim3((unsigned)blocks), dim3(ppb * S), (size_t)ppb * S * 4, st, topK, n, P, K, M,
-                       W, ppb, poff, pairs, seg, hub_id, hub_ptr, hub_mem, inter, F);
+    // tables in LDS when they take at most 48 KB (with the slot lists): at
+    // least three workgroups of 512 threads per CU
+    const size_t lds = (size_t)ppb * S * 4 + (size_t)ppb * ((2 * P + 31) / 32) * 4;
+    hipLaunchKernelGGL(k_first_mask, dim3((unsigned)blocks), dim3(ppb * S), lds, st, topK, n, P, K, M, W, ppb, pairs,
+                       seg, hub_id, hub_ptr, hub_mem, inter, F);
     return (int)hipGetLastError();
 }
 

@@ -187,13 +187,16 @@ struct ClusterArgs {
     int* labels;
     uint32_t* adj;       // L2 x M x W list-local co-hit adjacency
     unsigned long long* n_core;
+    int spread;          // n_core copies (spread_add): kSpread - 1, or 0 for one counter
 };
 int launch_degree(const ClusterArgs& c, hipStream_t st);
 int launch_corehub(const ClusterArgs& c, hipStream_t st);
 int launch_uf_init(const ClusterArgs& c, hipStream_t st);
 int launch_uf_lists(const ClusterArgs& c, hipStream_t st);
 int launch_core_minpts(const int* degree, long long n, int minPts, uint8_t* core, unsigned long long* n_core,
-                       hipStream_t st);
+                       int spread, hipStream_t st);
+// *out += the kSpread copies of a spread counter
+int launch_spread_fold(const unsigned long long* ctr, unsigned long long* out, hipStream_t st);
 int launch_uf_points(const ClusterArgs& c, hipStream_t st);
 int launch_uf_compress(const ClusterArgs& c, hipStream_t st);
 int launch_root_flags(const ClusterArgs& c, int* flags, hipStream_t st);

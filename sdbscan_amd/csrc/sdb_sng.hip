@@ -71,6 +71,7 @@ __global__ void k_sng_rev(int n_samples, const int* __restrict__ samples, const 
 __global__ void k_sng_degree(long long n, int L, int S2, int W, const uint32_t* __restrict__ E,
                              const int* __restrict__ rev, int minPts, int* __restrict__ degree,
                              uint8_t* __restrict__ core, unsigned long long* __restrict__ n_core) {
+    // n_core: kSpread copies (spread_add)
     const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = a < n;
     bool is_core = false;
@@ -83,7 +84,7 @@ __global__ void k_sng_degree(long long n, int L, int S2, int W, const uint32_t* 
         core[a] = is_core ? 1 : 0;
     }
     const unsigned long long b = __ballot(is_core);
-    if (lane_id() == 0 && b) atomicAdd(n_core, (unsigned long long)__popcll(b));
+    if (lane_id() == 0 && b) spread_add(n_core, kSpread - 1, (unsigned long long)__popcll(b));
 }
 
 __device__ __forceinline__ int sng_find(int* par, int x) {
@@ -233,6 +234,7 @@ __global__ void k_uniq_fill(long long n, const long long* __restrict__ ptr, cons
 
 __global__ void k_rowsize_core(long long n, const long long* __restrict__ ptr, int minPts, int* __restrict__ degree,
                                uint8_t* __restrict__ core, unsigned long long* __restrict__ n_core) {
+    // n_core: kSpread copies (spread_add)
     const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = a < n;
     bool is_core = false;
@@ -243,7 +245,7 @@ __global__ void k_rowsize_core(long long n, const long long* __restrict__ ptr, i
         core[a] = is_core ? 1 : 0;
     }
     const unsigned long long b = __ballot(is_core);
-    if (lane_id() == 0 && b) atomicAdd(n_core, (unsigned long long)__popcll(b));
+    if (lane_id() == 0 && b) spread_add(n_core, kSpread - 1, (unsigned long long)__popcll(b));
 }
 
 static inline unsigned nb256(long long n) { return (unsigned)((n + 255) / 256); }
