@@ -332,11 +332,12 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
         ENSURE(c->rowerr_max, 4);
         HIPCHK(hipMemsetAsync(c->rowerr_max.p, 0, 4, st));
     }
+    const int rho_pad = bf ? bf16_rho_pad(c) : 0;
     int errf = 0;
     if (!fused) {
         LCHK(launch_transform(c->X, n, d, ldx, q.metric, P_<float>(c->Xn), bf ? P_<uint16_t>(c->Xb) : nullptr,
                               bf ? P_<float>(c->rowerr) : nullptr, bf ? P_<uint32_t>(c->rowerr_max) : nullptr,
-                              P_<int>(c->err_flag), st));
+                              rho_pad, P_<int>(c->err_flag), st));
         HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (errf & 1) return fail(c, SDB_ENEGATIVE, "Error: X is not non-negative !");
@@ -406,6 +407,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ia.Xb_out = bf ? P_<uint16_t>(c->Xb) : nullptr;
     ia.rowerr = bf ? P_<float>(c->rowerr) : nullptr;
     ia.rowerr_max = bf ? P_<uint32_t>(c->rowerr_max) : nullptr;
+    ia.rho_pad = rho_pad;
     ia.err = P_<int>(c->err_flag);
     ia.signmask = P_<uint32_t>(c->bits);
     ia.sample_stride = stride;
@@ -770,6 +772,7 @@ static int run_distances_p(sdb_ctx* c, float eps, int precision, int* nq_out, do
     double band = 0, band32 = 0;
     compute_bands(q, prec, eps, &band, &band32);
     da.X = prec == SDB_BF16 ? (const void*)c->Xb.p : (const void*)c->Xn.p;
+    da.rho_pad = prec == SDB_BF16 ? bf16_rho_pad(c) : 0;
     da.X32 = P_<float>(c->Xn);
     da.grp = P_<uint32_t>(c->grp);
     da.tile_list = dt; da.tile_beg = dt + ntiles; da.tile_end = dt + 2 * ntiles;
@@ -1465,7 +1468,7 @@ static int ensure_transformed(sdb_ctx* c) {
     HIPCHK(hipMemsetAsync(c->err_flag.p, 0, 4, st));
     LCHK(launch_transform(c->X, n, c->d, c->ldx, q.metric, P_<float>(c->Xn), bf ? P_<uint16_t>(c->Xb) : nullptr,
                           bf ? P_<float>(c->rowerr) : nullptr, bf ? P_<uint32_t>(c->rowerr_max) : nullptr,
-                          P_<int>(c->err_flag), st));
+                          bf ? bf16_rho_pad(c) : 0, P_<int>(c->err_flag), st));
     int errf = 0;
     HIPCHK(hipMemcpyAsync(&errf, c->err_flag.p, 4, hipMemcpyDeviceToHost, st));
     if (bf) HIPCHK(hipMemcpyAsync(&c->rho_max, c->rowerr_max.p, 4, hipMemcpyDeviceToHost, st));

@@ -59,6 +59,13 @@ __device__ __forceinline__ void spread_add(unsigned long long* ctr, int spread, 
     atomicAdd(ctr + (size_t)(blockIdx.x & (unsigned)spread) * kSpreadStride, v);
 }
 
+// BF16 bits of a non-negative finite float rounded towards +infinity (an
+// upper bound of the value in BF16).
+__device__ __forceinline__ uint32_t bf16_up(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u >> 16) + ((u & 0xFFFFu) ? 1u : 0u);
+}
+
 __device__ __forceinline__ int wave_sum_i32(int v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);

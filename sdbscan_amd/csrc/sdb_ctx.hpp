@@ -289,6 +289,11 @@ static inline void compute_bands(const sdb_params& q, int prec, float eps, doubl
     }
 }
 
+// Whether the context's BF16 rows carry their rounding bound in their last
+// padding element: whenever a row has padding (d < ldx).  Every writer of the
+// BF16 rows and every tile that reads them follow this one rule.
+static inline int bf16_rho_pad(const sdb_ctx* c) { return c->d < c->ldx ? 1 : 0; }
+
 static inline sdb::ClusterArgs cluster_args(sdb_ctx* c) {
     const sdb_params& q = c->prm;
     sdb::ClusterArgs ca{};

@@ -133,7 +133,14 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
                     for (int u = 0; u < BATCH; ++u) {
                         const int v = v0 + u * WAVES * 64;
                         const int row = v / vpr, cv = v - row * vpr;
-                        if (v < total) *(uint4*)(smem + row * RSB + (size_t)cv * 16) = xs[u];
+                        uint4 x = xs[u];
+                        // BF16 rows with their bound in element ldx - 1 (the
+                        // high half of the row's last 16-byte vector): zero
+                        // there, so that it adds nothing to the dot products
+                        if constexpr (BF) {
+                            if (a.rho_pad && k0 + kn == a.ldx && cv == vpr - 1) x.w &= 0xFFFFu;
+                        }
+                        if (v < total) *(uint4*)(smem + row * RSB + (size_t)cv * 16) = x;
                     }
                 }
                 __syncthreads();
@@ -376,12 +383,22 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             for (int u = 0; u < BATCH; ++u) {
                 const int v = v0 + u * WAVES * 64;
                 const int row = v / (NS * 4), cv = v - row * (NS * 4);
-                if (v < total) *(uint4*)(smem + row * RSB + cv * 16) = xs[u];
+                uint4 x = xs[u];
+                if constexpr (!F32L2) {
+                    // rows with their bound in element ldx - 1 (the high half
+                    // of the row's last 16-byte vector): the list's largest
+                    // bound from there, and a zero in its place
+                    if (a.rho_pad && cv == vpr - 1 && v < total) {
+                        atomicMax(rc_lds, x.w & 0xFFFF0000u);  // bounds are >= 0
+                        x.w &= 0xFFFFu;
+                    }
+                }
+                if (v < total) *(uint4*)(smem + row * RSB + cv * 16) = x;
             }
         }
     }
     if constexpr (!F32L2) {
-        if (tid < M) atomicMax(rc_lds, __float_as_uint(a.rowerr[pts_lds[tid]]));  // bounds are >= 0
+        if (!a.rho_pad && tid < M) atomicMax(rc_lds, __float_as_uint(a.rowerr[pts_lds[tid]]));  // bounds are >= 0
     }
     __syncthreads();
     // thresholds with the list's largest candidate bound; the query's own
@@ -404,8 +421,13 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
 #pragma unroll
             for (int x = 0; x < EW; ++x)
                 Fw[x] = (valid && x < a.W) ? __builtin_nontemporal_load(&a.F[(size_t)ent * a.W + x]) : 0u;
-            // BF16: the query row's rounding bound; F32L2: its squared norm
-            const float eq = F32L2 ? a.rnorm2[ent_point(ent, a.magicS)] : a.rowerr[ent_point(ent, a.magicS)];
+            // BF16: the query row's rounding bound (from the row's last
+            // padding element, read below, when the rows carry it); F32L2:
+            // its squared norm
+            float eq = 0.0f;
+            if (F32L2) eq = a.rnorm2[ent_point(ent, a.magicS)];
+            else if (!a.rho_pad) eq = a.rowerr[ent_point(ent, a.magicS)];
+            uint32_t eqb = 0u;  // the last k-step's high word of this lane's row piece
             uint32_t ent2;
             bool valid2;
             entry_of(min(b + 2 * WAVES, nblk - 1), ent2, valid2);
@@ -449,6 +471,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
                         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, cf[t]), qv,
                                                                          acc[t], 0, 0, 0);
                 }
+                if (st == NS - 1) eqb = af[st].w;  // (the last real step: step_off clamps)
                 af[st] = *(const uint4*)(np + step_off(st));
                 if (st + 1 < NS) __builtin_amdgcn_sched_group_barrier(0x100, NT, 0);
                 __builtin_amdgcn_sched_group_barrier(0x008, F32L2 ? 4 * NT : NT, 0);
@@ -461,6 +484,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             }
 
             // epilogue: lane (r16, g) holds s(query r16, candidate t*16 + 4g + i) in acc[t][i]
+            if (!F32L2 && a.rho_pad) {
+                // element ldx - 1 sits in the high half of lane (r16, 3)'s word
+                eq = __uint_as_float((uint32_t)__shfl((int)eqb, (lane & 15) + 48, kWave) & 0xFFFF0000u);
+            }
             uint32_t hw[EW];
 #pragma unroll
             for (int x = 0; x < EW; ++x) hw[x] = 0u;

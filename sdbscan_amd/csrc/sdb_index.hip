@@ -23,7 +23,8 @@ namespace sdb {
 __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, long long n, int d,
                                                    int ldx, int metric, float* __restrict__ Xn,
                                                    uint16_t* __restrict__ Xb, float* __restrict__ rowerr,
-                                                   uint32_t* __restrict__ rowerr_max, int* __restrict__ err) {
+                                                   uint32_t* __restrict__ rowerr_max, int rho_pad,
+                                                   int* __restrict__ err) {
     const int lane = lane_id();
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const long long nwaves = (long long)gridDim.x * blockDim.x / kWave;
@@ -64,14 +65,17 @@ __global__ __launch_bounds__(256) void k_transform(const float* __restrict__ X, 
             if (ob) {
                 const float sv = metric == kChi2 ? sqrtf(v) : v;
                 const uint32_t h = f32_to_bf16_rne(sv);
-                ob[i] = (uint16_t)h;
+                if (!rho_pad || i != ldx - 1) ob[i] = (uint16_t)h;
                 const float df = __uint_as_float(h << 16) - sv;
                 sq = fmaf(df, df, sq);
             }
         }
         if (ob) {
             const float rho = bf16_row_err(wave_sum_f32(sq), metric == kChi2);
-            if (lane == 0) rowerr[p] = rho;
+            if (lane == 0) {
+                rowerr[p] = rho;
+                if (rho_pad) ob[ldx - 1] = (uint16_t)bf16_up(rho);
+            }
             rmax = fmaxf(rmax, rho);
         }
     }
@@ -788,7 +792,11 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
                         // BF16 rows, round to nearest even (hardware pair
                         // conversion); Chi2: of the square roots (the rows whose
                         // dot product bounds chi2 from below)
-                        const __amdgpu_buffer_rsrc_t rb = row_rsrc(a.Xb_out + p * a.ldx, a.ldx * 2);
+                        // (with the bound in element ldx - 1, the range ends
+                        // before it: the zero of that padding element is
+                        // dropped and the bound is stored below)
+                        const __amdgpu_buffer_rsrc_t rb =
+                            row_rsrc(a.Xb_out + p * a.ldx, (a.ldx - (a.rho_pad ? 1 : 0)) * 2);
                         auto bsrc = [&](int j) { return EMB == kEmbCos ? v[j] : sqrtf(v[j]); };
                         // with the squared rounding errors of the row (the
                         // per-pair band of the BF16 tiles); elements past d
@@ -812,7 +820,10 @@ __global__ __launch_bounds__(256, LOGD <= 10 ? 3 : (LOGD == 11 ? 2 : 1)) void k_
                             sq = fmaf(d0, d0, sq);
                         }
                         const float rho = bf16_row_err(wave_sum_f32(sq), EMB != kEmbCos);
-                        if (lane == 0) a.rowerr[p] = rho;
+                        if (lane == 0) {
+                            a.rowerr[p] = rho;
+                            if (a.rho_pad) a.Xb_out[p * a.ldx + a.ldx - 1] = (uint16_t)bf16_up(rho);
+                        }
                         rho_max = fmaxf(rho_max, rho);
                     }
                 }
@@ -1396,13 +1407,14 @@ int launch_rff_embed(const float* Xn, long long n, int d, int ldx, const float* 
 }
 
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
-                     uint16_t* Xb, float* rowerr, uint32_t* rowerr_max, int* err, hipStream_t st) {
+                     uint16_t* Xb, float* rowerr, uint32_t* rowerr_max, int rho_pad, int* err, hipStream_t st) {
     if (Xb && (!rowerr || !rowerr_max)) return -3;
+    if (rho_pad && d >= ldx) return -3;
     long long blocks = (n + 3) / 4;
     if (blocks > 65536) blocks = 65536;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_transform, dim3((unsigned)blocks), dim3(256), 0, st, X, n, d, ldx, metric, Xn, Xb, rowerr,
-                       rowerr_max, err);
+                       rowerr_max, rho_pad, err);
     return (int)hipGetLastError();
 }
 
@@ -1458,6 +1470,7 @@ int index_embedding(int metric, int d, int ker) {
 }
 
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st) {
+    if (a.rho_pad && (a.d >= a.ldx || !a.Xb_out)) return -3;
     switch (logD) {
         case 6: return launch_index_d<6>(a, mode, grid, st);
         case 7: return launch_index_d<7>(a, mode, grid, st);

@@ -44,6 +44,7 @@ struct IndexArgs {
     int ldE;
     float* rowerr;        // n: BF16 rounding bound of each row (full pass with Xb_out), or null
     uint32_t* rowerr_max; // its maximum (float bits), or null
+    int rho_pad;          // 1: each BF16 row also carries its bound, rounded up, in element ldx - 1
 };
 
 struct DistArgs {
@@ -79,6 +80,10 @@ struct DistArgs {
     // bound over the tile's list)
     const float* rowerr;
     float th_hit, th_near;
+    // the BF16 rows carry their bound (rounded up to BF16) in their last
+    // padding element, ldx - 1 (d < ldx): the tiles read it with the row and
+    // zero it in the staged candidate rows, so the dot products stay exact
+    int rho_pad;
     // L2 on the FP32 MFMA tiles: squared row norms (FP32) and the per-pair
     // decision on D = |x|^2 + |y|^2 - 2 x.y: hit iff D < eps2 - band, queued
     // iff not a hit and D <= eps2 + band, band = kappa (|x|^2 + |y|^2) + eps0
@@ -124,7 +129,7 @@ int index_embedding(int metric, int d, int ker);
 int launch_rff_embed(const float* Xn, long long n, int d, int ldx, const float* R, int half, float* E, int ldE,
                      hipStream_t st);
 int launch_transform(const float* X, long long n, int d, int ldx, int metric, float* Xn,
-                     uint16_t* Xb, float* rowerr, uint32_t* rowerr_max, int* err, hipStream_t st);
+                     uint16_t* Xb, float* rowerr, uint32_t* rowerr_max, int rho_pad, int* err, hipStream_t st);
 int launch_index(const IndexArgs& a, int logD, int mode, int grid, hipStream_t st);
 int launch_scatter_rows(const int* idx, const int* rows, int count, int width, int* dst, hipStream_t st);
 int launch_topm_threshold(const uint32_t* cmax, const uint32_t* cmin, int G, int P, int M,
