@@ -173,7 +173,7 @@ void sdb_destroy(sdb_ctx* c) {
                      &c->hub_excl, &c->hub_id, &c->hub_cnt, &c->hub_ptr, &c->hub_cur, &c->hub_mem,
                      &c->inv, &c->pos_hub, &c->pcnt, &c->poff, &c->pairs, &c->pseg, &c->F, &c->wgcnt, &c->wgoff, &c->grp_ptr, &c->grp,
                      &c->scan_tmp, &c->total, &c->err_flag, &c->row_slot, &c->row_buf, &c->E,
-                     &c->colsum, &c->queue, &c->qcount, &c->qkeys, &c->qvals, &c->qsort_tmp, &c->inter, &c->core_spread, &c->rnorm2, &c->rnorm2_max, &c->tiles, &c->degree, &c->core,
+                     &c->colsum, &c->queue, &c->qcount, &c->qkeys, &c->qvals, &c->qsort_tmp, &c->inter, &c->core_spread, &c->ffull, &c->rnorm2, &c->rnorm2_max, &c->tiles, &c->degree, &c->core,
                      &c->stats64, &c->corehub, &c->parent, &c->border, &c->labels, &c->adj,
                      &c->rootf, &c->rank, &c->nb_deg, &c->nb_ptr, &c->nb_cur, &c->nb_idx,
                      &c->nb_dist, &c->nb_sorted, &c->core_dist, &c->pos, &c->reach, &c->ns_ids,
@@ -556,6 +556,7 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     LCHK(launch_pair_fill(P_<int>(c->pos_hub), P_<int>(c->hub_ptr), P_<int>(c->hub_mem), L2, M, P_<int>(c->poff),
                           P_<uint32_t>(c->pairs), st));
     ENSURE(c->F, (size_t)n * S * W * 4);
+    ENSURE(c->ffull, (size_t)n * S);
     ENSURE(c->pseg, (size_t)L2 * 8);
     // lists that share many points (long pair segments, e.g. the extreme
     // points of blobs in many lists): the dense intersections replace one
@@ -570,7 +571,8 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     }
     LCHK(launch_first_mask(P_<int>(c->topK), n, P, K, M, W, P_<int>(c->poff), P_<uint32_t>(c->pairs), P_<int2>(c->pseg),
                            P_<int>(c->hub_id), P_<int>(c->hub_ptr), P_<int>(c->hub_mem),
-                           use_inter ? P_<uint32_t>(c->inter) : nullptr, P_<uint32_t>(c->F), st));
+                           use_inter ? P_<uint32_t>(c->inter) : nullptr, P_<uint32_t>(c->F), P_<uint8_t>(c->ffull),
+                           st));
     // group the (point, slot) entries by (query block, list): the distance
     // tiles then sweep the points block by block, and a 192 MB block of query
     // rows stays in the 256 MB last-level cache, so most of the 2K visits of a
@@ -593,7 +595,8 @@ int sdb_build_index(sdb_ctx* c, const uint8_t* hd3_bits, const float* Rh) {
     ENSURE(c->scan_tmp, scan_tmp_bytes((long long)NG * Gx));
     LCHK(launch_group_count(P_<int>(c->topK), n, P, K, L2, qblk, nblk, Gx, P_<int>(c->wgcnt), st));
     LCHK(scan_exclusive_i32(P_<int>(c->wgcnt), P_<int>(c->wgoff), (long long)NG * Gx, nullptr, c->scan_tmp.p, c->scan_tmp.bytes, st));
-    LCHK(launch_group_scatter(P_<int>(c->topK), n, P, K, L2, qblk, nblk, Gx, P_<int>(c->wgoff), P_<uint32_t>(c->grp), P_<int>(c->grp_ptr), st));
+    LCHK(launch_group_scatter(P_<int>(c->topK), n, P, K, L2, qblk, nblk, Gx, P_<int>(c->wgoff), P_<uint8_t>(c->ffull),
+                              P_<uint32_t>(c->grp), P_<int>(c->grp_ptr), st));
     c->h_grp_ptr.resize(NG + 1);
     HIPCHK(hipMemcpyAsync(c->h_grp_ptr.data(), c->grp_ptr.p, (size_t)(NG + 1) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(c->ev[2], st));

@@ -59,6 +59,19 @@ __device__ __forceinline__ void spread_add(unsigned long long* ctr, int spread, 
     atomicAdd(ctr + (size_t)(blockIdx.x & (unsigned)spread) * kSpreadStride, v);
 }
 
+// Group entries (the grouped (point, slot) entries the distance tiles walk):
+// bits 0-30 hold the entry a * S + s (n * S < 2^31), bit 31 is set when the
+// entry's first-occurrence mask is the full one (every list position, none
+// repeated and not the point itself), which the tiles then do not load.
+constexpr uint32_t kEntFull = 0x80000000u;
+__device__ __forceinline__ uint32_t ent_id(uint32_t g) { return g & ~kEntFull; }
+__device__ __forceinline__ bool ent_full(uint32_t g) { return (g & kEntFull) != 0u; }
+// Word x of the full first-occurrence mask over M positions.
+__device__ __forceinline__ uint32_t fmask_full_word(int x, int M) {
+    const int nb = M - 32 * x;
+    return nb >= 32 ? 0xFFFFFFFFu : (nb > 0 ? (1u << nb) - 1u : 0u);
+}
+
 // BF16 bits of a non-negative finite float rounded towards +infinity (an
 // upper bound of the value in BF16).
 __device__ __forceinline__ uint32_t bf16_up(float f) {

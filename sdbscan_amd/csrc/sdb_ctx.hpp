@@ -99,6 +99,7 @@ struct sdb_ctx {
     // dense list intersections (inter[l][L], W words) when lists share many points
     DevBuf inter;
     DevBuf core_spread;  // kSpread copies of the core count (spread_add)
+    DevBuf ffull;        // n x S: 1 where the first-occurrence mask is the full one
     // squared row norms for the L2 MFMA tiles and their maximum; whether the
     // last distance pass used those tiles (the accuracy sample follows it)
     DevBuf rnorm2, rnorm2_max;

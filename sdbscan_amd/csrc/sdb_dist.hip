@@ -98,13 +98,15 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
     for (int set = 0; set < nsets; ++set) {
         uint32_t ent[QT];
         long long pa[QT];
-        bool valid[QT];
+        bool valid[QT], ffl[QT];
         float nq[QT];  // L2X: squared norm of the lane's query row
 #pragma unroll
         for (int q = 0; q < QT; ++q) {
             int rr = beg + set * rows_per_set + (w * QT + q) * 16 + r16;
             valid[q] = rr < end;
-            ent[q] = a.grp[valid[q] ? rr : beg];
+            const uint32_t gq = a.grp[valid[q] ? rr : beg];
+            ent[q] = ent_id(gq);
+            ffl[q] = ent_full(gq);
             pa[q] = (long long)(ent[q] / (uint32_t)a.S);
             nq[q] = L2X ? a.rnorm2[pa[q]] : 0.0f;
         }
@@ -203,7 +205,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_mfma(DistArgs a) {
             uint32_t Fw[EW], Ew[EW];
 #pragma unroll
             for (int x = 0; x < EW; ++x) {
-                Fw[x] = x < a.W ? a.F[(size_t)ent[q] * a.W + x] : 0u;
+                Fw[x] = x < a.W ? (ffl[q] ? fmask_full_word(x, M) : a.F[(size_t)ent[q] * a.W + x]) : 0u;
                 Ew[x] = 0u;
             }
 #pragma unroll
@@ -314,10 +316,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
     const size_t rowb = (size_t)a.ldx * SZ;
 
     const int nblk = (end - beg + 15) >> 4;
-    auto entry_of = [&](int blk, uint32_t& e, bool& v) {
+    auto entry_of = [&](int blk, uint32_t& e, bool& v, bool& f) {
         const int rr = beg + blk * 16 + r16;
         v = rr < end;
-        e = a.grp[v ? rr : beg];
+        const uint32_t gv = a.grp[v ? rr : beg];
+        e = ent_id(gv);
+        f = ent_full(gv);
     };
     auto row_of = [&](uint32_t e) {
         const size_t p = (size_t)ent_point(e, a.magicS);
@@ -339,9 +343,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
     const bool active = w < nblk;
     int b = w;
     uint32_t ent, entn;
-    bool valid, validn;
-    entry_of(min(b, nblk - 1), ent, valid);
-    entry_of(min(b + WAVES, nblk - 1), entn, validn);
+    bool valid, validn, full, fulln;
+    entry_of(min(b, nblk - 1), ent, valid, full);
+    entry_of(min(b + WAVES, nblk - 1), entn, validn, fulln);
     uint4 af[NS];
     {
         const char* rp = row_of(ent);
@@ -420,7 +424,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             uint32_t Fw[EW];
 #pragma unroll
             for (int x = 0; x < EW; ++x)
-                Fw[x] = (valid && x < a.W) ? __builtin_nontemporal_load(&a.F[(size_t)ent * a.W + x]) : 0u;
+                Fw[x] = (valid && x < a.W)
+                            ? (full ? fmask_full_word(x, M) : __builtin_nontemporal_load(&a.F[(size_t)ent * a.W + x]))
+                            : 0u;
             // BF16: the query row's rounding bound (from the row's last
             // padding element, read below, when the rows carry it); F32L2:
             // its squared norm
@@ -429,8 +435,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
             else if (!a.rho_pad) eq = a.rowerr[ent_point(ent, a.magicS)];
             uint32_t eqb = 0u;  // the last k-step's high word of this lane's row piece
             uint32_t ent2;
-            bool valid2;
-            entry_of(min(b + 2 * WAVES, nblk - 1), ent2, valid2);
+            bool valid2, full2;
+            entry_of(min(b + 2 * WAVES, nblk - 1), ent2, valid2, full2);
             const char* np = row_of(entn);
 
             // the candidate fragments are the same for every block; an empty
@@ -544,8 +550,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_pipe(DistArgs a) {
                 }
             }
             if (!has_next) break;
-            ent = entn; valid = validn;
-            entn = ent2; validn = valid2;
+            ent = entn; valid = validn; full = fulln;
+            entn = ent2; validn = valid2; fulln = full2;
             b += WAVES;
         }
     }
@@ -689,12 +695,14 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu(DistArgs a) {
     for (int set = 0; set < nsets; ++set) {
         uint32_t ent[QV];
         long long pa[QV];
-        bool valid[QV];
+        bool valid[QV], ffl[QV];
 #pragma unroll
         for (int q = 0; q < QV; ++q) {
             int rr = beg + set * rows_per_set + w * QV + q;
             valid[q] = rr < end;
-            ent[q] = a.grp[valid[q] ? rr : beg];
+            const uint32_t gq = a.grp[valid[q] ? rr : beg];
+            ent[q] = ent_id(gq);
+            ffl[q] = ent_full(gq);
             pa[q] = (long long)(ent[q] / (uint32_t)a.S);
         }
         float acc[QV][CM];
@@ -742,7 +750,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu(DistArgs a) {
         for (int q = 0; q < QV; ++q) {
             uint32_t Fw[2 * CM];
 #pragma unroll
-            for (int x = 0; x < 2 * CM; ++x) Fw[x] = (valid[q] && x < a.W) ? a.F[(size_t)ent[q] * a.W + x] : 0u;
+            for (int x = 0; x < 2 * CM; ++x)
+                Fw[x] = (valid[q] && x < a.W) ? (ffl[q] ? fmask_full_word(x, M) : a.F[(size_t)ent[q] * a.W + x]) : 0u;
 #pragma unroll
             for (int c = 0; c < CM; ++c) {
                 const int ci = lane + 64 * c;
@@ -824,13 +833,15 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu1(DistArgs a) {
     const int nsets = (end - beg + per_set - 1) / per_set;
     float pre[QV][KPL];
     uint32_t entn[QV];
-    bool validn[QV];
+    bool validn[QV], ffn[QV];
     auto fetch = [&](int set) {
 #pragma unroll
         for (int q = 0; q < QV; ++q) {
             const int rr = beg + set * per_set + w * QV + q;
             validn[q] = rr < end;
-            entn[q] = a.grp[validn[q] ? rr : beg];
+            const uint32_t gq = a.grp[validn[q] ? rr : beg];
+            entn[q] = ent_id(gq);
+            ffn[q] = ent_full(gq);
             const long long pa = (long long)(entn[q] / (uint32_t)a.S);
             const float* xr = a.X32 + (size_t)pa * a.ldx;
 #pragma unroll
@@ -843,11 +854,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu1(DistArgs a) {
     if (nsets > 0) fetch(0);
     for (int set = 0; set < nsets; ++set) {
         uint32_t ent[QV];
-        bool valid[QV];
+        bool valid[QV], ffl[QV];
 #pragma unroll
         for (int q = 0; q < QV; ++q) {
             ent[q] = entn[q];
             valid[q] = validn[q];
+            ffl[q] = ffn[q];
 #pragma unroll
             for (int t = 0; t < KPL; ++t) {
                 const int k = lane + 64 * t;
@@ -877,7 +889,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu1(DistArgs a) {
         for (int q = 0; q < QV; ++q) {
             uint32_t Fw[2 * CM];
 #pragma unroll
-            for (int x = 0; x < 2 * CM; ++x) Fw[x] = (valid[q] && x < a.W) ? a.F[(size_t)ent[q] * a.W + x] : 0u;
+            for (int x = 0; x < 2 * CM; ++x)
+                Fw[x] = (valid[q] && x < a.W) ? (ffl[q] ? fmask_full_word(x, M) : a.F[(size_t)ent[q] * a.W + x]) : 0u;
 #pragma unroll
             for (int c = 0; c < CM; ++c) {
                 const int ci = lane + 64 * c;
@@ -951,13 +964,15 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu2(DistArgs a, int Mc) {
     const int nsets = (end - beg + per_set - 1) / per_set;
     float pre[QV][KPL];
     uint32_t entn[QV];
-    bool validn[QV];
+    bool validn[QV], ffn[QV];
     auto fetch = [&](int set) {
 #pragma unroll
         for (int q = 0; q < QV; ++q) {
             const int rr = beg + set * per_set + w * QV + q;
             validn[q] = rr < end;
-            entn[q] = a.grp[validn[q] ? rr : beg];
+            const uint32_t gq = a.grp[validn[q] ? rr : beg];
+            entn[q] = ent_id(gq);
+            ffn[q] = ent_full(gq);
             const long long pa = (long long)(entn[q] / (uint32_t)a.S);
             const float* xr = a.X32 + (size_t)pa * a.ldx;
 #pragma unroll
@@ -971,11 +986,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu2(DistArgs a, int Mc) {
     for (int set = 0; set < nsets; ++set) {
         float cur[QV][KPL];
         uint32_t ent[QV];
-        bool valid[QV];
+        bool valid[QV], ffl[QV];
 #pragma unroll
         for (int q = 0; q < QV; ++q) {
             ent[q] = entn[q];
             valid[q] = validn[q];
+            ffl[q] = ffn[q];
 #pragma unroll
             for (int t = 0; t < KPL; ++t) cur[q][t] = pre[q][t];
         }
@@ -1004,7 +1020,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_dist_valu2(DistArgs a, int Mc) {
             else if constexpr (METRIC == kChi2) dist = 1.0f - 2.0f * s;
             else if constexpr (METRIC == kJS) dist = 1.0f - s * 0.5f;
             else dist = 1.0f - s;
-            const uint32_t fw = (valid[q] && colok) ? a.F[(size_t)ent[q] * a.W + (ci >> 5)] : 0u;
+            const uint32_t fw = (valid[q] && colok)
+                                    ? (ffl[q] ? fmask_full_word(ci >> 5, M) : a.F[(size_t)ent[q] * a.W + (ci >> 5)])
+                                    : 0u;
             const bool act = valid[q] && colok && ((fw >> (ci & 31)) & 1u);
             const double dv = (double)dist;
             const bool hit = act && dv < lo;

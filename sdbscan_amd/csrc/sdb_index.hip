@@ -1214,13 +1214,14 @@ __global__ __launch_bounds__(256) void k_pair_sort(const int* __restrict__ poff,
 }
 
 // The mask words of entry (a, s) with list l: mine = the lists of a's
-// slots, bm = the bitmap of those lists, hs = hub_id[a].
+// slots, bm = the bitmap of those lists, hs = hub_id[a]; full[entry] = 1 when
+// the mask is the full one (the grouping marks those entries).
 __device__ __forceinline__ void first_mask_entry(int l, const int* mine, const uint32_t* bm, int s, int S, int P,
                                                  int M, int W, const int2* __restrict__ seg,
                                                  const uint32_t* __restrict__ pairs,
                                                  const uint32_t* __restrict__ inter, long long a, int hs,
                                                  const int* __restrict__ hub_ptr, const int* __restrict__ hub_mem,
-                                                 uint32_t* __restrict__ F) {
+                                                 uint32_t* __restrict__ F, uint8_t* __restrict__ full) {
     uint32_t m[kMaxW];
 #pragma unroll
     for (int w = 0; w < kMaxW; ++w) {
@@ -1296,6 +1297,14 @@ __device__ __forceinline__ void first_mask_entry(int l, const int* mine, const u
             }
         }
     }
+    // full masks are only flagged: the tiles never read F for a flagged
+    // entry, so its words are not stored
+    bool all = true;
+#pragma unroll
+    for (int w = 0; w < kMaxW; ++w)
+        if (w < W) all &= m[w] == fmask_full_word(w, M);
+    full[(size_t)a * S + s] = all ? 1 : 0;
+    if (all) return;
     uint32_t* Fo = F + ((size_t)a * S + s) * W;
     if (W == 4) {
         *(uint4*)Fo = uint4{m[0], m[1], m[2], m[3]};
@@ -1313,7 +1322,7 @@ __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK
                                                     const int* __restrict__ hub_ptr,
                                                     const int* __restrict__ hub_mem,
                                                     const uint32_t* __restrict__ inter,
-                                                    uint32_t* __restrict__ F) {
+                                                    uint32_t* __restrict__ F, uint8_t* __restrict__ full) {
     // LDS: the slot lists of the workgroup's points, then per point a bitmap
     // over the 2P list ids with the bits of its own slot lists set
     extern __shared__ int lids[];
@@ -1331,7 +1340,8 @@ __global__ __launch_bounds__(512) void k_first_mask(const int* __restrict__ topK
     if (live) atomicOr(&bm[pl * BW + (l >> 5)], 1u << (l & 31));
     __syncthreads();
     if (!live) return;
-    first_mask_entry(l, lids + pl * S, bm + pl * BW, s, S, P, M, W, seg, pairs, inter, a, hs, hub_ptr, hub_mem, F);
+    first_mask_entry(l, lids + pl * S, bm + pl * BW, s, S, P, M, W, seg, pairs, inter, a, hs, hub_ptr, hub_mem, F,
+                     full);
 }
 
 // Grouping of the (point, slot) entries by (query block, list) without
@@ -1371,6 +1381,7 @@ __global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ top
 __global__ __launch_bounds__(256) void k_group_scatter(const int* __restrict__ topK, long long n, int P,
                                                        int K, int L2, long long qblk, int Gx,
                                                        const int* __restrict__ wgoff,
+                                                       const uint8_t* __restrict__ full,
                                                        uint32_t* __restrict__ grp) {
     extern __shared__ int cur[];
     long long t0, t1;
@@ -1378,11 +1389,29 @@ __global__ __launch_bounds__(256) void k_group_scatter(const int* __restrict__ t
     for (int i = threadIdx.x; i < L2; i += blockDim.x)
         cur[i] = wgoff[((size_t)blockIdx.y * L2 + i) * Gx + blockIdx.x];
     __syncthreads();
-    for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    // the entries in the order of k_group_count's pass (the order inside a
+    // group changes how the distance tiles hit the caches: measured), with the
+    // next entry's loads issued before the current entry's atomic
+    auto load = [&](long long t, int& li, uint32_t& gv) {
         const uint32_t a = (uint32_t)t / (uint32_t)(2 * K);  // n * 2K < 2^31 (sdb_set_params)
         const int s = (int)((uint32_t)t - a * (uint32_t)(2 * K));
-        const int pos = atomicAdd(&cur[slot_list(topK, a, s, K, P)], 1);
-        grp[pos] = (uint32_t)t;
+        li = slot_list(topK, a, s, K, P);
+        gv = (uint32_t)t | (full[t] ? kEntFull : 0u);
+    };
+    long long t = t0 + threadIdx.x;
+    int li = 0;
+    uint32_t gv = 0u;
+    if (t < t1) load(t, li, gv);
+    while (t < t1) {
+        const long long tn = t + blockDim.x;
+        int lin = 0;
+        uint32_t gvn = 0u;
+        if (tn < t1) load(tn, lin, gvn);
+        const int pos = atomicAdd(&cur[li], 1);
+        grp[pos] = gv;
+        t = tn;
+        li = lin;
+        gv = gvn;
     }
 }
 
@@ -1572,7 +1601,7 @@ int launch_inter_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem
 
 int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const int* poff,
                       uint32_t* pairs, int2* seg, const int* hub_id, const int* hub_ptr, const int* hub_mem,
-                      const uint32_t* inter, uint32_t* F, hipStream_t st) {
+                      const uint32_t* inter, uint32_t* F, uint8_t* full, hipStream_t st) {
     const int S = 2 * K;
     hipLaunchKernelGGL(k_pair_sort, dim3(2 * P), dim3(256), 0, st, poff, M, pairs, seg);
     const int ppb = std::max(1, 512 / S);
@@ -1581,7 +1610,7 @@ int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, 
     // least three workgroups of 512 threads per CU
     const size_t lds = (size_t)ppb * S * 4 + (size_t)ppb * ((2 * P + 31) / 32) * 4;
     hipLaunchKernelGGL(k_first_mask, dim3((unsigned)blocks), dim3(ppb * S), lds, st, topK, n, P, K, M, W, ppb, pairs,
-                       seg, hub_id, hub_ptr, hub_mem, inter, F);
+                       seg, hub_id, hub_ptr, hub_mem, inter, F, full);
     return (int)hipGetLastError();
 }
 
@@ -1592,8 +1621,9 @@ int launch_group_count(const int* topK, long long n, int P, int K, int L2, long 
 }
 
 int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
-                         int Gx, const int* wgoff, uint32_t* grp, int* grp_ptr, hipStream_t st) {
-    hipLaunchKernelGGL(k_group_scatter, dim3(Gx, nblk), dim3(256), (size_t)L2 * 4, st, topK, n, P, K, L2, qblk, Gx, wgoff, grp);
+                         int Gx, const int* wgoff, const uint8_t* full, uint32_t* grp, int* grp_ptr, hipStream_t st) {
+    hipLaunchKernelGGL(k_group_scatter, dim3(Gx, nblk), dim3(256), (size_t)L2 * 4, st, topK, n, P, K, L2, qblk, Gx, wgoff,
+                       full, grp);
     const int NG = nblk * L2;
     hipLaunchKernelGGL(k_group_ptr, dim3((NG + 1 + 255) / 256), dim3(256), 0, st, wgoff, NG, Gx, n * 2 * K, grp_ptr);
     return (int)hipGetLastError();

@@ -150,14 +150,14 @@ int launch_inter_fill(const int* pos_hub, const int* hub_ptr, const int* hub_mem
                       uint32_t* inter, hipStream_t st);
 int launch_first_mask(const int* topK, long long n, int P, int K, int M, int W, const int* poff,
                       uint32_t* pairs, int2* seg, const int* hub_id, const int* hub_ptr, const int* hub_mem,
-                      const uint32_t* inter, uint32_t* F, hipStream_t st);
+                      const uint32_t* inter, uint32_t* F, uint8_t* full, hipStream_t st);
 int launch_group_count(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
                        int Gx, int* wgcnt, hipStream_t st);
 int launch_row_norm2(const float* X, long long n, int d, int ldx, float* out, uint32_t* maxbits, hipStream_t st);
 int launch_queue_keys(const unsigned long long* queue, int nq, const int* lists, int M, uint32_t* keys,
                       hipStream_t st);
 int launch_group_scatter(const int* topK, long long n, int P, int K, int L2, long long qblk, int nblk,
-                         int Gx, const int* wgoff, uint32_t* grp, int* grp_ptr, hipStream_t st);
+                         int Gx, const int* wgoff, const uint8_t* full, uint32_t* grp, int* grp_ptr, hipStream_t st);
 
 // distances
 int launch_dist_mfma(const DistArgs& a, int precision, int ntiles, hipStream_t st);

@@ -58,12 +58,14 @@ __global__ __launch_bounds__(512) void k_sweep_tiles(const SweepJob* __restrict_
         for (int set = 0; set < nsets; ++set) {
             uint32_t ent[QT];
             long long pa[QT];
-            bool valid[QT];
+            bool valid[QT], ffl[QT];
 #pragma unroll
             for (int q = 0; q < QT; ++q) {
                 const int rr = beg + set * rows_per_set + (w * QT + q) * 16 + r16;
                 valid[q] = rr < end;
-                ent[q] = J->grp[valid[q] ? rr : beg];
+                const uint32_t gq = J->grp[valid[q] ? rr : beg];
+                ent[q] = ent_id(gq);
+                ffl[q] = ent_full(gq);
                 pa[q] = (long long)ent_point(ent[q], J->magicS);
             }
             f32x4_s acc[QT][NT];
@@ -122,7 +124,8 @@ __global__ __launch_bounds__(512) void k_sweep_tiles(const SweepJob* __restrict_
             for (int q = 0; q < QT; ++q) {
                 uint32_t Fw[EW];
 #pragma unroll
-                for (int x = 0; x < EW; ++x) Fw[x] = (valid[q] && x < W) ? J->F[(size_t)ent[q] * W + x] : 0u;
+                for (int x = 0; x < EW; ++x)
+                    Fw[x] = (valid[q] && x < W) ? (ffl[q] ? fmask_full_word(x, M) : J->F[(size_t)ent[q] * W + x]) : 0u;
                 uint32_t act = 0u;  // bit 4t + i: pair (row 4g + i, column t*16 + r16) is evaluated
                 uint32_t er[4];
 #pragma unroll
