@@ -81,6 +81,75 @@ __global__ __launch_bounds__(256) void k_csr_dist(const float* X, int ldx, int d
     }
 }
 
+// The per-lane term of the neighbourhood distances (as k_csr_dist's fold).
+__device__ __forceinline__ float csr_term(int metric, float t, float xv, float yv) {
+    if (metric == kCosine) return fmaf(xv, yv, t);
+    if (metric == kL1) return t + fabsf(xv - yv);
+    if (metric == kL2) {
+        const float df = xv - yv;
+        return fmaf(df, df, t);
+    }
+    if (metric == kChi2) return (xv > 0.0f && yv > 0.0f) ? t + xv * yv / (xv + yv) : t;
+    const float xe = xv + 1e-6f, ye = yv + 1e-6f, u = xe + ye;
+    return t + xe * (logf(u / xe) * 1.4426950408889634f) + ye * (logf(u / ye) * 1.4426950408889634f);
+}
+
+// The same distances as k_csr_dist, term for term and in the same order
+// (lane l sums i = l, l + 64, ... < d in turn, then the xor butterfly), for
+// rows of at most 64 NR elements: the point's row stays in registers and J
+// neighbour rows are loaded together before their sums, so that a wave has
+// J row loads in flight instead of one.
+template <int NR, int J>
+__global__ __launch_bounds__(256) void k_csr_dist_j(const float* X, int ldx, int d, int metric, long long n,
+                                                    const long long* ptr, const int* idx, float* dist) {
+    const int lane = lane_id();
+    const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long a = wv; a < n; a += nwv) {
+        float xr[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const int i = lane + 64 * k;
+            xr[k] = i < d ? X[(size_t)a * ldx + i] : 0.0f;
+        }
+        const long long e = ptr[a + 1];
+        for (long long j = ptr[a]; j < e; j += J) {
+            float yv[J][NR];
+#pragma unroll
+            for (int u = 0; u < J; ++u) {
+                const long long c = j + u < e ? (long long)idx[j + u] : a;
+#pragma unroll
+                for (int k = 0; k < NR; ++k) {
+                    const int i = lane + 64 * k;
+                    yv[u][k] = i < d ? X[(size_t)c * ldx + i] : 0.0f;
+                }
+            }
+            float s[J];
+#pragma unroll
+            for (int u = 0; u < J; ++u) {
+                s[u] = 0.0f;
+#pragma unroll
+                for (int k = 0; k < NR; ++k)
+                    if (lane + 64 * k < d) s[u] = csr_term(metric, s[u], xr[k], yv[u][k]);
+            }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+                for (int u = 0; u < J; ++u) s[u] = s[u] + __shfl_xor(s[u], m, kWave);
+#pragma unroll
+            for (int u = 0; u < J; ++u) {
+                float r;
+                if (metric == kCosine) r = 1.0f - s[u];
+                else if (metric == kL1) r = s[u];
+                else if (metric == kL2) r = sqrtf(s[u]);
+                else if (metric == kChi2) r = 1.0f - 2.0f * s[u];
+                else r = 1.0f - s[u] * 0.5f;
+                if (lane == u && j + u < e) dist[j + u] = r;
+            }
+        }
+    }
+}
+
 __global__ void k_kth(long long n, const long long* ptr, const float* sorted, const uint8_t* core,
                       int k, float* out) {
     long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,18 +159,24 @@ __global__ void k_kth(long long n, const long long* ptr, const float* sorted, co
     out[a] = v;
 }
 
-__global__ void k_reach(long long n, const long long* ptr, const int* idx, const float* dist,
-                        const uint8_t* core, const float* core_dist, const int* pos,
-                        uint32_t* reach_key) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !core[i]) return;
-    const int pi = pos[i];
-    const float cd = core_dist[i];
-    for (long long j = ptr[i]; j < ptr[i + 1]; ++j) {
-        const int q = idx[j];
-        if (pos[q] <= pi) continue;
-        const float r = fmaxf(cd, dist[j]);
-        atomicMin(&reach_key[q], ord_key(r));
+// One wavefront per core point i, its lanes striding over i's neighbourhood
+// (the minima do not depend on the order of the updates).
+__global__ __launch_bounds__(256) void k_reach(long long n, const long long* ptr, const int* idx, const float* dist,
+                                               const uint8_t* core, const float* core_dist, const int* pos,
+                                               uint32_t* reach_key) {
+    const int lane = lane_id();
+    const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long i = wv; i < n; i += nwv) {
+        if (!core[i]) continue;
+        const int pi = pos[i];
+        const float cd = core_dist[i];
+        for (long long j = ptr[i] + lane; j < ptr[i + 1]; j += kWave) {
+            const int q = idx[j];
+            if (pos[q] <= pi) continue;
+            const uint32_t k = ord_key(fmaxf(cd, dist[j]));
+            if (k < (uint32_t)load_relaxed((const int*)&reach_key[q])) atomicMin(&reach_key[q], k);
+        }
     }
 }
 
@@ -245,7 +320,13 @@ int launch_csr_dist(const float* X, int ldx, int d, int metric, long long n, con
                     const int* idx, float* dist, hipStream_t st) {
     long long blocks = (n + 3) / 4;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_csr_dist, dim3((unsigned)blocks), dim3(256), 0, st, X, ldx, d, metric, n, ptr, idx, dist);
+    const dim3 g((unsigned)blocks), b(256);
+    if (d <= 64) hipLaunchKernelGGL((k_csr_dist_j<1, 8>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
+    else if (d <= 128) hipLaunchKernelGGL((k_csr_dist_j<2, 8>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
+    else if (d <= 256) hipLaunchKernelGGL((k_csr_dist_j<4, 4>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
+    else if (d <= 512) hipLaunchKernelGGL((k_csr_dist_j<8, 4>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
+    else if (d <= 1024) hipLaunchKernelGGL((k_csr_dist_j<16, 2>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
+    else hipLaunchKernelGGL(k_csr_dist, g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
     return (int)hipGetLastError();
 }
 
@@ -260,7 +341,9 @@ int launch_reach(long long n, const long long* ptr, const int* idx, const float*
                  hipStream_t st) {
     hipError_t e = hipMemsetAsync(reach_key, 0xff, (size_t)n * 4, st);
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL(k_reach, dim3(nb(n, 256)), dim3(256), 0, st, n, ptr, idx, dist, core, core_dist, pos, reach_key);
+    const long long rblocks = std::min<long long>(65536, (n + 3) / 4);
+    hipLaunchKernelGGL(k_reach, dim3((unsigned)std::max<long long>(rblocks, 1)), dim3(256), 0, st, n, ptr, idx, dist,
+                       core, core_dist, pos, reach_key);
     hipLaunchKernelGGL(k_reach_final, dim3(nb(n, 256)), dim3(256), 0, st, n, reach_key);
     return (int)hipGetLastError();
 }
