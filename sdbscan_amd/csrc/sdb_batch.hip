@@ -9,7 +9,8 @@
 // Per job: one index, one distance pass per eps (ascending, so the largest eps
 // is last and sOPTICS reuses it), one core test per minPts.  The tile rows of
 // the distance pass are tuned once per (n_features, metric, precision) class
-// by the first job of the class.
+// by the first job of the class.  The walks' host lists come from one pool
+// shared by the workers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,6 +27,7 @@
 #include <vector>
 
 #include "../../include/sdbscan_amd.h"
+#include "sdb_ctx.hpp"
 
 namespace {
 
@@ -61,7 +63,7 @@ int check_job(sdb_batch_job& j) {
     return SDB_OK;
 }
 
-void run_job(sdb_ctx* c, sdb_batch_job& j, int autotune, TuneTable& tune) {
+void run_job(sdb_ctx* c, sdb_batch_job& j, int autotune, TuneTable& tune, HostListPool& lists) {
     const double t0 = now_ms();
     const sdb_params& p = j.params;
     const long long n = p.n_points;
@@ -130,7 +132,7 @@ void run_job(sdb_ctx* c, sdb_batch_job& j, int autotune, TuneTable& tune) {
         std::vector<int> ord;
         int* op = j.ordering;
         if (!op) { ord.resize((size_t)j.n_minpts * n); op = ord.data(); }
-        r = sdb_optics_orders(c, j.n_minpts, j.minPts, op, j.reach);
+        r = optics_orders(c, j.n_minpts, j.minPts, op, j.reach, &lists);
         if (r) { job_fail(j, r, "optics_orders", c); return; }
     }
     j.ms = now_ms() - t0;
@@ -152,6 +154,7 @@ extern "C" int sdb_batch_run(int device, sdb_batch_job* jobs, int n_jobs, int n_
     });
     std::atomic<int> next{0};
     TuneTable tune;
+    HostListPool lists;  // the walks' host lists, reused from job to job
     const int nw = std::min(n_streams, std::max(n_jobs, 1));
     std::vector<int> create_rc(nw, SDB_OK);
     auto worker = [&](int w) {
@@ -167,7 +170,7 @@ extern "C" int sdb_batch_run(int device, sdb_batch_job* jobs, int n_jobs, int n_
                 std::snprintf(j.error, sizeof(j.error), "sdb_create failed on device %d", device);
                 continue;
             }
-            run_job(c, j, autotune, tune);
+            run_job(c, j, autotune, tune, lists);
         }
         if (c) sdb_destroy(c);
     };
@@ -175,6 +178,7 @@ extern "C" int sdb_batch_run(int device, sdb_batch_job* jobs, int n_jobs, int n_
     for (int w = 1; w < nw; ++w) th.emplace_back(worker, w);
     worker(0);
     for (auto& t : th) t.join();
+    lists.clear();
     if (wall_ms) *wall_ms = now_ms() - t0;
     for (int i = 0; i < n_jobs; ++i)
         if (jobs[i].status != SDB_OK) return jobs[i].status;

@@ -1211,10 +1211,13 @@ int sdb_get_neighbors(sdb_ctx* c, long long* ptr, int* idx, float* dist) {
     int r = build_neighbors(c);
     if (r) return r;
     const long long n = c->n;
-    HIPCHK(hipMemcpy(ptr, c->nb_ptr.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToHost));
+    hipStream_t st = c->st;
+    HIPCHK(hipMemcpyAsync(ptr, c->nb_ptr.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     const long long tot = ptr[n];
-    if (idx && tot) HIPCHK(hipMemcpy(idx, c->nb_idx.p, (size_t)tot * 4, hipMemcpyDeviceToHost));
-    if (dist && tot) HIPCHK(hipMemcpy(dist, c->nb_dist.p, (size_t)tot * 4, hipMemcpyDeviceToHost));
+    if (idx && tot) HIPCHK(hipMemcpyAsync(idx, c->nb_idx.p, (size_t)tot * 4, hipMemcpyDeviceToHost, st));
+    if (dist && tot) HIPCHK(hipMemcpyAsync(dist, c->nb_dist.p, (size_t)tot * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return SDB_OK;
 }
 
@@ -1364,7 +1367,10 @@ static void optics_walk(long long n, const long long* ptr, const int* idx, const
     }
 }
 
-int sdb_optics_orders(sdb_ctx* c, int n_minpts, const int* minPts, int* orderings, float* reaches) {
+}  // extern "C"
+
+// C++ linkage: shared with the batched drivers (declared in sdb_ctx.hpp)
+int optics_orders(sdb_ctx* c, int n_minpts, const int* minPts, int* orderings, float* reaches, HostListPool* pool) {
     if (!c || n_minpts < 1 || !minPts || !orderings) return SDB_EINVAL;
     if (!c->cored) return fail(c, SDB_ESTATE, "find_core first");
     // sample-sequence neighbourhoods carry the core flags of their own call
@@ -1376,10 +1382,19 @@ int sdb_optics_orders(sdb_ctx* c, int n_minpts, const int* minPts, int* ordering
     std::vector<long long> ptr(n + 1);
     int r = sdb_get_neighbors(c, ptr.data(), nullptr, nullptr);
     if (r) return r;
-    std::vector<int> idx_v(std::max<long long>(ptr[n], 1));
-    std::vector<float> dd_v(std::max<long long>(ptr[n], 1));
-    int* idx = idx_v.data();
-    float* dd = dd_v.data();
+    HostListPool own;
+    HostListPool& lp = pool ? *pool : own;
+    const size_t nent = (size_t)std::max<long long>(ptr[n], 1);
+    HostLists* lists = lp.take(nent * 8);
+    struct Give {
+        HostListPool& lp;
+        HostLists* h;
+        ~Give() { lp.give(h); }
+    } give{lp, lists};
+    if (!lists->reserve(nent * 8))
+        return fail(c, SDB_ENOMEM, "%zu bytes of host memory for the neighbourhood lists", nent * 8);
+    int* idx = (int*)lists->p;
+    float* dd = (float*)((char*)lists->p + nent * 4);
     r = sdb_get_neighbors(c, ptr.data(), idx, dd);
     if (r) return r;
     std::vector<std::vector<uint8_t>> core(n_minpts, std::vector<uint8_t>(n));
@@ -1412,6 +1427,12 @@ int sdb_optics_orders(sdb_ctx* c, int n_minpts, const int* minPts, int* ordering
     }
     if (c->sng) return SDB_OK;
     return sdb_find_core(c, eps, keep, nullptr, nullptr);  // the core flags of the last find_core again
+}
+
+extern "C" {
+
+int sdb_optics_orders(sdb_ctx* c, int n_minpts, const int* minPts, int* orderings, float* reaches) {
+    return optics_orders(c, n_minpts, minPts, orderings, reaches, nullptr);
 }
 
 int sdb_optics_order(sdb_ctx* c, int minPts, int* ordering_out, float* reach_out) {

@@ -3,12 +3,15 @@
 // and the batched sweep (sdb_sweep.hip).  Private to the library.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <sys/mman.h>
 
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstring>
 #include <cstdio>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -308,3 +311,89 @@ static inline sdb::ClusterArgs cluster_args(sdb_ctx* c) {
     return ca;
 }
 
+
+// Host memory for the neighbourhood lists the sOPTICS walks read (ids, then
+// distances): an anonymous mapping aligned to 2 MiB with transparent huge
+// pages asked for and every page faulted in when it is mapped
+// (MADV_POPULATE_WRITE; one store per 4 KiB page where the kernel lacks it).
+// The device-to-host copies then write into resident pages -- the runtime's
+// copies into pageable memory are serialised between host threads, and page
+// faults taken inside them stall every other thread's copy -- and no pass
+// zero-fills the lists first.  Grows, never shrinks.
+struct HostLists {
+    void* base = nullptr;  // the mapping
+    size_t mapped = 0;
+    void* p = nullptr;  // 2 MiB-aligned start inside it
+    size_t cap = 0;
+    HostLists() = default;
+    HostLists(const HostLists&) = delete;
+    HostLists& operator=(const HostLists&) = delete;
+    ~HostLists() { drop(); }
+    void drop() {
+        if (base) munmap(base, mapped);
+        base = p = nullptr;
+        mapped = cap = 0;
+    }
+    // true when at least `bytes` are mapped and resident
+    bool reserve(size_t bytes) {
+        if (bytes <= cap) return true;
+        drop();
+        const size_t huge = (size_t)2 << 20;
+        const size_t want = (bytes + huge - 1) / huge * huge;
+        void* m = mmap(nullptr, want + huge, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+        if (m == MAP_FAILED) return false;
+        base = m;
+        mapped = want + huge;
+        p = (void*)(((uintptr_t)m + huge - 1) & ~(uintptr_t)(huge - 1));
+        cap = want;
+        madvise(p, cap, MADV_HUGEPAGE);  // advisory: ignored where THP is off
+#ifdef MADV_POPULATE_WRITE
+        if (madvise(p, cap, MADV_POPULATE_WRITE) == 0) return true;
+#endif
+        for (size_t o = 0; o < cap; o += 4096) ((volatile char*)p)[o] = 0;
+        return true;
+    }
+};
+
+// The lists of a pool of host threads that run the walks of many contexts in
+// turn (the batched runs): each walk takes the free lists that fit best and
+// gives them back, so pages faulted in for one dataset serve the next.
+struct HostListPool {
+    std::mutex mu;
+    std::vector<std::unique_ptr<HostLists>> all;
+    std::vector<HostLists*> idle;
+    // the idle lists of the smallest capacity >= bytes, else the largest idle
+    // ones (grown by the caller's reserve), else new ones
+    HostLists* take(size_t bytes) {
+        std::lock_guard<std::mutex> g(mu);
+        int fit = -1, big = -1;
+        for (int i = 0; i < (int)idle.size(); ++i) {
+            const size_t cp = idle[i]->cap;
+            if (cp >= bytes && (fit < 0 || cp < idle[fit]->cap)) fit = i;
+            if (big < 0 || cp > idle[big]->cap) big = i;
+        }
+        const int k = fit >= 0 ? fit : big;
+        if (k >= 0) {
+            HostLists* h = idle[k];
+            idle.erase(idle.begin() + k);
+            return h;
+        }
+        all.emplace_back(new HostLists());
+        return all.back().get();
+    }
+    void give(HostLists* h) {
+        std::lock_guard<std::mutex> g(mu);
+        idle.push_back(h);
+    }
+    // unmaps every list (none may be taken)
+    void clear() {
+        std::lock_guard<std::mutex> g(mu);
+        idle.clear();
+        all.clear();
+    }
+};
+
+// sdb_optics_orders with the host lists taken from `pool` (a private set when
+// null).
+__attribute__((visibility("hidden"))) int optics_orders(sdb_ctx* c, int n_minpts, const int* minPts,
+                                                      int* orderings, float* reaches, HostListPool* pool);

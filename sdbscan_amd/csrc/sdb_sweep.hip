@@ -314,7 +314,7 @@ int chain_copy_out(sdb_batch_job& j, JobState& s) {
 // sOPTICS at the largest eps: the context takes that eps's hit masks, column
 // counts and degrees (the last eps of the chain's degree slices), then the
 // host walks of sdb_optics_orders.
-int optics_job(sdb_batch_job& j, JobState& s) {
+int optics_job(sdb_batch_job& j, JobState& s, HostListPool& lists) {
     sdb_ctx* c = s.c;
     HIPCHK(hipSetDevice(c->device));
     const SweepJob& d = s.desc;
@@ -339,7 +339,7 @@ int optics_job(sdb_batch_job& j, JobState& s) {
         ord.resize((size_t)nm * c->n);
         op = ord.data();
     }
-    return sdb_optics_orders(c, nm, j.minPts, op, j.reach);
+    return optics_orders(c, nm, j.minPts, op, j.reach, &lists);
 }
 
 }  // namespace
@@ -634,16 +634,22 @@ extern "C" int sdb_batch_sweep(int device, sdb_batch_job* jobs, int n_jobs, int 
     });
     st_ms[8] = now_ms() - t;
 
-    // 5. sOPTICS at the largest eps
+    // 5. sOPTICS at the largest eps; the walks and their neighbourhood lists
+    // scale with the point count, so the most points go first
     t = now_ms();
-    for_jobs(order, nw, [&](int i) {
+    std::vector<int> order5(order);
+    std::stable_sort(order5.begin(), order5.end(),
+                     [&](int x, int y) { return jobs[x].params.n_points > jobs[y].params.n_points; });
+    HostListPool lists;
+    for_jobs(order5, nw, [&](int i) {
         sdb_batch_job& j = jobs[i];
         if (j.status || !j.optics || (!j.ordering && !j.reach)) return;
         const double tj = now_ms();
-        const int r = optics_job(j, S[i]);
+        const int r = optics_job(j, S[i], lists);
         if (r) set_job_error(j, r, "optics", S[i].c);
         S[i].ms += now_ms() - tj;
     });
+    lists.clear();
     st_ms[9] = now_ms() - t;
     for (int i = 0; i < n_jobs; ++i) jobs[i].ms = S[i].ms;
     t = now_ms();
