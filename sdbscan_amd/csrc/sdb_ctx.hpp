@@ -316,10 +316,10 @@ static inline sdb::ClusterArgs cluster_args(sdb_ctx* c) {
 // distances): an anonymous mapping aligned to 2 MiB with transparent huge
 // pages asked for and every page faulted in when it is mapped
 // (MADV_POPULATE_WRITE; one store per 4 KiB page where the kernel lacks it).
-// The device-to-host copies then write into resident pages -- the runtime's
-// copies into pageable memory are serialised between host threads, and page
-// faults taken inside them stall every other thread's copy -- and no pass
-// zero-fills the lists first.  Grows, never shrinks.
+// The device-to-host copies then write into resident pages (copies into
+// pageable memory from several host threads wait on each other, so a page
+// fault inside one delays the rest), and no pass zero-fills the lists first.
+// Grows, never shrinks.
 struct HostLists {
     void* base = nullptr;  // the mapping
     size_t mapped = 0;
