@@ -36,7 +36,9 @@ int launch_gen_dirichlet(float* X, long long n, int d, unsigned long long seed, 
 int launch_csr_fill(const ClusterArgs& c, const long long* ptr, int* cursor, int* idx,
                     hipStream_t st);
 int launch_csr_dist(const float* X, int ldx, int d, int metric, long long n, const long long* ptr,
-                    const int* idx, float* dist, hipStream_t st);
+                    const int* idx, float* dist, int upper, hipStream_t st);
+int launch_csr_mirror(long long n, const long long* ptr, const int* idx, float* dist, unsigned int* missing,
+                      hipStream_t st);
 int launch_reach(long long n, const long long* ptr, const int* idx, const float* dist,
                  const uint8_t* core, const float* core_dist, const int* pos, uint32_t* reach_key,
                  hipStream_t st);
@@ -1195,8 +1197,30 @@ static int build_neighbors(sdb_ctx* c) {
         HIPCHK(hipStreamSynchronize(st));
         release(tmp);
         if (e != hipSuccess) return fail(c, SDB_EHIP, "segmented sort failed: %s", hipGetErrorString(e));
-        LCHK(launch_csr_dist(P_<float>(c->Xn), c->ldx, c->d, c->prm.metric, n, P_<long long>(c->nb_ptr),
-                             P_<int>(c->nb_idx), P_<float>(c->nb_dist), st));
+        // each symmetric pair's distance once: the entries with id >= the
+        // row's point, then the others copied from their mirror entries; if
+        // any entry has no mirror, every entry is computed.  Rows of up to
+        // 128 features are computed whole: there the mirror's binary searches
+        // cost more than the half pass saves (400k points: d = 32 68 -> 30 +
+        // 49 ms, d = 128 128 -> 65 + 74 ms, d = 784 766 -> 369 + 74 ms).
+        bool full = c->d <= 128 || c->d > 1024 || c->prm.metric == SDB_JS;
+        if (!full) {
+            DevBuf miss;
+            if (ensure(c, miss, 16)) return SDB_ENOMEM;
+            HIPCHK(hipMemsetAsync(miss.p, 0, 4, st));
+            LCHK(launch_csr_dist(P_<float>(c->Xn), c->ldx, c->d, c->prm.metric, n, P_<long long>(c->nb_ptr),
+                                 P_<int>(c->nb_idx), P_<float>(c->nb_dist), 1, st));
+            LCHK(launch_csr_mirror(n, P_<long long>(c->nb_ptr), P_<int>(c->nb_idx), P_<float>(c->nb_dist),
+                                   P_<unsigned int>(miss), st));
+            unsigned int missing = 0;
+            HIPCHK(hipMemcpyAsync(&missing, miss.p, 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            release(miss);
+            full = missing != 0;
+        }
+        if (full)
+            LCHK(launch_csr_dist(P_<float>(c->Xn), c->ldx, c->d, c->prm.metric, n, P_<long long>(c->nb_ptr),
+                                 P_<int>(c->nb_idx), P_<float>(c->nb_dist), 0, st));
     }
     HIPCHK(hipStreamSynchronize(st));
     c->have_nb = true;
@@ -1567,7 +1591,7 @@ static int build_neighbors_sng(sdb_ctx* c) {
     LCHK(launch_uniq_fill(n, P_<long long>(c->csr_ptr), sorted, P_<long long>(c->nb_ptr), P_<int>(c->nb_idx), st));
     if (utot > 0)
         LCHK(launch_csr_dist(P_<float>(c->Xn), c->ldx, c->d, c->prm.metric, n, P_<long long>(c->nb_ptr),
-                             P_<int>(c->nb_idx), P_<float>(c->nb_dist), st));
+                             P_<int>(c->nb_idx), P_<float>(c->nb_dist), 0, st));
     HIPCHK(hipStreamSynchronize(st));
     c->have_nb = true;
     c->sorted_dist = false;

@@ -46,7 +46,8 @@ __global__ void k_csr_fill(ClusterArgs c, const long long* ptr, int* cursor, int
 
 // FP32 distance of every CSR entry, one wavefront per point; sums run per
 // lane with stride 64 and combine through an xor butterfly, so d(a,c) and
-// d(c,a) are bitwise equal.
+// d(c,a) are bitwise equal for every metric but JS (whose two terms are added
+// in operand order).
 __global__ __launch_bounds__(256) void k_csr_dist(const float* X, int ldx, int d, int metric,
                                                   long long n, const long long* ptr, const int* idx,
                                                   float* dist) {
@@ -101,7 +102,8 @@ __device__ __forceinline__ float csr_term(int metric, float t, float xv, float y
 // J row loads in flight instead of one.
 template <int NR, int J>
 __global__ __launch_bounds__(256) void k_csr_dist_j(const float* X, int ldx, int d, int metric, long long n,
-                                                    const long long* ptr, const int* idx, float* dist) {
+                                                    const long long* ptr, const int* idx, float* dist,
+                                                    int upper) {
     const int lane = lane_id();
     const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
@@ -113,7 +115,17 @@ __global__ __launch_bounds__(256) void k_csr_dist_j(const float* X, int ldx, int
             xr[k] = i < d ? X[(size_t)a * ldx + i] : 0.0f;
         }
         const long long e = ptr[a + 1];
-        for (long long j = ptr[a]; j < e; j += J) {
+        long long jb = ptr[a];
+        if (upper) {
+            // skip the entries with id < a (the row is sorted by id): 64 ids a step
+            for (;;) {
+                const long long q = jb + lane;
+                const int below = __popcll(__ballot(q < e && (long long)idx[q] < a));
+                jb += below;
+                if (below < 64) break;
+            }
+        }
+        for (long long j = jb; j < e; j += J) {
             float yv[J][NR];
 #pragma unroll
             for (int u = 0; u < J; ++u) {
@@ -317,16 +329,58 @@ int launch_csr_fill(const ClusterArgs& c, const long long* ptr, int* cursor, int
 }
 
 int launch_csr_dist(const float* X, int ldx, int d, int metric, long long n, const long long* ptr,
-                    const int* idx, float* dist, hipStream_t st) {
+                    const int* idx, float* dist, int upper, hipStream_t st) {
+    if (upper && (d > 1024 || metric == kJS)) return (int)hipErrorInvalidValue;
     long long blocks = (n + 3) / 4;
     if (blocks > 65536) blocks = 65536;
     const dim3 g((unsigned)blocks), b(256);
-    if (d <= 64) hipLaunchKernelGGL((k_csr_dist_j<1, 8>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
-    else if (d <= 128) hipLaunchKernelGGL((k_csr_dist_j<2, 8>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
-    else if (d <= 256) hipLaunchKernelGGL((k_csr_dist_j<4, 4>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
-    else if (d <= 512) hipLaunchKernelGGL((k_csr_dist_j<8, 4>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
-    else if (d <= 1024) hipLaunchKernelGGL((k_csr_dist_j<16, 2>), g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
+#define SDB_CSR_ARGS X, ldx, d, metric, n, ptr, idx, dist, upper
+    if (d <= 64) hipLaunchKernelGGL((k_csr_dist_j<1, 8>), g, b, 0, st, SDB_CSR_ARGS);
+    else if (d <= 128) hipLaunchKernelGGL((k_csr_dist_j<2, 8>), g, b, 0, st, SDB_CSR_ARGS);
+    else if (d <= 256) hipLaunchKernelGGL((k_csr_dist_j<4, 4>), g, b, 0, st, SDB_CSR_ARGS);
+    else if (d <= 512) hipLaunchKernelGGL((k_csr_dist_j<8, 4>), g, b, 0, st, SDB_CSR_ARGS);
+    else if (d <= 1024) hipLaunchKernelGGL((k_csr_dist_j<16, 2>), g, b, 0, st, SDB_CSR_ARGS);
     else hipLaunchKernelGGL(k_csr_dist, g, b, 0, st, X, ldx, d, metric, n, ptr, idx, dist);
+#undef SDB_CSR_ARGS
+    return (int)hipGetLastError();
+}
+
+// The lower entries (a, c), c < a, of the neighbourhoods after the upper pass
+// of k_csr_dist_j: each takes the distance of the entry (c, a) of row c,
+// found by binary search (rows are sorted by id).  The two are the same
+// number: the sums run over the same products in the same lane order with
+// the operands swapped, and every term but JS's is symmetric in them
+// (fmaf(x, y, t) == fmaf(y, x, t), (x - y)^2 == (y - x)^2, |x - y|, xy / (x + y)).
+// Entries without a mirror are counted in *missing; the caller then runs the
+// full pass.  One wavefront per point, lanes over the row's lower entries.
+__global__ __launch_bounds__(256) void k_csr_mirror(long long n, const long long* ptr, const int* idx, float* dist,
+                                                    unsigned int* missing) {
+    const int lane = lane_id();
+    const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwv = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long a = wv; a < n; a += nwv) {
+        const long long e = ptr[a + 1];
+        for (long long j = ptr[a] + lane; j < e; j += 64) {
+            const int c = idx[j];
+            if ((long long)c >= a) break;
+            const long long ce = ptr[c + 1];
+            long long lo = ptr[c], hi = ce;
+            while (lo < hi) {
+                const long long mid = (lo + hi) >> 1;
+                if ((long long)idx[mid] < a) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < ce && (long long)idx[lo] == a) dist[j] = dist[lo];
+            else atomicAdd(missing, 1u);
+        }
+    }
+}
+
+int launch_csr_mirror(long long n, const long long* ptr, const int* idx, float* dist, unsigned int* missing,
+                      hipStream_t st) {
+    long long blocks = (n + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_csr_mirror, dim3((unsigned)blocks), dim3(256), 0, st, n, ptr, idx, dist, missing);
     return (int)hipGetLastError();
 }
 

@@ -484,6 +484,36 @@ def test_optics_core_dist_and_reachability(ctx):
     assert np.array_equal(dreach2, reach)
 
 
+def test_optics_mirrored_neighbour_distances_d784(ctx):
+    """Rows of more than 128 features: the neighbourhood distance pass computes
+    each symmetric pair once and copies the other entry from its mirror.  The
+    distances must equal the oracle's lane-order FP32 values bit for bit,
+    every entry (a, c) must hold the bits of (c, a), and the walk over them
+    must give the oracle's ordering and reachability."""
+    X = datagen.sphere_mixture(2500, 784, 12, seed=2171, spread=0.6)[0]
+    p, bits, R = run_index(ctx, X, K=5, M=40, n_proj=256)
+    eps, minPts = 0.35, 10
+    core, deg = ctx.find_core(eps, minPts)
+    ptr, idx, dist = ctx.get_neighbors()
+    tk, tm = ctx.get_index()
+    Xt = ctx.get_transformed()
+    _, _, (optr, oidx, odd), _, _ = O.find_core(Xt, S.COSINE, tk, tm, eps, minPts, mode=3)
+    n = X.shape[0]
+    print("d784 neighbourhoods: entries", int(ptr[-1]), flush=True)
+    assert ptr[-1] > 10 * n
+    assert np.array_equal(ptr.astype(np.int32), optr) and np.array_equal(idx, oidx)
+    assert np.array_equal(dist.view(np.uint32), odd.view(np.uint32))
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(ptr))
+    key = rows * n + idx
+    mirror = np.searchsorted(key, idx.astype(np.int64) * n + rows)
+    assert np.array_equal(key[mirror], idx.astype(np.int64) * n + rows)
+    assert np.array_equal(dist.view(np.uint32), dist[mirror].view(np.uint32))
+    cd = ctx.core_dist(minPts)
+    order, reach = O.optics(core, cd, ptr.astype(np.int32), idx, dist)
+    dorder, dreach = ctx.optics_order(minPts)
+    assert np.array_equal(dorder, order) and np.array_equal(dreach, reach)
+
+
 def test_find_core_minpts_reuses_distances(ctx):
     """A second find_core at the same eps only moves the core test; its
     results equal a fresh index's."""
