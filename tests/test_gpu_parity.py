@@ -484,22 +484,29 @@ def test_optics_core_dist_and_reachability(ctx):
     assert np.array_equal(dreach2, reach)
 
 
-def test_optics_mirrored_neighbour_distances_d784(ctx):
+@pytest.mark.parametrize("metric", [S.COSINE, S.CHI2])
+def test_optics_mirrored_neighbour_distances(ctx, metric):
     """Rows of more than 128 features: the neighbourhood distance pass computes
     each symmetric pair once and copies the other entry from its mirror.  The
     distances must equal the oracle's lane-order FP32 values bit for bit,
     every entry (a, c) must hold the bits of (c, a), and the walk over them
     must give the oracle's ordering and reachability."""
-    X = datagen.sphere_mixture(2500, 784, 12, seed=2171, spread=0.6)[0]
-    p, bits, R = run_index(ctx, X, K=5, M=40, n_proj=256)
-    eps, minPts = 0.35, 10
+    if metric == S.COSINE:
+        X = datagen.sphere_mixture(2500, 784, 12, seed=2171, spread=0.6)[0]
+        p, bits, R = run_index(ctx, X, K=5, M=40, n_proj=256)
+        eps = 0.35
+    else:
+        X, _ = datagen.dirichlet(2500, 300, k=5, seed=2173)
+        p, bits, R = run_index(ctx, X, metric=S.CHI2, K=5, M=40, n_proj=256, ker=1024)
+        eps = 0.3
+    minPts = 10
     core, deg = ctx.find_core(eps, minPts)
     ptr, idx, dist = ctx.get_neighbors()
     tk, tm = ctx.get_index()
     Xt = ctx.get_transformed()
-    _, _, (optr, oidx, odd), _, _ = O.find_core(Xt, S.COSINE, tk, tm, eps, minPts, mode=3)
+    _, _, (optr, oidx, odd), _, _ = O.find_core(Xt, metric, tk, tm, eps, minPts, mode=3)
     n = X.shape[0]
-    print("d784 neighbourhoods: entries", int(ptr[-1]), flush=True)
+    print(f"metric {metric} d={X.shape[1]} neighbourhoods: entries {int(ptr[-1])}, core {int(core.sum())}", flush=True)
     assert ptr[-1] > 10 * n
     assert np.array_equal(ptr.astype(np.int32), optr) and np.array_equal(idx, oidx)
     assert np.array_equal(dist.view(np.uint32), odd.view(np.uint32))
