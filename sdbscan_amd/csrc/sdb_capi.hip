@@ -42,7 +42,7 @@ int launch_csr_mirror(long long n, const long long* ptr, const int* idx, float* 
 int launch_reach(long long n, const long long* ptr, const int* idx, const float* dist,
                  const uint8_t* core, const float* core_dist, const int* pos, uint32_t* reach_key,
                  hipStream_t st);
-int launch_kth(long long n, const long long* ptr, const float* sorted, const uint8_t* core,
+int launch_kth(long long n, const long long* ptr, const float* dist, const uint8_t* core,
                int k, float* out, hipStream_t st);
 int launch_noise_list(const int* labels, long long n, int* noise, int* count, hipStream_t st);
 int launch_sng_fmask(long long n, int L, int S2, int Ms, int W, int n_samples, const int* win_end,
@@ -1172,7 +1172,7 @@ static int build_neighbors(sdb_ctx* c) {
     HIPCHK(hipMemsetAsync(c->nb_cur.p, 0, (size_t)n * 4, st));
     ENSURE(c->nb_idx, (size_t)std::max<long long>(tot, 1) * 4);
     ENSURE(c->nb_dist, (size_t)std::max<long long>(tot, 1) * 4);
-    ENSURE(c->nb_sorted, (size_t)std::max<long long>(tot, 1) * 8);
+    ENSURE(c->nb_sorted, (size_t)std::max<long long>(tot, 1) * 4);
     ClusterArgs ca = cluster_args(c);
     LCHK(launch_csr_fill(ca, P_<long long>(c->nb_ptr), P_<int>(c->nb_cur), P_<int>(c->nb_idx), st));
     {
@@ -1224,7 +1224,6 @@ static int build_neighbors(sdb_ctx* c) {
     }
     HIPCHK(hipStreamSynchronize(st));
     c->have_nb = true;
-    c->sorted_dist = false;
     return 0;
 }
 
@@ -1254,24 +1253,10 @@ int sdb_core_dist(sdb_ctx* c, int minPts, float* core_dist_out) {
     if (r) return r;
     hipStream_t st = c->st;
     const long long n = c->n;
-    long long tot = 0;
-    HIPCHK(hipMemcpy(&tot, P_<long long>(c->nb_ptr) + n, 8, hipMemcpyDeviceToHost));
     ENSURE(c->core_dist, (size_t)n * 4);
-    float* sorted = (float*)c->nb_sorted.p;
-    if (tot > 0 && !c->sorted_dist) {
-        size_t tb = 0;
-        hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, P_<float>(c->nb_dist), sorted, (int)tot, (int)n,
-                                                   P_<long long>(c->nb_ptr), P_<long long>(c->nb_ptr) + 1, 0, 32, st);
-        DevBuf tmp;
-        if (ensure(c, tmp, tb)) return SDB_ENOMEM;
-        hipError_t e = hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, tb, P_<float>(c->nb_dist), sorted, (int)tot, (int)n,
-                                                                  P_<long long>(c->nb_ptr), P_<long long>(c->nb_ptr) + 1, 0, 32, st);
-        HIPCHK(hipStreamSynchronize(st));
-        release(tmp);
-        if (e != hipSuccess) return fail(c, SDB_EHIP, "segmented sort failed: %s", hipGetErrorString(e));
-        c->sorted_dist = true;
-    }
-    LCHK(launch_kth(n, P_<long long>(c->nb_ptr), sorted, P_<uint8_t>(c->core), minPts - 2, P_<float>(c->core_dist), st));
+    // the (minPts-1)-th smallest distance of each core row, selected in place
+    LCHK(launch_kth(n, P_<long long>(c->nb_ptr), P_<float>(c->nb_dist), P_<uint8_t>(c->core), minPts - 2,
+                    P_<float>(c->core_dist), st));
     HIPCHK(hipMemcpyAsync(core_dist_out, c->core_dist.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return SDB_OK;
@@ -1561,7 +1546,7 @@ static int build_neighbors_sng(sdb_ctx* c) {
     ENSURE(c->csr_cur, (size_t)n * 8);
     HIPCHK(hipMemsetAsync(c->csr_cur.p, 0, (size_t)n * 8, st));
     ENSURE(c->csr_idx, (size_t)std::max<long long>(tot, 1) * 4);
-    ENSURE(c->nb_sorted, (size_t)std::max<long long>(tot, 1) * 8);
+    ENSURE(c->nb_sorted, (size_t)std::max<long long>(tot, 1) * 4);
     LCHK(launch_sng_csr_fill(n, L, S2, Ms, W, both, P_<uint32_t>(c->E), P_<int>(c->lists), P_<long long>(c->csr_ptr),
                              (unsigned long long*)c->csr_cur.p, P_<int>(c->csr_idx), st));
     int* sorted = (int*)c->nb_sorted.p;
@@ -1594,7 +1579,6 @@ static int build_neighbors_sng(sdb_ctx* c) {
                              P_<int>(c->nb_idx), P_<float>(c->nb_dist), 0, st));
     HIPCHK(hipStreamSynchronize(st));
     c->have_nb = true;
-    c->sorted_dist = false;
     return 0;
 }
 

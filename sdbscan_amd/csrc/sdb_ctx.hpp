@@ -121,7 +121,6 @@ struct sdb_ctx {
     // neighbourhoods / optics
     DevBuf nb_deg, nb_ptr, nb_cur, nb_idx, nb_dist, nb_sorted, core_dist, pos, reach;
     bool have_nb = false;
-    bool sorted_dist = false;  // nb_sorted holds the row-sorted neighbour distances
     bool clustered = false;
     // noise labelling
     DevBuf ns_ids, cproj, noise, nbest;

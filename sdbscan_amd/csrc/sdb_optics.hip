@@ -162,13 +162,69 @@ __global__ __launch_bounds__(256) void k_csr_dist_j(const float* X, int ldx, int
     }
 }
 
-__global__ void k_kth(long long n, const long long* ptr, const float* sorted, const uint8_t* core,
-                      int k, float* out) {
-    long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
-    float v = 0.0f;
-    if (core[a] && ptr[a + 1] - ptr[a] > k) v = sorted[ptr[a] + k];
-    out[a] = v;
+// Core distance of each core point: the k-th smallest distance of its
+// neighbourhood (k = minPts - 2; 0 for other points and for rows of at most k
+// entries), the element k of the row sorted by distance, found without
+// sorting.  A radix selection over the row's 32-bit order keys (the float's
+// bits with the sign bit flipped when clear and every bit flipped when set)
+// fixes four 8-bit digits from the most significant one; each digit comes
+// from a 256-bin LDS histogram of the row's keys that match the digits so
+// far.  One wavefront per workgroup and per point.
+__device__ __forceinline__ uint32_t raw_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float raw_unkey(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ __launch_bounds__(64) void k_kth_select(long long n, const long long* ptr, const float* dist,
+                                                   const uint8_t* core, int k, float* out) {
+    __shared__ unsigned int h[256];
+    const int lane = (int)threadIdx.x;
+    for (long long a = blockIdx.x; a < n; a += gridDim.x) {
+        const long long b = ptr[a], e = ptr[a + 1];
+        float v = 0.0f;
+        if (core[a] && e - b > k) {
+            uint32_t prefix = 0u;
+            unsigned int kk = (unsigned int)k;  // rank inside the keys matching prefix
+            for (int shift = 24; shift >= 0; shift -= 8) {
+                for (int i = lane; i < 256; i += 64) h[i] = 0u;
+                __syncthreads();
+                const uint32_t hm = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+                for (long long j = b + lane; j < e; j += 64) {
+                    const uint32_t u = raw_key(dist[j]);
+                    if ((u & hm) == prefix) atomicAdd(&h[(u >> shift) & 255u], 1u);
+                }
+                __syncthreads();
+                const unsigned int c0 = h[4 * lane], c1 = h[4 * lane + 1], c2 = h[4 * lane + 2], c3 = h[4 * lane + 3];
+                const unsigned int sum = c0 + c1 + c2 + c3;
+                unsigned int incl = sum;
+#pragma unroll
+                for (int m = 1; m < 64; m <<= 1) {
+                    const unsigned int t = __shfl_up(incl, m, kWave);
+                    if (lane >= m) incl += t;
+                }
+                const unsigned int excl = incl - sum;
+                // exactly one lane's four bins hold rank kk (kk < the matching count)
+                const bool mine = excl <= kk && kk < incl;
+                const int src = __ffsll((long long)__ballot(mine)) - 1;
+                unsigned int bin = 4u * (unsigned int)lane, rest = kk - excl;
+                if (rest >= c0) {
+                    rest -= c0; ++bin;
+                    if (rest >= c1) {
+                        rest -= c1; ++bin;
+                        if (rest >= c2) { rest -= c2; ++bin; }
+                    }
+                }
+                prefix |= (uint32_t)__shfl((int)bin, src, kWave) << shift;
+                kk = (unsigned int)__shfl((int)rest, src, kWave);
+                __syncthreads();  // every lane has read h before the next digit clears it
+            }
+            v = raw_unkey(prefix);
+        }
+        if (lane == 0) out[a] = v;
+    }
 }
 
 // One wavefront per core point i, its lanes striding over i's neighbourhood
@@ -384,9 +440,11 @@ int launch_csr_mirror(long long n, const long long* ptr, const int* idx, float* 
     return (int)hipGetLastError();
 }
 
-int launch_kth(long long n, const long long* ptr, const float* sorted, const uint8_t* core, int k,
+int launch_kth(long long n, const long long* ptr, const float* dist, const uint8_t* core, int k,
                float* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_kth, dim3(nb(n, 256)), dim3(256), 0, st, n, ptr, sorted, core, k, out);
+    if (k < 0) return (int)hipErrorInvalidValue;
+    const long long blocks = std::min<long long>(std::max<long long>(n, 1), 262144);
+    hipLaunchKernelGGL(k_kth_select, dim3((unsigned)blocks), dim3(64), 0, st, n, ptr, dist, core, k, out);
     return (int)hipGetLastError();
 }
 
