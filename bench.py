@@ -212,6 +212,42 @@ def l2_bench(args):
         "recheck": {"queued": st["n_queue"], "fp64": st["n_recheck64"], "band": st["band"]}}), flush=True)
 
 
+def batch_shapes(nds):
+    """The many-dataset workload: point counts log-uniform in [2,000, 500,000],
+    d in {32, 128, 784}, 3-30 mixture components, and 8 eps in [0.1, 0.4]."""
+    rng = np.random.default_rng(1003)
+    ns = np.exp(rng.uniform(np.log(2000), np.log(500000), nds)).astype(np.int64)
+    ds = rng.choice([32, 128, 784], nds)
+    ks = rng.integers(3, 31, nds)
+    eps = np.sort(rng.uniform(0.1, 0.4, 8)).astype(np.float32)
+    return ns, ds, ks, eps
+
+
+def optics_datasets_bench(args):
+    """sOPTICS on one context for the largest dataset of each width of the
+    many-dataset workload (index, core points at the largest eps, orderings
+    and reachability for minPts 10 and 50), one JSON line per dataset with the
+    wall time of sdb_optics_orders (neighbourhoods, their distances, core
+    distances, host walks, reachability)."""
+    import sdbscan_amd as S
+    from sdbscan_amd import datagen
+    ns, ds, ks, eps = batch_shapes(args.batch_datasets)
+    for d in (32, 128, 784):
+        i = max((j for j in range(len(ns)) if ds[j] == d), key=lambda j: ns[j])
+        X = datagen.sphere_mixture(int(ns[i]), int(d), int(ks[i]), seed=2000 + i)[0]
+        ctx = S.Context()
+        ctx.set_params(int(ns[i]), int(d), n_proj=P, topK=10, topM=50)
+        ctx.load(X)
+        ctx.build_index(S.hd3_bits(3 * 1024, 42))
+        ctx.find_core(float(eps[-1]), 10, want=False)
+        t0 = time.perf_counter()
+        order, reach = ctx.optics_orders([10, 50])
+        ms = (time.perf_counter() - t0) * 1e3
+        print(json.dumps({"dataset": int(i), "n": int(ns[i]), "d": int(d), "eps": float(eps[-1]),
+                          "minPts": [10, 50], "optics_orders_ms": round(ms, 1)}), flush=True)
+        ctx.close()
+
+
 def batch_bench(args):
     """Many-dataset sweep: 64 datasets of n log-uniform in [2,000, 500,000]
     and d in {32, 128, 784} (3-30-component Gaussian mixtures on the unit
@@ -224,12 +260,8 @@ def batch_bench(args):
     on a pool of streams (sdb_batch_run, autotuned)."""
     import sdbscan_amd as S
     from sdbscan_amd import datagen
-    rng = np.random.default_rng(1003)
     nds = args.batch_datasets
-    ns = np.exp(rng.uniform(np.log(2000), np.log(500000), nds)).astype(np.int64)
-    ds = rng.choice([32, 128, 784], nds)
-    ks = rng.integers(3, 31, nds)
-    eps = np.sort(rng.uniform(0.1, 0.4, 8)).astype(np.float32)
+    ns, ds, ks, eps = batch_shapes(nds)
     minPts = [10, 50]
     t0 = time.perf_counter()
     data = [datagen.sphere_mixture(int(n), int(d), int(k), seed=2000 + i)[0]
@@ -299,6 +331,8 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="run the many-dataset sweep (sdb_batch_run) instead of the 8M x 784 step")
     ap.add_argument("--batch-datasets", type=int, default=64)
+    ap.add_argument("--optics-datasets", action="store_true",
+                    help="sOPTICS on one context for the largest dataset of each width of the --batch workload")
     ap.add_argument("--streams", type=int, default=8)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 4 GiB device-to-device copy (profiler counter calibration)")
@@ -314,6 +348,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.optics_datasets:
+        optics_datasets_bench(args)
         return
     if args.workload == "l2":
         l2_bench(args)
